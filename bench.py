@@ -1,0 +1,180 @@
+"""Benchmark of the intracule hot path on one MI355X.
+
+Workload (the radial-plot configuration): a seeded 10-atom C/N/O/H cluster
+(bond lengths 1.8-2.9 bohr), 151 Cartesian s/p/d primitives with exponents
+log-uniform in [0.05, 500], an HF-like DM2 from 20 seeded occupied orbitals
+written as symmetry-unique records (about 6.6e7, 1.6 GB), and I(s) for
+s = 0..10 bohr in steps of 0.05 on 302-point Lebedev spheres, thresh 1e-12.
+The records are generated directly in HBM before timing.
+
+One step = one pass of the hot path over the resident records:
+inca_evaluate = decode + normalise + screen 1 + prefactors + polycoef +
+screen 2 + compaction + the quartet x grid-point contraction + reduction.
+`value` is reference-equivalent evals/s of the FP64 path (evals = quartets
+passing both screens x grid points); the mixed-precision path is timed the
+same way and reported beside it with its error against FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_VALU = 78.6e12   # MI355X FP64 vector, FLOP/s
+PEAK_FP32_VALU = 157.3e12  # MI355X FP32 vector (packed), FLOP/s
+THRESH = 1e-12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_per_eval(hist):
+    L = np.arange(len(hist))
+    return float(np.sum(hist * (16 + 2 * L))) / max(1, int(hist.sum()))
+
+
+def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
+    """Oracle (FP64 CPU restatement, OpenMP over quartets, all host cores) on
+    a seeded random subset of the records over the full grid."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(12345)
+    cores = os.cpu_count() or 1
+    prims = sysm.prims()
+    n_total = len(rec_host)
+
+    def run(n):
+        idx = np.sort(rng.choice(n_total, size=n, replace=False))
+        sub = rec_host[idx]
+        ijkl = np.ascontiguousarray(sub[:, :4])
+        d = np.ascontiguousarray(sub[:, 4:6]).view(np.float64).reshape(-1)
+        t0 = time.perf_counter()
+        _, cnt = O.intracule(prims, ijkl, d, grid_r, THRESH, nthreads=cores, dynamic=True)
+        return time.perf_counter() - t0, cnt
+
+    n = 4000
+    t, cnt = run(n)
+    n2 = int(min(n_total, max(n, n * seconds_target / max(t, 1e-3))))
+    t, cnt = run(n2)
+    evals = float(cnt[2]) * grid_r.shape[0]
+    return dict(value=evals / t, unit="evals/s", cores=cores, kind="port",
+                sample="%d seeded random records of %d (%d pass both screens) x %d grid "
+                       "points, FP64 oracle, OpenMP dynamic, %.1f s" % (
+                           n2, n_total, int(cnt[2]), grid_r.shape[0], t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--natoms", type=int, default=10)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        return 2
+
+    import torch
+    from inca_parallel_amd import capi, synth
+
+    t0 = time.perf_counter()
+    sysm = synth.config_cluster(seed=2, natoms=a.natoms)
+    recs = sysm.dm2_unique_torch(device="cuda")
+    torch.cuda.synchronize()
+    Q = recs.shape[0]
+    log("generated %d records (%d primitives) in %.1f s" % (Q, sysm.nprim, time.perf_counter() - t0))
+
+    eng = capi.Engine(0)
+    p = sysm.prims()
+    eng.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    nradi, G = eng.grid_radial_plot([[0.0, 10.0]], [0.05], [302])
+    eng.set_quartets(device_ptr=recs.data_ptr(), n=Q)
+    log("grid: %d radii, %d points" % (nradi, G))
+
+    results = {}
+    for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
+        for _ in range(a.warmup):
+            eng.evaluate(prec, THRESH, want_ivec=False)
+        torch.cuda.synchronize()
+        kern = []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            _, cnt = eng.evaluate(prec, THRESH, want_ivec=False)
+            kern.append(eng.timings())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        kern = np.array(kern)
+        results[name] = dict(step_s=dt, counts=cnt, kern_ms=kern.mean(axis=0),
+                             rintra=eng.radial_values())
+        log("%s: %.3f s/step, kernels (prep, eval, reduce) ms = %s, counts %s" % (
+            name, dt, np.round(kern.mean(axis=0), 2), cnt))
+
+    hist = eng.class_histogram(64)
+    cnt = results["fp64"]["counts"]
+    q2 = int(cnt[2])
+    evals = float(q2) * G
+    W = flops_per_eval(hist)
+    r64 = results["fp64"]
+    rm = results["mixed"]
+    eval_s = r64["kern_ms"][1] * 1e-3
+    achieved = evals * W / eval_s
+    ref = r64["rintra"]
+    mixed_err = float(np.max(np.abs(rm["rintra"] - ref) / np.maximum(np.abs(ref), 1e-300)))
+
+    out = {
+        "metric": "quartet x gridpoint evals/s; time to 1e-8 rel integral, mixed vs FP64 GPU",
+        "value": evals / r64["step_s"],
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r64["step_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded cluster, basis and HF-like DM2 generated in HBM)",
+        "config": {
+            "workload": "radial_plot: 10-atom C/N/O/H cluster, %d s/p/d primitives, "
+                        "%d symmetry-unique DM2 records, s=0..10 step 0.05 x LD0302 "
+                        "(%d points), thresh 1e-12" % (sysm.nprim, Q, G),
+            "records": Q, "quartets_after_screen1": int(cnt[1]),
+            "quartets_computed": q2, "grid_points": G, "flop_per_eval": W,
+        },
+        "roofline": {
+            "bound": "valu", "kernel": "k_eval_fp64",
+            "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_VALU, "traffic": None,
+            "note": "FLOP = 16 + 2*Lsum per eval (FMA = 2), exp not counted; "
+                    "eval kernel %.1f ms per launch" % (eval_s * 1e3),
+        },
+        "mixed": {
+            "value": evals / rm["step_s"], "unit": "evals/s",
+            "speedup_vs_fp64": r64["step_s"] / rm["step_s"],
+            "max_rel_err_rintra_vs_fp64": mixed_err,
+            "eval_kernel_ms": float(rm["kern_ms"][1]),
+        },
+        "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
+                      "reduce": float(r64["kern_ms"][2])},
+    }
+    if not a.no_cpu:
+        rec_host = recs.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(sysm, rec_host, eng.get_grid()["r"])
+    print(json.dumps(out))
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,261 @@
+"""ctypes binding of libinca_hip.so, the C-ABI declared in include/inca_hip.h.
+
+This is the Python face of the same boundary the Fortran host binds with
+ISO_C_BINDING; tests and bench.py drive the engine through it.  Loading
+fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libinca_hip.so")
+
+INCA_FP64 = 0
+INCA_MIXED = 1
+
+# every symbol include/inca_hip.h declares
+SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitives",
+           "inca_grid_multicenter", "inca_grid_radial_plot", "inca_grid_cube",
+           "inca_grid_points", "inca_get_grid", "inca_get_radial_layout",
+           "inca_load_dm2p", "inca_set_quartets", "inca_evaluate", "inca_get_ivec",
+           "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
+           "inca_class_histogram",
+           "inca_lebedev", "inca_gauleg"]
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = C.POINTER(C.c_int64)
+
+
+class IncaError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libinca_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IncaError("libinca_hip.so not found at %s: run "
+                        "`python -m inca_parallel_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.inca_create.argtypes = [C.c_int32, C.POINTER(vp)]
+    L.inca_destroy.argtypes = [vp]
+    L.inca_last_error.argtypes = [vp]
+    L.inca_last_error.restype = C.c_char_p
+    L.inca_set_primitives.argtypes = [vp, C.c_int32, _dp, _ip, _dp, _dp, _dp, C.c_int32]
+    L.inca_grid_multicenter.argtypes = [vp, C.c_int32, _dp, _ip, _ip, _dp, _dp, C.c_int32,
+                                        _i64, _i64, _i64]
+    L.inca_grid_radial_plot.argtypes = [vp, C.c_int32, _dp, _dp, _ip, C.c_int32, _i64, _i64]
+    L.inca_grid_cube.argtypes = [vp, _dp, _dp, _ip, _i64]
+    L.inca_grid_points.argtypes = [vp, C.c_int64, _dp]
+    L.inca_get_grid.argtypes = [vp, vp, vp, vp, vp]
+    L.inca_get_radial_layout.argtypes = [vp, vp, vp]
+    L.inca_load_dm2p.argtypes = [vp, C.c_char_p, _i64, C.POINTER(C.c_int32)]
+    L.inca_set_quartets.argtypes = [vp, C.c_int64, vp, C.c_int32]
+    L.inca_evaluate.argtypes = [vp, C.c_int32, C.c_double, vp, vp]
+    L.inca_get_ivec.argtypes = [vp, vp]
+    L.inca_integrate.argtypes = [vp, vp, vp, vp, vp]
+    L.inca_radial_values.argtypes = [vp, vp]
+    L.inca_smearing.argtypes = [vp, C.c_double, C.c_int32, C.c_double, vp, vp, vp]
+    L.inca_get_timings.argtypes = [vp, _dp]
+    L.inca_class_histogram.argtypes = [vp, C.c_int32, vp]
+    L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
+    L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
+    for s in SYMBOLS:
+        if s != "inca_last_error":
+            getattr(L, s).restype = C.c_int
+    _lib = L
+    return L
+
+
+def lebedev(n):
+    """(n,4) array [x, y, z, w] of the n-point Lebedev-Laikov rule (sum w = 1)."""
+    L = load()
+    x, y, z, w = (np.zeros(n) for _ in range(4))
+    if L.inca_lebedev(int(n), x, y, z, w) != 0:
+        raise IncaError("no Lebedev rule with %d points" % n)
+    return np.stack([x, y, z, w], axis=1)
+
+
+def gauleg(n):
+    L = load()
+    t, w = np.zeros(n), np.zeros(n)
+    if L.inca_gauleg(int(n), t, w) != 0:
+        raise IncaError("bad Gauss-Legendre order %d" % n)
+    return t, w
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Engine:
+    """One device context: primitives, a grid, DM2 records, Ivec."""
+
+    def __init__(self, device: int = 0):
+        self._L = load()
+        self._h = C.c_void_p()
+        rc = self._L.inca_create(int(device), C.byref(self._h))
+        if rc != 0:
+            raise IncaError("inca_create failed (%d): %s" % (
+                rc, self._L.inca_last_error(None).decode()))
+        self.G = 0
+        self.nquad = 0
+        self.nradi = 0
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise IncaError("%s failed (%d): %s" % (
+                what, rc, self._L.inca_last_error(self._h).decode()))
+
+    def close(self):
+        if self._h:
+            self._L.inca_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_primitives(self, alpha, tmn, x, y, z, normalize=True):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        n = len(alpha)
+        tmn_cm = np.ascontiguousarray(np.asarray(tmn, np.int32).reshape(n, 3).T.reshape(-1))
+        self._ck(self._L.inca_set_primitives(
+            self._h, n, alpha, tmn_cm, np.ascontiguousarray(x, np.float64),
+            np.ascontiguousarray(y, np.float64), np.ascontiguousarray(z, np.float64),
+            int(bool(normalize))), "inca_set_primitives")
+
+    def grid_multicenter(self, cent, nradc, nangc, sfalpha, Ps, nosym=False):
+        cent = np.ascontiguousarray(cent, np.float64).reshape(-1)
+        nq = len(nradc)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._ck(self._L.inca_grid_multicenter(
+            self._h, nq, cent, np.ascontiguousarray(nradc, np.int32),
+            np.ascontiguousarray(nangc, np.int32), np.ascontiguousarray(sfalpha, np.float64),
+            np.ascontiguousarray(Ps, np.float64), int(bool(nosym)),
+            C.byref(a), C.byref(b), C.byref(c)), "inca_grid_multicenter")
+        self.G, self.nquad = c.value, nq
+        return a.value, b.value, c.value
+
+    def grid_radial_plot(self, tart, step, nang, nosym=False):
+        tart = np.ascontiguousarray(tart, np.float64).reshape(-1)
+        nr, ng = C.c_int64(), C.c_int64()
+        self._ck(self._L.inca_grid_radial_plot(
+            self._h, len(step), tart, np.ascontiguousarray(step, np.float64),
+            np.ascontiguousarray(nang, np.int32), int(bool(nosym)),
+            C.byref(nr), C.byref(ng)), "inca_grid_radial_plot")
+        self.G, self.nradi = ng.value, nr.value
+        return nr.value, ng.value
+
+    def grid_cube(self, center, step, np_i):
+        n = C.c_int64()
+        self._ck(self._L.inca_grid_cube(
+            self._h, np.ascontiguousarray(center, np.float64),
+            np.ascontiguousarray(step, np.float64), np.ascontiguousarray(np_i, np.int32),
+            C.byref(n)), "inca_grid_cube")
+        self.G = n.value
+        return n.value
+
+    def grid_points(self, r):
+        r = np.ascontiguousarray(r, np.float64).reshape(-1)
+        self._ck(self._L.inca_grid_points(self._h, r.size // 3, r), "inca_grid_points")
+        self.G = r.size // 3
+
+    def get_grid(self):
+        G = self.G
+        r = np.zeros((G, 3))
+        w, wv = np.zeros(G), np.zeros(G)
+        mp = np.zeros(G, np.int32)
+        self._ck(self._L.inca_get_grid(self._h, _ptr(r), _ptr(w), _ptr(wv), _ptr(mp)),
+                 "inca_get_grid")
+        return dict(r=r, w=w, w_vee=wv, map=mp)
+
+    def radial_layout(self):
+        radi = np.zeros(self.nradi)
+        smn = np.zeros(self.nradi, np.int32)
+        self._ck(self._L.inca_get_radial_layout(self._h, _ptr(radi), _ptr(smn)),
+                 "inca_get_radial_layout")
+        return radi, smn
+
+    def load_dm2p(self, path):
+        n = C.c_int64()
+        fmt = C.c_int32()
+        self._ck(self._L.inca_load_dm2p(self._h, os.fsencode(path), C.byref(n), C.byref(fmt)),
+                 "inca_load_dm2p")
+        return n.value, fmt.value
+
+    def set_quartets(self, records=None, ijkl=None, d=None, device_ptr=None, n=None):
+        """records: structured/raw 24-byte records; or ijkl (n,4) int32 + d (n,);
+        or device_ptr (int) + n for records already in device memory."""
+        if device_ptr is not None:
+            self._ck(self._L.inca_set_quartets(self._h, int(n), C.c_void_p(int(device_ptr)), 1),
+                     "inca_set_quartets")
+            return
+        if records is None:
+            ijkl = np.asarray(ijkl, np.int32).reshape(-1, 4)
+            records = np.empty(len(ijkl), dtype=[("ijkl", "<i4", 4), ("d", "<f8")])
+            records["ijkl"] = ijkl
+            records["d"] = d
+        records = np.ascontiguousarray(records)
+        nrec = records.nbytes // 24
+        self._ck(self._L.inca_set_quartets(self._h, nrec, records.ctypes.data, 0),
+                 "inca_set_quartets")
+
+    def evaluate(self, precision=INCA_FP64, thresh=1e-12, want_ivec=True):
+        iv = np.zeros(max(self.G, 1)) if want_ivec else None
+        cnt = np.zeros(4, np.int64)
+        self._ck(self._L.inca_evaluate(self._h, int(precision), float(thresh), _ptr(iv),
+                                       cnt.ctypes.data), "inca_evaluate")
+        return (iv[:self.G] if want_ivec else None), cnt
+
+    def ivec(self):
+        iv = np.zeros(max(self.G, 1))
+        self._ck(self._L.inca_get_ivec(self._h, iv.ctypes.data), "inca_get_ivec")
+        return iv[:self.G]
+
+    def integrate(self):
+        ti, tv = C.c_double(), C.c_double()
+        ipc, vpc = np.zeros(self.nquad), np.zeros(self.nquad)
+        self._ck(self._L.inca_integrate(self._h, C.addressof(ti), C.addressof(tv),
+                                        ipc.ctypes.data, vpc.ctypes.data), "inca_integrate")
+        return ti.value, tv.value, ipc, vpc
+
+    def radial_values(self):
+        out = np.zeros(max(self.nradi, 1))
+        self._ck(self._L.inca_radial_values(self._h, out.ctypes.data), "inca_radial_values")
+        return out[:self.nradi]
+
+    def smearing(self, u_max, n_bins=1000, sigma=0.15):
+        u, P, Ia = np.zeros(n_bins), np.zeros(n_bins), np.zeros(n_bins)
+        self._ck(self._L.inca_smearing(self._h, float(u_max), int(n_bins), float(sigma),
+                                       u.ctypes.data, P.ctypes.data, Ia.ctypes.data),
+                 "inca_smearing")
+        return u, P, Ia
+
+    def class_histogram(self, nbins=64):
+        h = np.zeros(nbins, np.int64)
+        self._ck(self._L.inca_class_histogram(self._h, int(nbins), h.ctypes.data),
+                 "inca_class_histogram")
+        return h
+
+    def timings(self):
+        t = np.zeros(3)
+        self._ck(self._L.inca_get_timings(self._h, t), "inca_get_timings")
+        return t

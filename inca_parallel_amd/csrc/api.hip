@@ -1,0 +1,391 @@
+// extern "C" entry points of libinca_hip.so (declared in include/inca_hip.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace inca;
+
+namespace {
+std::string g_create_err;
+
+#define CHECK_CTX(c) \
+  do {               \
+    if (!(c)) return INCA_EINVAL; \
+  } while (0)
+
+// normalisation of a primitive (mod_intracule.f90:148-150)
+double normprim(double alpha, int t, int m, int n) {
+  auto dfact = [](int a) {  // mod_numbers.f90:14-27
+    int d = 1;
+    if (a % 2 == 0)
+      for (int i = 1; i <= a / 2; ++i) d *= 2 * i;
+    else
+      for (int i = 1; i <= (a + 1) / 2; ++i) d *= 2 * i - 1;
+    return d;
+  };
+  const double pi = 4.0 * std::atan(1.0);
+  double df = (double)(dfact(2 * t - 1) * dfact(2 * m - 1) * dfact(2 * n - 1));
+  return std::pow(2.0 * alpha / pi, 0.75) *
+         std::sqrt(std::pow(4.0 * alpha, (double)(t + m + n)) * (1.0 / df));
+}
+}  // namespace
+
+extern "C" {
+
+int inca_create(int32_t device, inca_ctx **out) {
+  if (!out) return INCA_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    g_create_err = std::string("no HIP device available: ") + hipGetErrorString(e);
+    return INCA_EHIP;
+  }
+  if (device < 0 || device >= n) {
+    g_create_err = "device index out of range";
+    return INCA_EINVAL;
+  }
+  if ((e = hipSetDevice(device)) != hipSuccess) {
+    g_create_err = std::string("hipSetDevice failed: ") + hipGetErrorString(e);
+    return INCA_EHIP;
+  }
+  inca_ctx *c = new inca_ctx();
+  c->device = device;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc((void **)&c->d_cnt, 4 * sizeof(unsigned long long))) != hipSuccess) {
+    g_create_err = std::string("context set-up failed: ") + hipGetErrorString(e);
+    delete c;
+    return INCA_EHIP;
+  }
+  for (auto &ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) {
+      g_create_err = "hipEventCreate failed";
+      delete c;
+      return INCA_EHIP;
+    }
+  *out = c;
+  return 0;
+}
+
+int inca_destroy(inca_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *bufs[] = {c->d_prim, c->d_tmn, c->d_pair, c->d_gx, c->d_gy, c->d_gz, c->d_w,
+                  c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
+                  c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char *inca_last_error(const inca_ctx *c) {
+  return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+int inca_set_primitives(inca_ctx *ctx, int32_t nprim, const double *alpha, const int32_t *tmn,
+                        const double *x, const double *y, const double *z, int32_t normalize) {
+  CHECK_CTX(ctx);
+  if (nprim < 1 || !alpha || !tmn || !x || !y || !z)
+    return set_err(ctx, INCA_EINVAL, "inca_set_primitives: bad arguments");
+  INCA_HIP(hipSetDevice(ctx->device));
+  int tmax = 0;
+  for (int64_t i = 0; i < 3LL * nprim; ++i) {
+    if (tmn[i] < 0) return set_err(ctx, INCA_EINVAL, "negative angular exponent");
+    if (tmn[i] > tmax) tmax = tmn[i];
+  }
+  if (4 * tmax > kLmaxAxis)
+    return set_err(ctx, INCA_ERANGE,
+                   "per-axis angular momentum 4*%d exceeds the Gauss-Hermite tables (L <= 21)",
+                   tmax);
+  for (int i = 0; i < nprim; ++i)
+    if (!(alpha[i] > 0.0)) return set_err(ctx, INCA_EINVAL, "non-positive exponent %d", i + 1);
+  std::vector<double> prim(5 * (size_t)nprim);
+  for (int i = 0; i < nprim; ++i) {
+    prim[i] = alpha[i];
+    prim[nprim + i] = x[i];
+    prim[2 * nprim + i] = y[i];
+    prim[3 * nprim + i] = z[i];
+    prim[4 * nprim + i] = normprim(alpha[i], tmn[i], tmn[nprim + i], tmn[2 * nprim + i]);
+  }
+  if (ctx->d_prim) (void)hipFree(ctx->d_prim);
+  if (ctx->d_tmn) (void)hipFree(ctx->d_tmn);
+  ctx->d_prim = nullptr;
+  ctx->d_tmn = nullptr;
+  INCA_HIP(hipMalloc((void **)&ctx->d_prim, prim.size() * sizeof(double)));
+  INCA_HIP(hipMalloc((void **)&ctx->d_tmn, 3 * (size_t)nprim * sizeof(int32_t)));
+  INCA_HIP(hipMemcpyAsync(ctx->d_prim, prim.data(), prim.size() * sizeof(double),
+                          hipMemcpyHostToDevice, ctx->stream));
+  INCA_HIP(hipMemcpyAsync(ctx->d_tmn, tmn, 3 * (size_t)nprim * sizeof(int32_t),
+                          hipMemcpyHostToDevice, ctx->stream));
+  ctx->nprim = nprim;
+  ctx->normalize = normalize ? 1 : 0;
+  ctx->lmax_axis = 4 * tmax;
+  int rc = build_pair_table(ctx);
+  if (rc) return rc;
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int inca_grid_multicenter(inca_ctx *ctx, int32_t nquad, const double *cent,
+                          const int32_t *nradc, const int32_t *nangc, const double *sfalpha,
+                          const double *Ps, int32_t nosym, int64_t *maxgrid, int64_t *rgrid,
+                          int64_t *rrgrid) {
+  CHECK_CTX(ctx);
+  if (!cent || !nradc || !nangc || !sfalpha || !Ps)
+    return set_err(ctx, INCA_EINVAL, "inca_grid_multicenter: null argument");
+  INCA_HIP(hipSetDevice(ctx->device));
+  int64_t a = 0, b = 0, c = 0;
+  int rc = grid_multicenter(ctx, nquad, cent, nradc, nangc, sfalpha, Ps, nosym, &a, &b, &c);
+  if (maxgrid) *maxgrid = a;
+  if (rgrid) *rgrid = b;
+  if (rrgrid) *rrgrid = c;
+  return rc;
+}
+
+int inca_grid_radial_plot(inca_ctx *ctx, int32_t nblock, const double *tart, const double *step,
+                          const int32_t *nang, int32_t nosym, int64_t *nradi, int64_t *ngrid) {
+  CHECK_CTX(ctx);
+  if (!tart || !step || !nang) return set_err(ctx, INCA_EINVAL, "null argument");
+  INCA_HIP(hipSetDevice(ctx->device));
+  int64_t a = 0, b = 0;
+  int rc = grid_radial_plot(ctx, nblock, tart, step, nang, nosym, &a, &b);
+  if (nradi) *nradi = a;
+  if (ngrid) *ngrid = b;
+  return rc;
+}
+
+int inca_grid_cube(inca_ctx *ctx, const double *center, const double *step, const int32_t *np,
+                   int64_t *ngrid) {
+  CHECK_CTX(ctx);
+  if (!center || !step || !np) return set_err(ctx, INCA_EINVAL, "null argument");
+  INCA_HIP(hipSetDevice(ctx->device));
+  int64_t n = 0;
+  int rc = grid_cube(ctx, center, step, np, &n);
+  if (ngrid) *ngrid = n;
+  return rc;
+}
+
+int inca_grid_points(inca_ctx *ctx, int64_t n, const double *r) {
+  CHECK_CTX(ctx);
+  if (n > 0 && !r) return set_err(ctx, INCA_EINVAL, "null points");
+  INCA_HIP(hipSetDevice(ctx->device));
+  return grid_points(ctx, n, r);
+}
+
+int inca_get_grid(inca_ctx *ctx, double *r, double *w, double *w_vee, int32_t *center_map) {
+  CHECK_CTX(ctx);
+  if (ctx->grid_kind == GRID_NONE) return set_err(ctx, INCA_EINVAL, "no grid");
+  const int64_t G = ctx->G;
+  if (G == 0) return 0;
+  hipStream_t st = ctx->stream;
+  if (r) {
+    std::vector<double> x(G), y(G), z(G);
+    INCA_HIP(hipMemcpyAsync(x.data(), ctx->d_gx, G * sizeof(double), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipMemcpyAsync(y.data(), ctx->d_gy, G * sizeof(double), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipMemcpyAsync(z.data(), ctx->d_gz, G * sizeof(double), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipStreamSynchronize(st));
+    for (int64_t g = 0; g < G; ++g) {
+      r[3 * g] = x[g];
+      r[3 * g + 1] = y[g];
+      r[3 * g + 2] = z[g];
+    }
+  }
+  if (w) INCA_HIP(hipMemcpyAsync(w, ctx->d_w, G * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (w_vee)
+    INCA_HIP(hipMemcpyAsync(w_vee, ctx->d_wvee, G * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (center_map)
+    INCA_HIP(hipMemcpyAsync(center_map, ctx->d_map, G * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int inca_get_radial_layout(inca_ctx *ctx, double *radi, int32_t *smn) {
+  CHECK_CTX(ctx);
+  if (ctx->grid_kind != GRID_RADIAL) return set_err(ctx, INCA_EINVAL, "no radial-scan grid");
+  for (size_t i = 0; i < ctx->radi.size(); ++i) {
+    if (radi) radi[i] = ctx->radi[i];
+    if (smn) smn[i] = ctx->smn[i];
+  }
+  return 0;
+}
+
+int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec, int32_t *format) {
+  CHECK_CTX(ctx);
+  if (!path) return set_err(ctx, INCA_EINVAL, "null path");
+  INCA_HIP(hipSetDevice(ctx->device));
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return set_err(ctx, INCA_EIO, "cannot open %s", path);
+  std::fseek(f, 0, SEEK_END);
+  const long long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int32_t head[2] = {0, 0};
+  if (size >= 8) {
+    if (std::fread(head, 4, 2, f) != 2) {
+      std::fclose(f);
+      return set_err(ctx, INCA_EIO, "cannot read %s", path);
+    }
+    std::fseek(f, 0, SEEK_SET);
+  }
+  // mod_intracule.f90:167-184
+  int fmt = 0;
+  if (head[0] == 24 && head[1] == 0) fmt = 3;
+  else if (head[0] == 24) fmt = 1;
+  else if (head[0] == 0 && size >= 8) fmt = 2;
+  if (fmt == 3 || fmt == 2) {
+    std::fclose(f);
+    return set_err(ctx, INCA_EFORMAT,
+                   "Unsupported file format with markers. Please convert to stream format.");
+  }
+  const int64_t cap = fmt == 1 ? size / 32 : size / 24;
+  if (ensure(ctx, ctx->d_rec, ctx->rec_cap, (size_t)(cap > 0 ? cap : 1) * 24)) {
+    std::fclose(f);
+    return INCA_ENOMEM;
+  }
+  const size_t chunk_rec = 1 << 21;  // 2M records = 48 MiB per staging copy
+  std::vector<char> buf(chunk_rec * 32), pay(chunk_rec * 24);
+  int64_t got = 0;
+  bool stop = false;
+  while (!stop && got < cap) {
+    size_t want = (size_t)std::min<int64_t>(chunk_rec, cap - got);
+    size_t rs = fmt == 1 ? 32 : 24;
+    size_t n = std::fread(buf.data(), rs, want, f);
+    if (n < want) stop = true;
+    size_t keep = n;
+    if (fmt == 1) {
+      keep = 0;
+      for (size_t r = 0; r < n; ++r) {
+        int32_t m0, m1;
+        std::memcpy(&m0, buf.data() + 32 * r, 4);
+        std::memcpy(&m1, buf.data() + 32 * r + 28, 4);
+        if (m0 != 24 || m1 != 24) {  // a record that is not 24 bytes ends the read
+          stop = true;
+          break;
+        }
+        std::memcpy(pay.data() + 24 * keep, buf.data() + 32 * r + 4, 24);
+        ++keep;
+      }
+    } else {
+      std::memcpy(pay.data(), buf.data(), 24 * n);
+    }
+    if (keep) {
+      hipError_t e = hipMemcpy((char *)ctx->d_rec + 24 * got, pay.data(), 24 * keep,
+                               hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        std::fclose(f);
+        return hip_fail(ctx, e, "hipMemcpy(.dm2p)", __LINE__);
+      }
+    }
+    got += (int64_t)keep;
+  }
+  std::fclose(f);
+  ctx->nrec = got;
+  if (nrec) *nrec = got;
+  if (format) *format = fmt;
+  return 0;
+}
+
+int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records, int32_t on_device) {
+  CHECK_CTX(ctx);
+  if (n < 0 || (n > 0 && !records)) return set_err(ctx, INCA_EINVAL, "bad record array");
+  INCA_HIP(hipSetDevice(ctx->device));
+  if (int rc = ensure(ctx, ctx->d_rec, ctx->rec_cap, (size_t)(n > 0 ? n : 1) * 24))
+    return rc;
+  if (n > 0)
+    INCA_HIP(hipMemcpyAsync(ctx->d_rec, records, 24 * (size_t)n,
+                            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->nrec = n;
+  return 0;
+}
+
+int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh, double *ivec,
+                  int64_t *counts) {
+  CHECK_CTX(ctx);
+  if (ctx->nprim == 0) return set_err(ctx, INCA_EINVAL, "primitives not set");
+  if (ctx->grid_kind == GRID_NONE) return set_err(ctx, INCA_EINVAL, "grid not set");
+  if (!ctx->d_rec) return set_err(ctx, INCA_EINVAL, "DM2 records not set");
+  if (precision != INCA_FP64 && precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "unknown precision %d", precision);
+  INCA_HIP(hipSetDevice(ctx->device));
+  int64_t cnt[4] = {0, 0, 0, 0};
+  INCA_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  int rc = prepare_quartets(ctx, thresh, precision, cnt);
+  if (rc) return rc;
+  if ((rc = evaluate_grid(ctx, precision))) return rc;
+  INCA_HIP(hipEventSynchronize(ctx->ev[3]));
+  float a = 0, b = 0, c = 0;
+  INCA_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+  INCA_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+  INCA_HIP(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+  ctx->t_ms[0] = a;
+  ctx->t_ms[1] = b;
+  ctx->t_ms[2] = c;
+  if (counts)
+    for (int i = 0; i < 4; ++i) counts[i] = cnt[i];
+  if (ivec && ctx->G > 0)
+    INCA_HIP(hipMemcpy(ivec, ctx->d_ivec, ctx->G * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int inca_get_ivec(inca_ctx *ctx, double *ivec) {
+  CHECK_CTX(ctx);
+  if (!ctx->have_ivec) return set_err(ctx, INCA_EINVAL, "no Ivec: call inca_evaluate first");
+  if (ctx->G > 0)
+    INCA_HIP(hipMemcpy(ivec, ctx->d_ivec, ctx->G * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int inca_integrate(inca_ctx *ctx, double *integral, double *vee, double *ipc, double *vpc) {
+  CHECK_CTX(ctx);
+  INCA_HIP(hipSetDevice(ctx->device));
+  return grid_integrate(ctx, integral, vee, ipc, vpc);
+}
+
+int inca_radial_values(inca_ctx *ctx, double *rintra) {
+  CHECK_CTX(ctx);
+  INCA_HIP(hipSetDevice(ctx->device));
+  return grid_radial_values(ctx, rintra);
+}
+
+int inca_smearing(inca_ctx *ctx, double u_max, int32_t n_bins, double sigma, double *u,
+                  double *P, double *Iavg) {
+  CHECK_CTX(ctx);
+  if (!u || !P || !Iavg) return set_err(ctx, INCA_EINVAL, "null output");
+  INCA_HIP(hipSetDevice(ctx->device));
+  return grid_smearing(ctx, u_max, n_bins, sigma, u, P, Iavg);
+}
+
+int inca_class_histogram(inca_ctx *ctx, int32_t nbins, int64_t *hist) {
+  CHECK_CTX(ctx);
+  if (nbins < 1 || !hist) return set_err(ctx, INCA_EINVAL, "bad histogram arguments");
+  INCA_HIP(hipSetDevice(ctx->device));
+  return class_histogram(ctx, nbins, hist);
+}
+
+int inca_get_timings(inca_ctx *ctx, double *t) {
+  CHECK_CTX(ctx);
+  for (int i = 0; i < 3; ++i) t[i] = ctx->t_ms[i];
+  return 0;
+}
+
+int inca_lebedev(int32_t n, double *x, double *y, double *z, double *w) {
+  if (!x || !y || !z || !w) return INCA_EINVAL;
+  return lebedev(n, x, y, z, w) ? INCA_ERANGE : 0;
+}
+
+int inca_gauleg(int32_t n, double *t, double *w) {
+  if (n < 1 || !t || !w) return INCA_EINVAL;
+  gauleg(n, t, w);
+  return 0;
+}
+
+}  // extern "C"

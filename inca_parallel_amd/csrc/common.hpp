@@ -1,0 +1,160 @@
+// Internal declarations shared by the libinca_hip.so translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/inca_hip.h"
+
+namespace inca {
+
+// Largest per-axis polynomial degree the Gauss-Hermite tables cover
+// (mod_intrastuff.f90:11-15: nodes for L <= 21).
+constexpr int kLmaxAxis = 21;
+constexpr int kNpMax = kLmaxAxis + 1;
+
+enum GridKind { GRID_NONE = 0, GRID_MULTI = 1, GRID_RADIAL = 2, GRID_CUBE = 3, GRID_POINTS = 4 };
+
+// Per primitive pair (a,b), the quantities the reference recomputes per
+// record: JA8 bound, Gaussian-product exponent and centre
+// (mod_intracule.f90:297-311).
+struct PairEntry {
+  double J;     // JA8(a as i, b as k)
+  double aik;   // alpha_a + alpha_b
+  double eR2;   // (alpha_a alpha_b / aik) |R_a - R_b|^2
+  double P[3];  // Gaussian-product centre
+  double pad[2];
+};
+
+struct DeviceBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace inca
+
+struct inca_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // primitives
+  int nprim = 0;
+  int normalize = 1;
+  int lmax_axis = 0;  // 4 * max(t) over primitives
+  double *d_prim = nullptr;  // SoA: alpha | x | y | z | norm  (5*nprim)
+  int32_t *d_tmn = nullptr;  // axis-major t: tmn[ax*nprim + i]
+  inca::PairEntry *d_pair = nullptr;
+
+  // grid
+  int grid_kind = inca::GRID_NONE;
+  int64_t G = 0;
+  double *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;  // SoA points
+  double *d_w = nullptr, *d_wvee = nullptr;
+  int32_t *d_map = nullptr;       // 1-based centre per point
+  int nquad = 0;
+  std::vector<int64_t> centre_first;  // first point of each centre (+ end)
+  std::vector<double> radi;           // radial scan radii
+  std::vector<int32_t> smn;           // kept points per radius
+
+  // DM2 records (24 B each)
+  int64_t nrec = 0;
+  char *d_rec = nullptr;
+  size_t rec_cap = 0;
+
+  // prepared quartets
+  int qstride = 0;       // doubles per prepared record
+  int64_t q2 = 0;
+  double *d_q = nullptr;      // compacted prepared records
+  size_t q_cap = 0;
+  double *d_qtmp = nullptr;   // uncompacted records (slot = record index)
+  size_t qtmp_cap = 0;
+  float *d_qf = nullptr;      // FP32 copy for the mixed kernel
+  size_t qf_cap = 0;
+
+  // scratch
+  uint8_t *d_flag = nullptr;
+  size_t flag_cap = 0;
+  int64_t *d_idx = nullptr;
+  size_t idx_cap = 0;
+  int64_t *d_bc = nullptr;
+  size_t bc_cap = 0;
+  unsigned long long *d_cnt = nullptr;  // device counters
+  double *d_partial = nullptr;
+  size_t partial_cap = 0;
+  double *d_ivec = nullptr;
+  size_t ivec_cap = 0;
+  int have_ivec = 0;
+
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double t_ms[3] = {0, 0, 0};
+};
+
+namespace inca {
+
+int set_err(inca_ctx *c, int code, const char *fmt, ...);
+int hip_fail(inca_ctx *c, hipError_t e, const char *what, int line);
+
+#define INCA_HIP(call)                                              \
+  do {                                                              \
+    hipError_t e_ = (call);                                         \
+    if (e_ != hipSuccess) return ::inca::hip_fail(ctx, e_, #call, __LINE__); \
+  } while (0)
+
+// grow-only device buffer
+template <class T>
+int ensure(inca_ctx *ctx, T *&ptr, size_t &cap, size_t n) {
+  if (n <= cap && ptr) return 0;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  size_t bytes = (n ? n : 1) * sizeof(T);
+  hipError_t e = hipMalloc((void **)&ptr, bytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return set_err(ctx, INCA_ENOMEM, "hipMalloc of %zu bytes failed: %s", bytes,
+                   hipGetErrorString(e));
+  }
+  cap = n;
+  return 0;
+}
+
+// Deterministic stream compaction of a 0/1 byte flag array:
+// idx[0..total) = positions of set flags, in increasing order.
+int compact_flags(inca_ctx *ctx, const uint8_t *d_flag, int64_t n,
+                  int64_t *d_idx, int64_t *total);
+
+// host-side tables
+void gauleg(int n, double *t, double *w);
+int lebedev(int n, double *x, double *y, double *z, double *w);
+
+// launchers (grid.hip)
+int grid_multicenter(inca_ctx *ctx, int nquad, const double *cent,
+                     const int32_t *nradc, const int32_t *nangc,
+                     const double *sfalpha, const double *Ps, int nosym,
+                     int64_t *maxgrid, int64_t *rgrid, int64_t *rrgrid);
+int grid_radial_plot(inca_ctx *ctx, int nblock, const double *tart,
+                     const double *step, const int32_t *nang, int nosym,
+                     int64_t *nradi, int64_t *ngrid);
+int grid_cube(inca_ctx *ctx, const double *center, const double *step,
+              const int32_t *np, int64_t *ngrid);
+int grid_points(inca_ctx *ctx, int64_t n, const double *r);
+int grid_integrate(inca_ctx *ctx, double *integral, double *vee,
+                   double *ipc, double *vpc);
+int grid_radial_values(inca_ctx *ctx, double *rintra);
+int grid_smearing(inca_ctx *ctx, double u_max, int n_bins, double sigma,
+                  double *u, double *P, double *Iavg);
+
+// launchers (quartet.hip)
+int build_pair_table(inca_ctx *ctx);
+int prepare_quartets(inca_ctx *ctx, double thresh, int precision,
+                     int64_t *counts);
+
+// launchers (eval.hip)
+int evaluate_grid(inca_ctx *ctx, int precision);
+int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist);
+
+}  // namespace inca

@@ -1,0 +1,321 @@
+// The hot grid loop (mod_intracule.f90:345-358) on the device:
+//   Ivec(g) += A exp(-|rp|^2) Vx(rp_x) Vy(rp_y) Vz(rp_z),  rp = sqe (r_g + P_ik - P_jl)
+// summed over every prepared quartet.
+//
+// Mapping.  A block of 256 lanes owns a tile of 256*P grid points (P points
+// per lane in registers) and one chunk of the quartet list.  Every lane
+// walks the same quartets, so a quartet's data (sqe, sqe*d, class, the
+// A-scaled coefficients) is wave-uniform: it comes in through scalar loads
+// and sits in SGPRs, there is no divergence whatever the angular-momentum
+// class, and the Horner loops have uniform trip counts.  Each (tile, chunk)
+// writes its sums to partial[chunk][g]; k_reduce adds the chunks in a fixed
+// order, so results are bitwise reproducible.
+//
+// Arithmetic.  This is exp-bound VALU work (no MFMA: nothing here is a
+// matrix product).  FP64 path: rp by one FMA per axis, exp(-s) by
+// Cody-Waite reduction to r = -s + k ln2 (|r| <= ln2/2), the degree-12
+// Taylor polynomial of e^r (truncation error 0.3466^13/13! < 2e-16), and an
+// exponent shift.  Mixed path: rp in FP64 (the cancellation-prone step),
+// everything after it in FP32 -- hardware exp2, FP32 Horner -- with a
+// compensated (Kahan) FP32 sum over tiles of quartets that is folded into
+// an FP64 accumulator at the end of every tile.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace inca {
+namespace {
+
+constexpr int kT = 256;
+
+__device__ __forceinline__ double exp_neg(double s) {
+  // exp(-s) for s >= 0 (s <= 1000 clamp keeps k inside int range; e^-1000 = 0)
+  s = fmin(s, 1000.0);
+  const double kLog2e = 1.4426950408889634;
+  const double kLn2Hi = 6.93147180369123816490e-01;  // Cody-Waite split of ln 2
+  const double kLn2Lo = 1.90821492927058770002e-10;
+  const double k = rint(s * kLog2e);
+  double r = fma(k, kLn2Hi, -s);
+  r = fma(k, kLn2Lo, r);
+  // Taylor coefficients 1/n!, n = 12 .. 0
+  double p = 2.08767569878680989792e-09;
+  p = fma(p, r, 2.50521083854417187751e-08);
+  p = fma(p, r, 2.75573192239858906526e-07);
+  p = fma(p, r, 2.75573192239858906526e-06);
+  p = fma(p, r, 2.48015873015873015873e-05);
+  p = fma(p, r, 1.98412698412698412698e-04);
+  p = fma(p, r, 1.38888888888888888889e-03);
+  p = fma(p, r, 8.33333333333333333333e-03);
+  p = fma(p, r, 4.16666666666666666667e-02);
+  p = fma(p, r, 1.66666666666666666667e-01);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, -(int)k);
+}
+
+struct QHeader {
+  double sqe, ax, ay, az;
+  int Lx, Ly, Lz;
+};
+
+__device__ __forceinline__ QHeader load_header(const double *__restrict__ rec) {
+  QHeader h;
+  h.sqe = rec[0];
+  h.ax = rec[1];
+  h.ay = rec[2];
+  h.az = rec[3];
+  const unsigned cls = (unsigned)__double_as_longlong(rec[4]);
+  h.Lx = cls & 0xff;
+  h.Ly = (cls >> 8) & 0xff;
+  h.Lz = (cls >> 16) & 0xff;
+  return h;
+}
+
+template <int P>
+__global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
+                                                  int64_t nq, int64_t qchunk,
+                                                  const double *__restrict__ gx,
+                                                  const double *__restrict__ gy,
+                                                  const double *__restrict__ gz, int64_t G,
+                                                  double *__restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * (kT * P) + threadIdx.x;
+  double x[P], y[P], z[P], acc[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * kT;
+    const bool ok = g < G;
+    x[k] = ok ? gx[g] : 0.0;
+    y[k] = ok ? gy[g] : 0.0;
+    z[k] = ok ? gz[g] : 0.0;
+    acc[k] = 0.0;
+  }
+  const int64_t q0 = (int64_t)blockIdx.y * qchunk;
+  const int64_t q1 = q0 + qchunk < nq ? q0 + qchunk : nq;
+  for (int64_t qi = q0; qi < q1; ++qi) {
+    const double *__restrict__ rec = q + qi * qstride;
+    const QHeader h = load_header(rec);
+    const double *__restrict__ cx = rec + 5;
+    const double *__restrict__ cy = cx + h.Lx + 1;
+    const double *__restrict__ cz = cy + h.Ly + 1;
+    double px[P], py[P], pz[P], vx[P], vy[P], vz[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      px[k] = fma(h.sqe, x[k], h.ax);
+      py[k] = fma(h.sqe, y[k], h.ay);
+      pz[k] = fma(h.sqe, z[k], h.az);
+      vx[k] = cx[0];
+      vy[k] = cy[0];
+      vz[k] = cz[0];
+    }
+    for (int p = 1; p <= h.Lx; ++p) {
+      const double c = cx[p];
+#pragma unroll
+      for (int k = 0; k < P; ++k) vx[k] = fma(vx[k], px[k], c);
+    }
+    for (int p = 1; p <= h.Ly; ++p) {
+      const double c = cy[p];
+#pragma unroll
+      for (int k = 0; k < P; ++k) vy[k] = fma(vy[k], py[k], c);
+    }
+    for (int p = 1; p <= h.Lz; ++p) {
+      const double c = cz[p];
+#pragma unroll
+      for (int k = 0; k < P; ++k) vz[k] = fma(vz[k], pz[k], c);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double s = fma(px[k], px[k], fma(py[k], py[k], pz[k] * pz[k]));
+      acc[k] = fma(exp_neg(s) * vx[k] * vy[k], vz[k], acc[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * kT;
+    if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
+  }
+}
+
+// Mixed precision: FP64 rp, FP32 exp/polynomials, Kahan FP32 per tile of
+// kMixTile quartets folded into FP64.
+constexpr int kMixTile = 128;
+
+template <int P>
+__global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q, int qstride,
+                                                   int64_t nq, int64_t qchunk,
+                                                   const double *__restrict__ gx,
+                                                   const double *__restrict__ gy,
+                                                   const double *__restrict__ gz, int64_t G,
+                                                   double *__restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * (kT * P) + threadIdx.x;
+  double x[P], y[P], z[P], acc[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * kT;
+    const bool ok = g < G;
+    x[k] = ok ? gx[g] : 0.0;
+    y[k] = ok ? gy[g] : 0.0;
+    z[k] = ok ? gz[g] : 0.0;
+    acc[k] = 0.0;
+  }
+  const int64_t q0 = (int64_t)blockIdx.y * qchunk;
+  const int64_t q1 = q0 + qchunk < nq ? q0 + qchunk : nq;
+  // -log2(e) split so that s*log2(e) is formed to FP32 rounding without the
+  // 1.3e-8 relative bias of a single-float constant
+  const float kNL2eHi = -1.44269502162933349609375f;
+  const float kNL2eLo = -1.925963033500011e-08f;
+  for (int64_t t0 = q0; t0 < q1; t0 += kMixTile) {
+    const int64_t t1 = t0 + kMixTile < q1 ? t0 + kMixTile : q1;
+    float sum[P], comp[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      sum[k] = 0.0f;
+      comp[k] = 0.0f;
+    }
+    for (int64_t qi = t0; qi < t1; ++qi) {
+      const double *__restrict__ rec = q + qi * qstride;
+      const QHeader h = load_header(rec);
+      const double *__restrict__ cx = rec + 5;
+      const double *__restrict__ cy = cx + h.Lx + 1;
+      const double *__restrict__ cz = cy + h.Ly + 1;
+      float px[P], py[P], pz[P], vx[P], vy[P], vz[P];
+      const float c0x = (float)cx[0], c0y = (float)cy[0], c0z = (float)cz[0];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        px[k] = (float)fma(h.sqe, x[k], h.ax);
+        py[k] = (float)fma(h.sqe, y[k], h.ay);
+        pz[k] = (float)fma(h.sqe, z[k], h.az);
+        vx[k] = c0x;
+        vy[k] = c0y;
+        vz[k] = c0z;
+      }
+      for (int p = 1; p <= h.Lx; ++p) {
+        const float c = (float)cx[p];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vx[k] = fmaf(vx[k], px[k], c);
+      }
+      for (int p = 1; p <= h.Ly; ++p) {
+        const float c = (float)cy[p];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vy[k] = fmaf(vy[k], py[k], c);
+      }
+      for (int p = 1; p <= h.Lz; ++p) {
+        const float c = (float)cz[p];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vz[k] = fmaf(vz[k], pz[k], c);
+      }
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const float s = fmaf(px[k], px[k], fmaf(py[k], py[k], pz[k] * pz[k]));
+        const float e = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo));
+        const float term = e * vx[k] * vy[k] * vz[k];
+        const float yk = term - comp[k];
+        const float tk = sum[k] + yk;
+        comp[k] = (tk - sum[k]) - yk;
+        sum[k] = tk;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) acc[k] += (double)sum[k] - (double)comp[k];
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * kT;
+    if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_reduce(const double *__restrict__ partial, int nchunks,
+                                               int64_t G, double *__restrict__ ivec) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= G) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunks; ++c) s += partial[(int64_t)c * G + g];
+  ivec[g] = s;
+}
+
+__global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q, int qstride,
+                                                   int64_t nq, int nbins,
+                                                   unsigned long long *__restrict__ hist) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nq) return;
+  const unsigned cls = (unsigned)__double_as_longlong(q[i * qstride + 4]);
+  int L = (int)((cls & 0xff) + ((cls >> 8) & 0xff) + ((cls >> 16) & 0xff));
+  if (L >= nbins) L = nbins - 1;
+  atomicAdd(&hist[L], 1ULL);
+}
+
+__global__ void k_zero(double *p, int64_t n) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g < n) p[g] = 0.0;
+}
+
+}  // namespace
+
+int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
+  unsigned long long *d_h = nullptr;
+  INCA_HIP(hipMalloc((void **)&d_h, nbins * sizeof(unsigned long long)));
+  std::vector<unsigned long long> h(nbins, 0);
+  hipError_t e = hipMemcpy(d_h, h.data(), nbins * sizeof(unsigned long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ctx->q2 > 0) {
+    hipLaunchKernelGGL(k_class_hist, dim3((unsigned)((ctx->q2 + kT - 1) / kT)), dim3(kT), 0,
+                       ctx->stream, ctx->d_q, ctx->qstride, ctx->q2, nbins, d_h);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(h.data(), d_h, nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  (void)hipFree(d_h);
+  if (e != hipSuccess) return hip_fail(ctx, e, "class histogram", __LINE__);
+  for (int i = 0; i < nbins; ++i) hist[i] = (int64_t)h[i];
+  return 0;
+}
+
+int evaluate_grid(inca_ctx *ctx, int precision) {
+  hipStream_t st = ctx->stream;
+  const int64_t G = ctx->G, nq = ctx->q2;
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_ivec, ctx->ivec_cap, (size_t)(G > 0 ? G : 1)))) return rc;
+  INCA_HIP(hipEventRecord(ctx->ev[1], st));
+  if (G == 0) {
+    INCA_HIP(hipEventRecord(ctx->ev[2], st));
+    INCA_HIP(hipEventRecord(ctx->ev[3], st));
+    ctx->have_ivec = 1;
+    return 0;
+  }
+  if (nq == 0) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st, ctx->d_ivec, G);
+    INCA_HIP(hipGetLastError());
+    INCA_HIP(hipEventRecord(ctx->ev[2], st));
+    INCA_HIP(hipEventRecord(ctx->ev[3], st));
+    ctx->have_ivec = 1;
+    return 0;
+  }
+  constexpr int P = 4;
+  const int64_t tiles = (G + kT * P - 1) / (kT * P);
+  // enough (tile, chunk) blocks to fill 256 CUs several times over
+  int64_t nchunks = (8192 + tiles - 1) / tiles;
+  if (nchunks > nq) nchunks = nq;
+  if (nchunks > 65535) nchunks = 65535;
+  if (nchunks < 1) nchunks = 1;
+  const int64_t qchunk = (nq + nchunks - 1) / nchunks;
+  nchunks = (nq + qchunk - 1) / qchunk;
+  if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
+  dim3 grid((unsigned)tiles, (unsigned)nchunks);
+  if (precision == INCA_MIXED)
+    hipLaunchKernelGGL(k_eval_mixed<P>, grid, dim3(kT), 0, st, ctx->d_q, ctx->qstride, nq, qchunk,
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_partial);
+  else
+    hipLaunchKernelGGL(k_eval_fp64<P>, grid, dim3(kT), 0, st, ctx->d_q, ctx->qstride, nq, qchunk,
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_partial);
+  INCA_HIP(hipGetLastError());
+  INCA_HIP(hipEventRecord(ctx->ev[2], st));
+  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
+                     ctx->d_partial, (int)nchunks, G, ctx->d_ivec);
+  INCA_HIP(hipGetLastError());
+  INCA_HIP(hipEventRecord(ctx->ev[3], st));
+  ctx->have_ivec = 1;
+  return 0;
+}
+
+}  // namespace inca

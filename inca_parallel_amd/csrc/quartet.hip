@@ -1,0 +1,402 @@
+// DM2 quartet preparation on the device.
+//
+// The reference handles each .dm2p record inside its OpenMP loop
+// (mod_intracule.f90:276-344): normalise D, compute both Gaussian products
+// and JA8 bounds, screen 1, the quartet prefactors, three polycoef fits
+// (Gauss-Hermite sums + Vandermonde solve), screen 2.  Here:
+//   k_pair_table   one thread per primitive pair (a,b): JA8, a+b, eR^2, P.
+//                  Screen 1 then costs two table reads per record instead of
+//                  two JA8 evaluations.
+//   k_first_bad    the reader's stop rule (first record with a bad index).
+//   k_prep<NPM>    one thread per record: screens + prefactors + polycoef;
+//                  writes the prepared record into slot n and a pass flag.
+//   compaction     deterministic scan of the flags, then a gather of the
+//                  surviving records in file order.
+// Records are read straight from HBM as three 8-byte words per 24-byte
+// record, so a wavefront streams 1536 contiguous bytes.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace inca {
+namespace {
+
+constexpr int kT = 256;
+
+// Gauss-Hermite nodes (roots of H_n, n = 1..11) and weights
+// w = 2^(n-1) n! sqrt(pi) / (n^2 H_{n-1}(x)^2)  (mod_intrastuff.f90:198-341)
+__constant__ double c_gh_x[11][11];
+__constant__ double c_gh_w[11][11];
+// w_m of the second screen (mod_intracule.f90:136-143)
+__constant__ double c_wm[kNpMax];
+
+const double kGhRoot[11][11] = {
+    {0},
+    {-0.70710678118654746, 0.70710678118654746},
+    {-1.2247448713915889, 0, 1.2247448713915889},
+    {-1.6506801238857847, -0.52464762327529035, 0.52464762327529035, 1.6506801238857847},
+    {-2.0201828704560856, -0.95857246461381851, 0, 0.95857246461381851, 2.0201828704560856},
+    {-2.3506049736744923, -1.335849074013697, -0.43607741192761651, 0.43607741192761651,
+     1.335849074013697, 2.3506049736744923},
+    {-2.6519613568352334, -1.6735516287674714, -0.81628788285896459, 0, 0.81628788285896459,
+     1.6735516287674714, 2.6519613568352334},
+    {-2.9306374202572441, -1.981656756695843, -1.1571937124467802, -0.38118699020732211,
+     0.38118699020732211, 1.1571937124467802, 1.981656756695843, 2.9306374202572441},
+    {-3.1909932017815277, -2.2665805845318432, -1.4685532892166679, -0.72355101875283756, 0,
+     0.72355101875283756, 1.4685532892166679, 2.2665805845318432, 3.1909932017815277},
+    {-3.4361591188377374, -2.5327316742327897, -1.7566836492998816, -1.0366108297895136,
+     -0.34290132722370459, 0.34290132722370459, 1.0366108297895136, 1.7566836492998816,
+     2.5327316742327897, 3.4361591188377374},
+    {-3.6684708465595826, -2.7832900997816514, -2.0259480158257555, -1.3265570844949328,
+     -0.65680956688209979, 0, 0.65680956688209979, 1.3265570844949328, 2.0259480158257555,
+     2.7832900997816514, 3.6684708465595826}};
+
+__device__ __forceinline__ double ipow(double x, int e) {
+  double r = 1.0;
+  for (int k = 0; k < e; ++k) r *= x;
+  return r;
+}
+
+// Largest Gauss-Hermite root for nn nodes: JA8's x_max (:44-113)
+__device__ __forceinline__ double gh_xmax(int nn) {
+  if (nn <= 1) return 0.0;
+  if (nn > 11) nn = 11;
+  return c_gh_x[nn - 1][nn - 1];
+}
+
+__global__ __launch_bounds__(kT) void k_pair_table(int nprim, const double *__restrict__ prim,
+                                                   const int32_t *__restrict__ tmn,
+                                                   PairEntry *__restrict__ pair) {
+  int64_t id = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (id >= (int64_t)nprim * nprim) return;
+  const int a = (int)(id / nprim), b = (int)(id % nprim);
+  const double *al = prim, *X = prim + nprim, *Y = prim + 2 * nprim, *Z = prim + 3 * nprim;
+  const double alfi = al[a], alfk = al[b];
+  const double aik = alfi + alfk;
+  const double eik = alfi * alfk * (1.0 / aik);
+  const double dx = X[a] - X[b], dy = Y[a] - Y[b], dz = Z[a] - Z[b];
+  const double rik2 = dx * dx + dy * dy + dz * dz;
+  // JA8 (mod_intrastuff.f90:115-122)
+  const int ti = tmn[a], mi = tmn[nprim + a], ni = tmn[2 * nprim + a];
+  const int tk = tmn[b], mk = tmn[nprim + b], nk = tmn[2 * nprim + b];
+  const double s = sqrt(2.0 / aik);
+  const double x0 = gh_xmax(ti + tk + 1), x1 = gh_xmax(mi + mk + 1), x2 = gh_xmax(ni + nk + 1);
+  const double L = (double)(ti + tk + mi + mk + ni + nk);
+  const double adx = fabs(dx), ady = fabs(dy), adz = fabs(dz);
+  double J = pow(M_PI, 1.5) * pow(2.0 * aik, -(L + 1.5)) * exp(-2.0 * eik * rik2);
+  J *= ipow(x0 + alfk * s * adx, 2 * ti) * ipow(x0 + alfi * s * adx, 2 * tk);
+  J *= ipow(x1 + alfk * s * ady, 2 * mi) * ipow(x1 + alfi * s * ady, 2 * mk);
+  J *= ipow(x2 + alfk * s * adz, 2 * ni) * ipow(x2 + alfi * s * adz, 2 * nk);
+  PairEntry e;
+  e.J = J;
+  e.aik = aik;
+  e.eR2 = eik * rik2;
+  e.P[0] = (alfi * X[a] + alfk * X[b]) * (1.0 / aik);
+  e.P[1] = (alfi * Y[a] + alfk * Y[b]) * (1.0 / aik);
+  e.P[2] = (alfi * Z[a] + alfk * Z[b]) * (1.0 / aik);
+  e.pad[0] = e.pad[1] = 0.0;
+  pair[id] = e;
+}
+
+__global__ __launch_bounds__(kT) void k_first_bad(const int32_t *__restrict__ rec, int64_t n,
+                                                  int nprim,
+                                                  unsigned long long *__restrict__ first) {
+  int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r >= n) return;
+  const int32_t *p = rec + 6 * r;
+  int i = p[0], j = p[1], k = p[2], l = p[3];
+  bool bad = i < 1 || i > nprim || j < 1 || k < 1 || l < 1 || j > nprim || k > nprim || l > nprim;
+  if (bad) atomicMin(first, (unsigned long long)r);
+}
+
+// polycoef for one axis (mod_intrastuff.f90:127-194).  Returns 1 when the
+// reference would zero the coefficients (INFO != 0 or any |C| > 1e3).
+template <int NPM>
+__device__ int polycoef(const double R[4], double Rik, double Rjl, double Rijkl, const int t[4],
+                        int np, double sqe, double invz, double alf, double *C) {
+  const int L = np - 1;
+  const int n = L / 2 + 1;  // Gauss-Hermite nodes, exact for degree 2n-1 >= L
+  if (np == 1) {
+    C[0] = c_gh_w[0][0];
+    return 0;
+  }
+  double M[NPM * NPM];
+  for (int i = 0; i < np; ++i) {
+    const double xx = -0.5 + (double)(i + 1);
+    const double y = sqe * (xx + Rik - Rjl);
+    // M(i, j) = y^(L - j): fill from the last column (power 0) leftwards
+    double pw = 1.0;
+    for (int j = L; j >= 0; --j) {
+      M[i + np * j] = pw;
+      pw *= y;
+    }
+    const double am = (alf - 0.5) * xx, ap = (alf + 0.5) * xx;
+    double c = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double a = invz * c_gh_x[n - 1][k];
+      const double w = ipow(a + am + (Rijkl - R[0]), t[0]) * ipow(a + ap + (Rijkl - R[1]), t[1]) *
+                       ipow(a + am + (Rijkl - R[2]), t[2]) * ipow(a + ap + (Rijkl - R[3]), t[3]);
+      c += c_gh_w[n - 1][k] * w;
+    }
+    C[i] = c;
+  }
+  // partial-pivot LU + solve in reference LAPACK order (dgetrf2 / dgetrs)
+  int piv[NPM];
+  int info = 0;
+  for (int k = 0; k < np; ++k) {
+    int p = k;
+    double amax = fabs(M[k + np * k]);
+    for (int i = k + 1; i < np; ++i) {
+      double v = fabs(M[i + np * k]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    piv[k] = p;
+    if (M[p + np * k] != 0.0) {
+      if (p != k)
+        for (int j = 0; j < np; ++j) {
+          double tt = M[k + np * j];
+          M[k + np * j] = M[p + np * j];
+          M[p + np * j] = tt;
+        }
+      const double pv = M[k + np * k];
+      if (fabs(pv) >= 2.2250738585072014e-308) {
+        const double rp = 1.0 / pv;
+        for (int i = k + 1; i < np; ++i) M[i + np * k] *= rp;
+      } else {
+        for (int i = k + 1; i < np; ++i) M[i + np * k] /= pv;
+      }
+    } else if (info == 0) {
+      info = k + 1;
+    }
+    for (int j = k + 1; j < np; ++j) {
+      const double u = M[k + np * j];
+      for (int i = k + 1; i < np; ++i) M[i + np * j] -= M[i + np * k] * u;
+    }
+  }
+  int bad = (info != 0);
+  if (!bad) {
+    for (int k = 0; k < np; ++k)
+      if (piv[k] != k) {
+        double tt = C[k];
+        C[k] = C[piv[k]];
+        C[piv[k]] = tt;
+      }
+    for (int k = 0; k < np; ++k)
+      if (C[k] != 0.0)
+        for (int i = k + 1; i < np; ++i) C[i] -= C[k] * M[i + np * k];
+    for (int k = np - 1; k >= 0; --k)
+      if (C[k] != 0.0) {
+        C[k] /= M[k + np * k];
+        for (int i = 0; i < k; ++i) C[i] -= C[k] * M[i + np * k];
+      }
+    for (int i = 0; i < np; ++i)
+      if (!(fabs(C[i]) <= 1.0e3)) bad = 1;
+  }
+  if (bad)
+    for (int i = 0; i < np; ++i) C[i] = 0.0;
+  return bad;
+}
+
+// One thread per record.  Output record (qstride doubles):
+//   [0] sqe  [1..3] sqe*(P_ik - P_jl)  [4] class bits Lx | Ly<<8 | Lz<<16
+//   [5 ..]  A*Cx(0..Lx), Cy(0..Ly), Cz(0..Lz)   (C(0) = highest power)
+template <int NPM>
+__global__ __launch_bounds__(kT) void k_prep(
+    const int32_t *__restrict__ rec, int64_t nvalid, int nprim, const double *__restrict__ prim,
+    const int32_t *__restrict__ tmn, const PairEntry *__restrict__ pair, double lim,
+    int normalize, int qstride, double *__restrict__ qout, uint8_t *__restrict__ flag,
+    unsigned long long *__restrict__ cnt) {
+  const int64_t n = (int64_t)blockIdx.x * kT + threadIdx.x;
+  int pass1 = 0, pass2 = 0, nfail = 0;
+  if (n < nvalid) {
+    const int32_t *p = rec + 6 * n;
+    const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+    double D;
+    __builtin_memcpy(&D, p + 4, sizeof(double));
+    const double *nrm = prim + 4 * nprim;
+    double DMval = D;
+    if (normalize) DMval = nrm[i] * nrm[j] * nrm[k] * nrm[l] * D;
+    const PairEntry pik = pair[(int64_t)i * nprim + k];
+    const PairEntry pjl = pair[(int64_t)j * nprim + l];
+    const double screen1 = fabs(DMval) * sqrt(pik.J * pjl.J);
+    if (screen1 >= lim) {
+      pass1 = 1;
+      const double aik = pik.aik, ajl = pjl.aik;
+      const double zeta = aik + ajl;
+      const double eijkl = (aik * ajl) * (1.0 / zeta);
+      const double sqe = sqrt(eijkl);
+      const double invz = 1.0 / sqrt(zeta);
+      const double alf = 0.5 * (aik - ajl) * (1.0 / zeta);
+      const double A = DMval * pow(zeta, -1.5) * exp(-pik.eR2 - pjl.eR2);
+      double *out = qout + n * (int64_t)qstride;
+      int off = 5;
+      int Ls[3];
+      double U = A;
+      double C[NPM];
+      for (int ax = 0; ax < 3; ++ax) {
+        const double *Rx = prim + (1 + ax) * nprim;
+        const double R[4] = {Rx[i], Rx[j], Rx[k], Rx[l]};
+        const int *tt = tmn + ax * nprim;
+        const int t[4] = {tt[i], tt[j], tt[k], tt[l]};
+        const int L = t[0] + t[1] + t[2] + t[3];
+        Ls[ax] = L;
+        const double Pijkl = (aik * pik.P[ax] + ajl * pjl.P[ax]) * (1.0 / zeta);
+        nfail += polycoef<NPM>(R, pik.P[ax], pjl.P[ax], Pijkl, t, L + 1, sqe, invz, alf, C);
+        double u = 0.0;
+        for (int q = 0; q <= L; ++q) u += C[q] * c_wm[q];
+        U *= u;
+        const double scale = (ax == 0) ? A : 1.0;
+        for (int q = 0; q <= L; ++q) out[off + q] = scale * C[q];
+        off += L + 1;
+      }
+      if (fabs(U) >= lim) {
+        pass2 = 1;
+        out[0] = sqe;
+        out[1] = sqe * (pik.P[0] - pjl.P[0]);
+        out[2] = sqe * (pik.P[1] - pjl.P[1]);
+        out[3] = sqe * (pik.P[2] - pjl.P[2]);
+        out[4] = __longlong_as_double((long long)(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16)));
+      }
+    }
+    flag[n] = (uint8_t)pass2;
+  }
+  // wave-aggregated counters
+  unsigned long long b1 = __ballot(pass1);
+  unsigned long long bf = 0;
+  int wsum = nfail;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) wsum += __shfl_xor(wsum, d, 64);
+  (void)bf;
+  if (__lane_id() == 0) {
+    if (b1) atomicAdd(&cnt[1], (unsigned long long)__popcll(b1));
+    if (wsum) atomicAdd(&cnt[3], (unsigned long long)wsum);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict__ src,
+                                                       const int64_t *__restrict__ idx,
+                                                       int64_t n, int stride,
+                                                       double *__restrict__ dst) {
+  // one thread per (record, word)
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= n * stride) return;
+  const int64_t r = t / stride;
+  const int w = (int)(t - r * stride);
+  dst[t] = src[idx[r] * stride + w];
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+bool g_const_ready[64] = {false};
+
+int upload_constants(inca_ctx *ctx) {
+  if (ctx->device >= 0 && ctx->device < 64 && g_const_ready[ctx->device]) return 0;
+  double w[11][11] = {};
+  const double sqrtpi = std::sqrt(std::acos(-1.0));
+  for (int n = 1; n <= 11; ++n) {
+    double factn = 1.0;
+    for (int i = 2; i <= n; ++i) factn *= i;
+    const double cons = std::pow(2.0, (double)(n - 1)) * factn * sqrtpi / (double)(n * n);
+    for (int i = 0; i < n; ++i) {
+      const double x = kGhRoot[n - 1][i];
+      double h0 = 1.0, h1 = 2.0 * x;  // H_{n-1}(x) by recurrence
+      double h = h0;
+      if (n >= 2) {
+        for (int k = 2; k < n; ++k) {
+          double h2 = 2.0 * x * h1 - 2.0 * (k - 1) * h0;
+          h0 = h1;
+          h1 = h2;
+        }
+        h = h1;
+      }
+      w[n - 1][i] = cons * (1.0 / (h * h));
+    }
+  }
+  double wm[kNpMax];
+  for (int p = 1; p <= kNpMax; ++p)
+    wm[p - 1] = (p == 1) ? 1.0 : std::pow(p * 0.5, p * 0.5) * std::exp(-(double)p * 0.5);
+  INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gh_x), kGhRoot, sizeof(kGhRoot)));
+  INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gh_w), w, sizeof(w)));
+  INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_wm), wm, sizeof(wm)));
+  if (ctx->device >= 0 && ctx->device < 64) g_const_ready[ctx->device] = true;
+  return 0;
+}
+
+}  // namespace
+
+int build_pair_table(inca_ctx *ctx) {
+  int rc;
+  if ((rc = upload_constants(ctx))) return rc;
+  const int64_t np2 = (int64_t)ctx->nprim * ctx->nprim;
+  if (ctx->d_pair) (void)hipFree(ctx->d_pair);
+  ctx->d_pair = nullptr;
+  INCA_HIP(hipMalloc((void **)&ctx->d_pair, (size_t)np2 * sizeof(PairEntry)));
+  hipLaunchKernelGGL(k_pair_table, dim3(nblk(np2)), dim3(kT), 0, ctx->stream, ctx->nprim,
+                     ctx->d_prim, ctx->d_tmn, ctx->d_pair);
+  INCA_HIP(hipGetLastError());
+  return 0;
+}
+
+int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
+  (void)precision;
+  hipStream_t st = ctx->stream;
+  const int nprim = ctx->nprim;
+  const int64_t nrec = ctx->nrec;
+  int rc;
+  // stride: 5 header words + up to 3 (L+1) coefficients, even for 16-B rows
+  const int la = ctx->lmax_axis;
+  int stride = 5 + 3 * (la + 1);
+  stride += stride & 1;
+  ctx->qstride = stride;
+  unsigned long long hcnt[4] = {(unsigned long long)nrec, 0, 0, 0};
+  INCA_HIP(hipMemcpyAsync(ctx->d_cnt, hcnt, sizeof(hcnt), hipMemcpyHostToDevice, st));
+  if (nrec > 0) {
+    hipLaunchKernelGGL(k_first_bad, dim3(nblk(nrec)), dim3(kT), 0, st,
+                       (const int32_t *)ctx->d_rec, nrec, nprim, ctx->d_cnt);
+    INCA_HIP(hipGetLastError());
+  }
+  INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  const int64_t nvalid = (int64_t)hcnt[0];
+  if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)(nvalid > 0 ? nvalid : 1) * stride)))
+    return rc;
+  if ((rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)(nvalid > 0 ? nvalid : 1)))) return rc;
+  if ((rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)(nvalid > 0 ? nvalid : 1)))) return rc;
+  // mod_intracule.f90:163
+  const double lim = thresh * (1.0 / ((double)nprim * ((double)nprim + 1.0) * 0.5));
+  if (nvalid > 0) {
+#define INCA_PREP(NPM)                                                                        \
+  hipLaunchKernelGGL(k_prep<NPM>, dim3(nblk(nvalid)), dim3(kT), 0, st,                       \
+                     (const int32_t *)ctx->d_rec, nvalid, nprim, ctx->d_prim, ctx->d_tmn,     \
+                     ctx->d_pair, lim, ctx->normalize, stride, ctx->d_qtmp, ctx->d_flag,      \
+                     ctx->d_cnt)
+    if (la <= 4) INCA_PREP(5);
+    else if (la <= 8) INCA_PREP(9);
+    else if (la <= 12) INCA_PREP(13);
+    else if (la <= 16) INCA_PREP(17);
+    else INCA_PREP(22);
+#undef INCA_PREP
+    INCA_HIP(hipGetLastError());
+  }
+  int64_t q2 = 0;
+  if ((rc = compact_flags(ctx, ctx->d_flag, nvalid, ctx->d_idx, &q2))) return rc;
+  if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride))) return rc;
+  if (q2 > 0) {
+    hipLaunchKernelGGL(k_gather_records, dim3(nblk(q2 * stride)), dim3(kT), 0, st, ctx->d_qtmp,
+                       ctx->d_idx, q2, stride, ctx->d_q);
+    INCA_HIP(hipGetLastError());
+  }
+  INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(hcnt), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  ctx->q2 = q2;
+  counts[0] = nvalid;
+  counts[1] = (int64_t)hcnt[1];
+  counts[2] = q2;
+  counts[3] = (int64_t)hcnt[3];
+  return 0;
+}
+
+}  // namespace inca

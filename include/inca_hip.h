@@ -1,0 +1,167 @@
+/*
+ * inca_hip.h -- C-ABI of libinca_hip.so, the MI355X (gfx950) engine behind
+ * the INCA intracule drop-in.
+ *
+ * The reference (markelgtx/INCA_parallel) has no FFI: its hot path is the
+ * Fortran module procedure intracule::calculate_intracule
+ * (src/mod_intracule.f90:18), which reads module globals and writes files.
+ * The project's Fortran host keeps that procedure and its globals and calls
+ * the entry points below through ISO_C_BINDING interfaces; each entry point
+ * names the reference code whose work it takes over.
+ *
+ * Conventions
+ *  - Plain C types only; every function returns 0 on success or a negative
+ *    INCA_E* code, and inca_last_error() gives the message.
+ *  - Arrays use the reference's Fortran layouts: r(3,G) is point-major
+ *    (x,y,z of point g at r[3g..3g+2]); tmn(nprim,3) is column-major.
+ *  - Logical flags cross as int32 (0 = false).
+ *  - Host arrays are owned by the caller and only read during the call;
+ *    device memory is owned by the context until inca_destroy().
+ *  - One context serves one host thread; it is not thread-safe.
+ */
+#ifndef INCA_HIP_H
+#define INCA_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct inca_ctx inca_ctx;
+
+enum {
+  INCA_OK = 0,
+  INCA_EINVAL = -1,   /* bad argument or call order */
+  INCA_EHIP = -2,     /* HIP runtime error */
+  INCA_EIO = -3,      /* file could not be opened or read */
+  INCA_EFORMAT = -4,  /* unsupported .dm2p format (mod_intracule.f90:192-194) */
+  INCA_ENOMEM = -5,   /* device allocation failed */
+  INCA_ERANGE = -6    /* size or angular momentum beyond what is supported */
+};
+
+enum { INCA_FP64 = 0, INCA_MIXED = 1 };
+
+/* Create a context on HIP device `device` (one stream, default priority). */
+int inca_create(int32_t device, inca_ctx **ctx);
+int inca_destroy(inca_ctx *ctx);
+/* Message of the last failing call (or of inca_create when ctx is NULL). */
+const char *inca_last_error(const inca_ctx *ctx);
+
+/* Primitive tables: geninfo Alpha, TMN, Xn/Yn/Zn (mod_geninfo.f90:16-19).
+ * normalize != 0 multiplies every DM2 value by N_i N_j N_k N_l with the
+ * normalisation of mod_intracule.f90:145-152 (calculate_intracule's
+ * normalize_dm2p argument).  Also builds the primitive-pair table used by
+ * screen 1 (JA8, mod_intrastuff.f90:32-123, called at
+ * mod_intracule.f90:297-305). */
+int inca_set_primitives(inca_ctx *ctx, int32_t nprim, const double *alpha,
+                        const int32_t *tmn, const double *x, const double *y,
+                        const double *z, int32_t normalize);
+
+/* Multicentre radial x Lebedev grid with TFVC weights: gridpoints + becke
+ * (mod_build_grid.f90:13-277, 474-575), called at mod_intracule.f90:91-97.
+ * cent(3,nquad); nradc/nangc/sfalpha/Ps(nquad).  Returns the reference's
+ * counters maxgrid, rgrid, rrgrid (mod_intracule.f90:416-418). */
+int inca_grid_multicenter(inca_ctx *ctx, int32_t nquad, const double *cent,
+                          const int32_t *nradc, const int32_t *nangc,
+                          const double *sfalpha, const double *Ps,
+                          int32_t nosym, int64_t *maxgrid, int64_t *rgrid,
+                          int64_t *rrgrid);
+
+/* Radial-scan grid: gridpoints2 (mod_build_grid.f90:281-408), called at
+ * mod_intracule.f90:99-104.  tart(2,nblock) = (start,end) per block.
+ * Returns the number of radii and of kept points. */
+int inca_grid_radial_plot(inca_ctx *ctx, int32_t nblock, const double *tart,
+                          const double *step, const int32_t *nang,
+                          int32_t nosym, int64_t *nradi, int64_t *ngrid);
+
+/* Cube grid: gridpoints3 (mod_build_grid.f90:412-440), called at
+ * mod_intracule.f90:106-112.  Point order: x outer, y middle, z inner. */
+int inca_grid_cube(inca_ctx *ctx, const double *center, const double *step,
+                   const int32_t *np, int64_t *ngrid);
+
+/* Explicit points r(3,n): intracule_at_zero / intracule_two_points
+ * (mod_intracule.f90:114-129). */
+int inca_grid_points(inca_ctx *ctx, int64_t n, const double *r);
+
+/* Copy the current grid to the host (any pointer may be NULL):
+ * r(3,G) = rrrg/rpg/rg, w = rweight (multicentre) or w_ang (radial scan),
+ * w_vee = rweight_vee, center_map = point_center_map (1-based)
+ * (mod_quadratures.f90:5-21). */
+int inca_get_grid(inca_ctx *ctx, double *r, double *w, double *w_vee,
+                  int32_t *center_map);
+/* Radial-scan layout: radi(nradi) and smn(nradi) (mod_quadratures.f90:16-18) */
+int inca_get_radial_layout(inca_ctx *ctx, double *radi, int32_t *smn);
+
+/* Read a .dm2p file into device memory, with the reference's format
+ * detection (mod_intracule.f90:164-195): flat 24-byte records, or Fortran
+ * sequential records with 4-byte markers.  8-byte-marker and zero-marker
+ * files fail with INCA_EFORMAT.  *format = 0 flat, 1 sequential.
+ * *nrec = records stored (reading stops at EOF or at a bad marker; index
+ * validity is applied by inca_evaluate). */
+int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec,
+                   int32_t *format);
+/* Same records from memory: n packed 24-byte records (int32 i,j,k,l;
+ * double D).  on_device != 0: `records` is a device pointer that is copied
+ * device-to-device (the caller keeps ownership). */
+int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
+                      int32_t on_device);
+
+/* The quartet loop of calculate_intracule (mod_intracule.f90:198-388):
+ * decode + normalise (:276-285), screen 1 (:297-305), prefactors and
+ * polycoef on three axes (:307-341, mod_intrastuff.f90:127-194), screen 2
+ * (:342-344) and the grid loop (:345-358) summed over all quartets.
+ *   precision: INCA_FP64 (reference arithmetic) or INCA_MIXED (FP32
+ *              Gaussian factors and polynomials, compensated FP32 tile sums
+ *              folded into FP64).
+ *   thresh:    the namelist thresh; lim = thresh / (nprim(nprim+1)/2).
+ *   ivec:      optional host copy of Ivec(G) (may be NULL).
+ *   counts:    [read, after screen 1, after screen 2, polycoef failures]
+ *              (mod_intracule.f90:419-421; the last is the number of
+ *              per-axis fits the reference zeroes at mod_intrastuff.f90:170).
+ * Records are consumed up to the first invalid index (i outside 1..nprim,
+ * or j,k,l outside 1..nprim) as the reference's reader does (:239-240). */
+int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh,
+                  double *ivec, int64_t *counts);
+
+/* Copy Ivec(G) of the last evaluation. */
+int inca_get_ivec(inca_ctx *ctx, double *ivec);
+
+/* FP64 grid reductions of mod_intracule.f90:441-455 on the device:
+ * integral = sum rweight*Ivec, vee = sum rweight_vee*Ivec, and the TFVC
+ * per-centre split (each of length nquad; may be NULL).  Multicentre grid
+ * only. */
+int inca_integrate(inca_ctx *ctx, double *integral, double *vee,
+                   double *int_per_centre, double *vee_per_centre);
+
+/* Radial scan values rintra(i) = sum_k w_ang*Ivec over radius i
+ * (mod_intracule.f90:522-530).  rintra has nradi entries. */
+int inca_radial_values(inca_ctx *ctx, double *rintra);
+
+/* radial_smearing (mod_intracule.f90:576-649) on the multicentre grid and
+ * the last Ivec: u, P, Iavg each n_bins long. */
+int inca_smearing(inca_ctx *ctx, double u_max, int32_t n_bins, double sigma,
+                  double *u, double *P, double *Iavg);
+
+/* Kernel time of the last inca_evaluate, in milliseconds, measured with
+ * HIP events on the context's stream: t[0] quartet preparation (decode,
+ * screens, polycoef, compaction), t[1] grid evaluation, t[2] reduction.
+ * Replaces the reference's omp_get_wtime stamps (mod_intracule.f90:88-503). */
+int inca_get_timings(inca_ctx *ctx, double *t);
+
+/* Histogram of Lsum = Lx+Ly+Lz over the quartets that passed both screens in
+ * the last inca_evaluate: hist[L] for L = 0..nbins-1 (the per-eval work is
+ * 16 + 2 Lsum FLOP + 1 exp, so this gives the algorithmic FLOP count).
+ * No reference counterpart: a measurement aid. */
+int inca_class_histogram(inca_ctx *ctx, int32_t nbins, int64_t *hist);
+
+/* Lebedev-Laikov rule with n points as unit vectors and weights summing to
+ * one (lebedev.f LDnnnn).  Returns INCA_ERANGE for an unsupported n. */
+int inca_lebedev(int32_t n, double *x, double *y, double *z, double *w);
+
+/* Gauss-Legendre nodes and weights on [-1,1] (mod_quadratures.f90:37-86). */
+int inca_gauleg(int32_t n, double *t, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
