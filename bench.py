@@ -116,7 +116,7 @@ def main():
         dt = (time.perf_counter() - t0) / a.steps
         kern = np.array(kern)
         results[name] = dict(step_s=dt, counts=cnt, kern_ms=kern.mean(axis=0),
-                             rintra=eng.radial_values())
+                             rintra=eng.radial_values(), split=eng.mixed_split())
         log("%s: %.3f s/step, kernels (prep, eval, reduce) ms = %s, counts %s" % (
             name, dt, np.round(kern.mean(axis=0), 2), cnt))
 
@@ -164,6 +164,7 @@ def main():
             "speedup_vs_fp64": r64["step_s"] / rm["step_s"],
             "max_rel_err_rintra_vs_fp64": mixed_err,
             "eval_kernel_ms": float(rm["kern_ms"][1]),
+            "quartets_fp64": int(rm["split"][0]), "quartets_fp32": int(rm["split"][1]),
         },
         "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
                       "reduce": float(r64["kern_ms"][2])},

@@ -23,7 +23,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_grid_points", "inca_get_grid", "inca_get_radial_layout",
            "inca_load_dm2p", "inca_set_quartets", "inca_evaluate", "inca_get_ivec",
            "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
-           "inca_class_histogram",
+           "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -67,6 +67,8 @@ def load():
     L.inca_smearing.argtypes = [vp, C.c_double, C.c_int32, C.c_double, vp, vp, vp]
     L.inca_get_timings.argtypes = [vp, _dp]
     L.inca_class_histogram.argtypes = [vp, C.c_int32, vp]
+    L.inca_set_mixed_tail.argtypes = [vp, C.c_double]
+    L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
     for s in SYMBOLS:
@@ -254,6 +256,15 @@ class Engine:
         self._ck(self._L.inca_class_histogram(self._h, int(nbins), h.ctypes.data),
                  "inca_class_histogram")
         return h
+
+    def set_mixed_tail(self, tail):
+        self._ck(self._L.inca_set_mixed_tail(self._h, float(tail)), "inca_set_mixed_tail")
+
+    def mixed_split(self):
+        a, b, t = C.c_int64(), C.c_int64(), C.c_double()
+        self._ck(self._L.inca_get_mixed_split(self._h, C.addressof(a), C.addressof(b),
+                                              C.addressof(t)), "inca_get_mixed_split")
+        return a.value, b.value, t.value
 
     def timings(self):
         t = np.zeros(3)

@@ -336,6 +336,21 @@ int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh, double *ivec,
   return 0;
 }
 
+int inca_set_mixed_tail(inca_ctx *ctx, double tail) {
+  CHECK_CTX(ctx);
+  if (!(tail >= 0.0 && tail <= 1.0)) return set_err(ctx, INCA_EINVAL, "tail must be in [0,1]");
+  ctx->mixed_tail = tail;
+  return 0;
+}
+
+int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32, double *tau) {
+  CHECK_CTX(ctx);
+  if (n_fp64) *n_fp64 = ctx->q_fp64;
+  if (n_fp32) *n_fp32 = ctx->q2 - ctx->q_fp64;
+  if (tau) *tau = ctx->mixed_tau;
+  return 0;
+}
+
 int inca_get_ivec(inca_ctx *ctx, double *ivec) {
   CHECK_CTX(ctx);
   if (!ctx->have_ivec) return set_err(ctx, INCA_EINVAL, "no Ivec: call inca_evaluate first");

@@ -15,6 +15,10 @@ namespace inca {
 // (mod_intrastuff.f90:11-15: nodes for L <= 21).
 constexpr int kLmaxAxis = 21;
 constexpr int kNpMax = kLmaxAxis + 1;
+// Prepared-quartet record: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
+// Lx | Ly<<8 | Lz<<16, [5] |A Ux Uy Uz| (the screen-2 magnitude), then the
+// coefficients A*Cx(0..Lx), Cy(0..Ly), Cz(0..Lz) (C(0) = highest power).
+constexpr int kQHead = 6;
 
 enum GridKind { GRID_NONE = 0, GRID_MULTI = 1, GRID_RADIAL = 2, GRID_CUBE = 3, GRID_POINTS = 4 };
 
@@ -68,6 +72,9 @@ struct inca_ctx {
   // prepared quartets
   int qstride = 0;       // doubles per prepared record
   int64_t q2 = 0;
+  int64_t q_fp64 = 0;         // mixed mode: leading records evaluated in FP64
+  double mixed_tail = 0.02;   // share of sum |U| left to the FP32 kernel
+  double mixed_tau = 0.0;     // |U| threshold chosen for the split
   double *d_q = nullptr;      // compacted prepared records
   size_t q_cap = 0;
   double *d_qtmp = nullptr;   // uncompacted records (slot = record index)

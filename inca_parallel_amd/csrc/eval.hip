@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   for (int64_t qi = q0; qi < q1; ++qi) {
     const double *__restrict__ rec = q + qi * qstride;
     const QHeader h = load_header(rec);
-    const double *__restrict__ cx = rec + 5;
+    const double *__restrict__ cx = rec + kQHead;
     const double *__restrict__ cy = cx + h.Lx + 1;
     const double *__restrict__ cz = cy + h.Ly + 1;
     double px[P], py[P], pz[P], vx[P], vy[P], vz[P];
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
     for (int64_t qi = t0; qi < t1; ++qi) {
       const double *__restrict__ rec = q + qi * qstride;
       const QHeader h = load_header(rec);
-      const double *__restrict__ cx = rec + 5;
+      const double *__restrict__ cx = rec + kQHead;
       const double *__restrict__ cy = cx + h.Lx + 1;
       const double *__restrict__ cz = cy + h.Ly + 1;
       float px[P], py[P], pz[P], vx[P], vy[P], vz[P];
@@ -237,12 +237,18 @@ __global__ __launch_bounds__(kT) void k_reduce(const double *__restrict__ partia
 __global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q, int qstride,
                                                    int64_t nq, int nbins,
                                                    unsigned long long *__restrict__ hist) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= nq) return;
-  const unsigned cls = (unsigned)__double_as_longlong(q[i * qstride + 4]);
-  int L = (int)((cls & 0xff) + ((cls >> 8) & 0xff) + ((cls >> 16) & 0xff));
-  if (L >= nbins) L = nbins - 1;
-  atomicAdd(&hist[L], 1ULL);
+  __shared__ unsigned int h[64];
+  if (threadIdx.x < 64) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < nq; i += (int64_t)gridDim.x * kT) {
+    const unsigned cls = (unsigned)__double_as_longlong(q[i * qstride + 4]);
+    int L = (int)((cls & 0xff) + ((cls >> 8) & 0xff) + ((cls >> 16) & 0xff));
+    if (L >= nbins) L = nbins - 1;
+    atomicAdd(&h[L], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && threadIdx.x < nbins && h[threadIdx.x])
+    atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 
 __global__ void k_zero(double *p, int64_t n) {
@@ -253,12 +259,13 @@ __global__ void k_zero(double *p, int64_t n) {
 }  // namespace
 
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
+  if (nbins > 64) nbins = 64;
   unsigned long long *d_h = nullptr;
   INCA_HIP(hipMalloc((void **)&d_h, nbins * sizeof(unsigned long long)));
   std::vector<unsigned long long> h(nbins, 0);
   hipError_t e = hipMemcpy(d_h, h.data(), nbins * sizeof(unsigned long long), hipMemcpyHostToDevice);
   if (e == hipSuccess && ctx->q2 > 0) {
-    hipLaunchKernelGGL(k_class_hist, dim3((unsigned)((ctx->q2 + kT - 1) / kT)), dim3(kT), 0,
+    hipLaunchKernelGGL(k_class_hist, dim3(1024), dim3(kT), 0,
                        ctx->stream, ctx->d_q, ctx->qstride, ctx->q2, nbins, d_h);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -277,15 +284,12 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   int rc;
   if ((rc = ensure(ctx, ctx->d_ivec, ctx->ivec_cap, (size_t)(G > 0 ? G : 1)))) return rc;
   INCA_HIP(hipEventRecord(ctx->ev[1], st));
-  if (G == 0) {
-    INCA_HIP(hipEventRecord(ctx->ev[2], st));
-    INCA_HIP(hipEventRecord(ctx->ev[3], st));
-    ctx->have_ivec = 1;
-    return 0;
-  }
-  if (nq == 0) {
-    hipLaunchKernelGGL(k_zero, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st, ctx->d_ivec, G);
-    INCA_HIP(hipGetLastError());
+  if (G == 0 || nq == 0) {
+    if (G > 0) {
+      hipLaunchKernelGGL(k_zero, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
+                         ctx->d_ivec, G);
+      INCA_HIP(hipGetLastError());
+    }
     INCA_HIP(hipEventRecord(ctx->ev[2], st));
     INCA_HIP(hipEventRecord(ctx->ev[3], st));
     ctx->have_ivec = 1;
@@ -293,21 +297,30 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   }
   constexpr int P = 4;
   const int64_t tiles = (G + kT * P - 1) / (kT * P);
-  // enough (tile, chunk) blocks to fill 256 CUs several times over
-  int64_t nchunks = (8192 + tiles - 1) / tiles;
-  if (nchunks > nq) nchunks = nq;
-  if (nchunks > 65535) nchunks = 65535;
-  if (nchunks < 1) nchunks = 1;
-  const int64_t qchunk = (nq + nchunks - 1) / nchunks;
-  nchunks = (nq + qchunk - 1) / qchunk;
+  // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
+  const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
+  const int64_t nr[2] = {n64, nq - n64};
+  int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
+  for (int r = 0; r < 2; ++r) {
+    if (nr[r] == 0) continue;
+    // enough (tile, chunk) blocks to fill 256 CUs several times over,
+    // split between the two ranges by their quartet counts
+    int64_t want = (int64_t)(8192.0 * (double)nr[r] / (double)nq) / tiles + 1;
+    if (want > nr[r]) want = nr[r];
+    if (want > 65535) want = 65535;
+    qch[r] = (nr[r] + want - 1) / want;
+    nch[r] = (nr[r] + qch[r] - 1) / qch[r];
+  }
+  const int64_t nchunks = nch[0] + nch[1];
   if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
-  dim3 grid((unsigned)tiles, (unsigned)nchunks);
-  if (precision == INCA_MIXED)
-    hipLaunchKernelGGL(k_eval_mixed<P>, grid, dim3(kT), 0, st, ctx->d_q, ctx->qstride, nq, qchunk,
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_partial);
-  else
-    hipLaunchKernelGGL(k_eval_fp64<P>, grid, dim3(kT), 0, st, ctx->d_q, ctx->qstride, nq, qchunk,
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_partial);
+  if (nch[0] > 0)
+    hipLaunchKernelGGL(k_eval_fp64<P>, dim3((unsigned)tiles, (unsigned)nch[0]), dim3(kT), 0, st,
+                       ctx->d_q, ctx->qstride, nr[0], qch[0], ctx->d_gx, ctx->d_gy, ctx->d_gz,
+                       G, ctx->d_partial);
+  if (nch[1] > 0)
+    hipLaunchKernelGGL(k_eval_mixed<P>, dim3((unsigned)tiles, (unsigned)nch[1]), dim3(kT), 0, st,
+                       ctx->d_q + n64 * ctx->qstride, ctx->qstride, nr[1], qch[1], ctx->d_gx,
+                       ctx->d_gy, ctx->d_gz, G, ctx->d_partial + nch[0] * G);
   INCA_HIP(hipGetLastError());
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,

@@ -14,6 +14,7 @@
 //                  surviving records in file order.
 // Records are read straight from HBM as three 8-byte words per 24-byte
 // record, so a wavefront streams 1536 contiguous bytes.
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -200,9 +201,7 @@ __device__ int polycoef(const double R[4], double Rik, double Rjl, double Rijkl,
   return bad;
 }
 
-// One thread per record.  Output record (qstride doubles):
-//   [0] sqe  [1..3] sqe*(P_ik - P_jl)  [4] class bits Lx | Ly<<8 | Lz<<16
-//   [5 ..]  A*Cx(0..Lx), Cy(0..Ly), Cz(0..Lz)   (C(0) = highest power)
+// One thread per record; the output record layout is described at kQHead.
 template <int NPM>
 __global__ __launch_bounds__(kT) void k_prep(
     const int32_t *__restrict__ rec, int64_t nvalid, int nprim, const double *__restrict__ prim,
@@ -232,7 +231,7 @@ __global__ __launch_bounds__(kT) void k_prep(
       const double alf = 0.5 * (aik - ajl) * (1.0 / zeta);
       const double A = DMval * pow(zeta, -1.5) * exp(-pik.eR2 - pjl.eR2);
       double *out = qout + n * (int64_t)qstride;
-      int off = 5;
+      int off = kQHead;
       int Ls[3];
       double U = A;
       double C[NPM];
@@ -259,6 +258,7 @@ __global__ __launch_bounds__(kT) void k_prep(
         out[2] = sqe * (pik.P[1] - pjl.P[1]);
         out[3] = sqe * (pik.P[2] - pjl.P[2]);
         out[4] = __longlong_as_double((long long)(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16)));
+        out[5] = fabs(U);
       }
     }
     flag[n] = (uint8_t)pass2;
@@ -286,6 +286,44 @@ __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict_
   const int64_t r = t / stride;
   const int w = (int)(t - r * stride);
   dst[t] = src[idx[r] * stride + w];
+}
+
+// Histogram of the binary exponent of |A Ux Uy Uz| over records that passed
+// both screens (LDS per block, then one global atomic per non-empty bin).
+constexpr int kMagBins = 2100;  // ilogb in [-1075, 1024]
+constexpr int kMagOff = 1075;
+
+__global__ __launch_bounds__(kT) void k_mag_hist(const double *__restrict__ qtmp, int stride,
+                                                 const uint8_t *__restrict__ flag, int64_t n,
+                                                 unsigned long long *__restrict__ hist) {
+  __shared__ unsigned int h[kMagBins];
+  for (int b = threadIdx.x; b < kMagBins; b += kT) h[b] = 0;
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < n; r += (int64_t)gridDim.x * kT) {
+    if (!flag[r]) continue;
+    const double u = qtmp[r * stride + 5];
+    int b = (u > 0.0) ? ilogb(u) + kMagOff : 0;
+    b = b < 0 ? 0 : (b >= kMagBins ? kMagBins - 1 : b);
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kMagBins; b += kT)
+    if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+
+// heavy (|U| >= tau) or light (|U| < tau) among the records that passed
+__global__ __launch_bounds__(kT) void k_split_flags(const double *__restrict__ qtmp, int stride,
+                                                    const uint8_t *__restrict__ pass, int64_t n,
+                                                    double tau, int heavy,
+                                                    uint8_t *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r >= n) return;
+  uint8_t f = 0;
+  if (pass[r]) {
+    const bool big = qtmp[r * stride + 5] >= tau;
+    f = heavy ? big : !big;
+  }
+  out[r] = f;
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -341,14 +379,13 @@ int build_pair_table(inca_ctx *ctx) {
 }
 
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
-  (void)precision;
   hipStream_t st = ctx->stream;
   const int nprim = ctx->nprim;
   const int64_t nrec = ctx->nrec;
   int rc;
-  // stride: 5 header words + up to 3 (L+1) coefficients, even for 16-B rows
+  // stride: header words + up to 3 (L+1) coefficients, even for 16-B rows
   const int la = ctx->lmax_axis;
-  int stride = 5 + 3 * (la + 1);
+  int stride = kQHead + 3 * (la + 1);
   stride += stride & 1;
   ctx->qstride = stride;
   unsigned long long hcnt[4] = {(unsigned long long)nrec, 0, 0, 0};
@@ -384,7 +421,61 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   int64_t q2 = 0;
   if ((rc = compact_flags(ctx, ctx->d_flag, nvalid, ctx->d_idx, &q2))) return rc;
   if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride))) return rc;
-  if (q2 > 0) {
+  ctx->q_fp64 = q2;
+  ctx->mixed_tau = 0.0;
+  if (precision == INCA_MIXED && q2 > 0 && ctx->mixed_tail > 0.0) {
+    // Precision by magnitude: the quartets carrying all but `mixed_tail` of
+    // sum |U| stay FP64, the light tail goes to the FP32 kernel.
+    unsigned long long *d_h = nullptr;
+    INCA_HIP(hipMalloc((void **)&d_h, kMagBins * sizeof(unsigned long long)));
+    std::vector<unsigned long long> h(kMagBins, 0);
+    INCA_HIP(hipMemcpyAsync(d_h, h.data(), kMagBins * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_mag_hist, dim3(1024), dim3(kT), 0, st, ctx->d_qtmp, stride,
+                       ctx->d_flag, nvalid, d_h);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h.data(), d_h, kMagBins * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d_h);
+    if (e != hipSuccess) return hip_fail(ctx, e, "magnitude histogram", __LINE__);
+    // mass of a bin ~ count * 2^(exponent + 1/2)
+    std::vector<double> mass(kMagBins);
+    double total = 0.0;
+    for (int b = 0; b < kMagBins; ++b) {
+      // cap the bin scale so that 2^1024 never appears (empty bins stay 0)
+      mass[b] = h[b] ? (double)h[b] * std::ldexp(1.5, std::min(b - kMagOff, 1000)) : 0.0;
+      total += mass[b];
+    }
+    double cum = 0.0;
+    int cut = 0;
+    while (cut < kMagBins && cum + mass[cut] <= ctx->mixed_tail * total) cum += mass[cut++];
+    const double tau = std::ldexp(1.0, cut - kMagOff);
+    ctx->mixed_tau = tau;
+    uint8_t *d_f2 = nullptr;
+    INCA_HIP(hipMalloc((void **)&d_f2, (size_t)nvalid));
+    int64_t nh = 0, nl = 0;
+    hipLaunchKernelGGL(k_split_flags, dim3(nblk(nvalid)), dim3(kT), 0, st, ctx->d_qtmp, stride,
+                       ctx->d_flag, nvalid, tau, 1, d_f2);
+    rc = compact_flags(ctx, d_f2, nvalid, ctx->d_idx, &nh);
+    if (!rc && nh > 0)
+      hipLaunchKernelGGL(k_gather_records, dim3(nblk(nh * stride)), dim3(kT), 0, st,
+                         ctx->d_qtmp, ctx->d_idx, nh, stride, ctx->d_q);
+    if (!rc) {
+      hipLaunchKernelGGL(k_split_flags, dim3(nblk(nvalid)), dim3(kT), 0, st, ctx->d_qtmp,
+                         stride, ctx->d_flag, nvalid, tau, 0, d_f2);
+      rc = compact_flags(ctx, d_f2, nvalid, ctx->d_idx, &nl);
+    }
+    if (!rc && nl > 0)
+      hipLaunchKernelGGL(k_gather_records, dim3(nblk(nl * stride)), dim3(kT), 0, st,
+                         ctx->d_qtmp, ctx->d_idx, nl, stride, ctx->d_q + nh * stride);
+    e = hipStreamSynchronize(st);
+    (void)hipFree(d_f2);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "mixed split", __LINE__);
+    ctx->q_fp64 = nh;
+  } else if (q2 > 0) {
     hipLaunchKernelGGL(k_gather_records, dim3(nblk(q2 * stride)), dim3(kT), 0, st, ctx->d_qtmp,
                        ctx->d_idx, q2, stride, ctx->d_q);
     INCA_HIP(hipGetLastError());

@@ -123,6 +123,17 @@ int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
 int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh,
                   double *ivec, int64_t *counts);
 
+/* Mixed precision by magnitude: after screening, quartets are ranked by
+ * their screen-2 magnitude |A Ux Uy Uz| (mod_intracule.f90:342); the heaviest
+ * ones, carrying all but `tail` of the summed magnitude, are evaluated in
+ * FP64 and the rest in FP32.  tail = 1 evaluates everything in FP32; the
+ * default is 0.02.  No reference counterpart (the reference is FP64 only). */
+int inca_set_mixed_tail(inca_ctx *ctx, double tail);
+/* Split used by the last INCA_MIXED evaluation: quartets in FP64 and in
+ * FP32, and the magnitude threshold between them. */
+int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
+                         double *tau);
+
 /* Copy Ivec(G) of the last evaluation. */
 int inca_get_ivec(inca_ctx *ctx, double *ivec);
 

@@ -105,3 +105,22 @@ def test_mixed_precision_integral(engine, ch4):
     print("mixed integral rel err %.3e, max pointwise %.3e" % (rel, pt))
     assert rel < 1e-8
     assert pt < 1e-5
+
+
+def test_mixed_tail_sweep(engine, ch4):
+    """Error of the mixed path against FP64 as the FP32 share grows."""
+    _setup(engine, ch4)
+    _multicentre(engine, ch4)
+    engine.evaluate(capi.INCA_FP64, THRESH)
+    t64 = engine.integrate()[0]
+    errs = {}
+    for tail in (1.0, 0.2, 0.05, 0.02, 0.005):
+        engine.set_mixed_tail(tail)
+        engine.evaluate(capi.INCA_MIXED, THRESH)
+        tm = engine.integrate()[0]
+        n64, n32, tau = engine.mixed_split()
+        errs[tail] = abs(tm - t64) / abs(t64)
+        print("tail %-6g fp64 %7d fp32 %7d tau %.3e  integral rel err %.3e" % (
+            tail, n64, n32, tau, errs[tail]))
+    engine.set_mixed_tail(0.02)
+    assert errs[0.02] < 1e-8
