@@ -102,6 +102,11 @@ def main():
     eng.set_quartets(device_ptr=recs.data_ptr(), n=Q)
     log("grid: %d radii, %d points" % (nradi, G))
 
+    # tile autotuning on this grid and these quartets (outside the timed steps)
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    tiles = eng.autotune()
+    log("autotuned points per lane: fp64 %d, fp32 %d" % tiles)
+
     results = {}
     for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
         for _ in range(a.warmup):
@@ -151,6 +156,7 @@ def main():
                         "(%d points), thresh 1e-12" % (sysm.nprim, Q, G),
             "records": Q, "quartets_after_screen1": int(cnt[1]),
             "quartets_computed": q2, "grid_points": G, "flop_per_eval": W,
+            "points_per_lane": {"fp64": tiles[0], "fp32": tiles[1]},
         },
         "roofline": {
             "bound": "valu", "kernel": "k_eval_fp64",

@@ -24,6 +24,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_load_dm2p", "inca_set_quartets", "inca_evaluate", "inca_get_ivec",
            "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
+           "inca_set_tiles", "inca_autotune",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -68,6 +69,8 @@ def load():
     L.inca_get_timings.argtypes = [vp, _dp]
     L.inca_class_histogram.argtypes = [vp, C.c_int32, vp]
     L.inca_set_mixed_tail.argtypes = [vp, C.c_double]
+    L.inca_set_tiles.argtypes = [vp, C.c_int32, C.c_int32]
+    L.inca_autotune.argtypes = [vp, vp]
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
@@ -265,6 +268,14 @@ class Engine:
         self._ck(self._L.inca_get_mixed_split(self._h, C.addressof(a), C.addressof(b),
                                               C.addressof(t)), "inca_get_mixed_split")
         return a.value, b.value, t.value
+
+    def set_tiles(self, p64, p32):
+        self._ck(self._L.inca_set_tiles(self._h, int(p64), int(p32)), "inca_set_tiles")
+
+    def autotune(self):
+        best = np.zeros(2, np.int32)
+        self._ck(self._L.inca_autotune(self._h, best.ctypes.data), "inca_autotune")
+        return int(best[0]), int(best[1])
 
     def timings(self):
         t = np.zeros(3)

@@ -351,6 +351,21 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32, double
   return 0;
 }
 
+int inca_set_tiles(inca_ctx *ctx, int32_t p64, int32_t p32) {
+  CHECK_CTX(ctx);
+  auto ok = [](int p) { return p == 1 || p == 2 || p == 4 || p == 8; };
+  if (!ok(p64) || !ok(p32)) return set_err(ctx, INCA_EINVAL, "points per lane must be 1, 2, 4 or 8");
+  ctx->tile_p[0] = p64;
+  ctx->tile_p[1] = p32;
+  return 0;
+}
+
+int inca_autotune(inca_ctx *ctx, int32_t *best) {
+  CHECK_CTX(ctx);
+  INCA_HIP(hipSetDevice(ctx->device));
+  return autotune(ctx, best);
+}
+
 int inca_get_ivec(inca_ctx *ctx, double *ivec) {
   CHECK_CTX(ctx);
   if (!ctx->have_ivec) return set_err(ctx, INCA_EINVAL, "no Ivec: call inca_evaluate first");

@@ -75,6 +75,7 @@ struct inca_ctx {
   int64_t q_fp64 = 0;         // mixed mode: leading records evaluated in FP64
   double mixed_tail = 0.02;   // share of sum |U| left to the FP32 kernel
   double mixed_tau = 0.0;     // |U| threshold chosen for the split
+  int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   double *d_q = nullptr;      // compacted prepared records
   size_t q_cap = 0;
   double *d_qtmp = nullptr;   // uncompacted records (slot = record index)
@@ -163,5 +164,6 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision,
 // launchers (eval.hip)
 int evaluate_grid(inca_ctx *ctx, int precision);
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist);
+int autotune(inca_ctx *ctx, int32_t *best);
 
 }  // namespace inca

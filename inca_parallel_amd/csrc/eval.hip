@@ -278,6 +278,47 @@ int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
   return 0;
 }
 
+namespace {
+
+// Launch one evaluation kernel over quartets [q, q+n) in `nch` chunks of
+// `qch` quartets with P points per lane; partial sums go to partial[chunk][g].
+int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, int64_t qch,
+                int64_t nch, double *partial) {
+  const int64_t G = ctx->G;
+  const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  dim3 grid((unsigned)tiles, (unsigned)nch);
+  hipStream_t st = ctx->stream;
+#define INCA_EVAL(PP)                                                                          \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL(k_eval_mixed<PP>, grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,       \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, partial);                            \
+  else                                                                                         \
+    hipLaunchKernelGGL(k_eval_fp64<PP>, grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,        \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, partial);
+  switch (P) {
+    case 1: INCA_EVAL(1) break;
+    case 2: INCA_EVAL(2) break;
+    case 8: INCA_EVAL(8) break;
+    default: INCA_EVAL(4) break;
+  }
+#undef INCA_EVAL
+  INCA_HIP(hipGetLastError());
+  return 0;
+}
+
+// chunking of n quartets: enough (tile, chunk) blocks to fill 256 CUs
+// several times over (`share` of the 8192-block budget)
+void chunking(int64_t n, int64_t tiles, double share, int64_t *qch, int64_t *nch) {
+  int64_t want = (int64_t)(8192.0 * share) / tiles + 1;
+  if (want > n) want = n;
+  if (want > 65535) want = 65535;
+  if (want < 1) want = 1;
+  *qch = (n + want - 1) / want;
+  *nch = (n + *qch - 1) / *qch;
+}
+
+}  // namespace
+
 int evaluate_grid(inca_ctx *ctx, int precision) {
   hipStream_t st = ctx->stream;
   const int64_t G = ctx->G, nq = ctx->q2;
@@ -295,33 +336,26 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
     ctx->have_ivec = 1;
     return 0;
   }
-  constexpr int P = 4;
-  const int64_t tiles = (G + kT * P - 1) / (kT * P);
   // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
   const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
   const int64_t nr[2] = {n64, nq - n64};
   int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
   for (int r = 0; r < 2; ++r) {
     if (nr[r] == 0) continue;
-    // enough (tile, chunk) blocks to fill 256 CUs several times over,
-    // split between the two ranges by their quartet counts
-    int64_t want = (int64_t)(8192.0 * (double)nr[r] / (double)nq) / tiles + 1;
-    if (want > nr[r]) want = nr[r];
-    if (want > 65535) want = 65535;
-    qch[r] = (nr[r] + want - 1) / want;
-    nch[r] = (nr[r] + qch[r] - 1) / qch[r];
+    const int P = ctx->tile_p[r];
+    const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+    chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
   }
   const int64_t nchunks = nch[0] + nch[1];
   if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
-  if (nch[0] > 0)
-    hipLaunchKernelGGL(k_eval_fp64<P>, dim3((unsigned)tiles, (unsigned)nch[0]), dim3(kT), 0, st,
-                       ctx->d_q, ctx->qstride, nr[0], qch[0], ctx->d_gx, ctx->d_gy, ctx->d_gz,
-                       G, ctx->d_partial);
-  if (nch[1] > 0)
-    hipLaunchKernelGGL(k_eval_mixed<P>, dim3((unsigned)tiles, (unsigned)nch[1]), dim3(kT), 0, st,
-                       ctx->d_q + n64 * ctx->qstride, ctx->qstride, nr[1], qch[1], ctx->d_gx,
-                       ctx->d_gy, ctx->d_gz, G, ctx->d_partial + nch[0] * G);
-  INCA_HIP(hipGetLastError());
+  if (nch[0] > 0 &&
+      (rc = launch_eval(ctx, false, ctx->tile_p[0], ctx->d_q, nr[0], qch[0], nch[0],
+                        ctx->d_partial)))
+    return rc;
+  if (nch[1] > 0 &&
+      (rc = launch_eval(ctx, true, ctx->tile_p[1], ctx->d_q + n64 * ctx->qstride, nr[1], qch[1],
+                        nch[1], ctx->d_partial + nch[0] * G)))
+    return rc;
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
                      ctx->d_partial, (int)nchunks, G, ctx->d_ivec);
@@ -329,6 +363,48 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   INCA_HIP(hipEventRecord(ctx->ev[3], st));
   ctx->have_ivec = 1;
   return 0;
+}
+
+// Time every points-per-lane choice for both kernels on a slice of the
+// prepared quartets of the current grid and keep the fastest.
+int autotune(inca_ctx *ctx, int32_t *best) {
+  const int64_t G = ctx->G, nq = ctx->q2;
+  if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
+  const int cand[4] = {1, 2, 4, 8};
+  const int64_t n = nq < 262144 ? nq : 262144;
+  hipEvent_t a, b;
+  INCA_HIP(hipEventCreate(&a));
+  INCA_HIP(hipEventCreate(&b));
+  int rc = 0;
+  for (int kind = 0; kind < 2 && !rc; ++kind) {
+    float best_ms = 1e30f;
+    int best_p = 4;
+    for (int c = 0; c < 4 && !rc; ++c) {
+      const int P = cand[c];
+      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      int64_t qch, nch;
+      chunking(n, tiles, 1.0, &qch, &nch);
+      if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nch * G)))) break;
+      float ms = 0.f;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {  // first launch warms up
+        INCA_HIP(hipEventRecord(a, ctx->stream));
+        rc = launch_eval(ctx, kind == 1, P, ctx->d_q, n, qch, nch, ctx->d_partial);
+        INCA_HIP(hipEventRecord(b, ctx->stream));
+        INCA_HIP(hipEventSynchronize(b));
+        INCA_HIP(hipEventElapsedTime(&ms, a, b));
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_p = P;
+      }
+    }
+    ctx->tile_p[kind] = best_p;
+    if (best) best[kind] = best_p;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->have_ivec = 0;  // the partial buffer no longer matches Ivec
+  return rc;
 }
 
 }  // namespace inca

@@ -134,6 +134,15 @@ int inca_set_mixed_tail(inca_ctx *ctx, double tail);
 int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
                          double *tau);
 
+/* Evaluation tiles: grid points per lane (1, 2, 4 or 8) of the FP64 and of
+ * the FP32 kernel.  inca_autotune times each choice on a slice of the
+ * quartets prepared by the last inca_evaluate, on the current grid, keeps
+ * the fastest and returns it in best[0] (FP64) and best[1] (FP32).  It
+ * overwrites the partial sums, so call inca_evaluate again before reading
+ * Ivec.  No reference counterpart. */
+int inca_set_tiles(inca_ctx *ctx, int32_t points_fp64, int32_t points_fp32);
+int inca_autotune(inca_ctx *ctx, int32_t *best);
+
 /* Copy Ivec(G) of the last evaluation. */
 int inca_get_ivec(inca_ctx *ctx, double *ivec);
 
