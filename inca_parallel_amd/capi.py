@@ -24,7 +24,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_load_dm2p", "inca_set_quartets", "inca_evaluate", "inca_get_ivec",
            "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
-           "inca_set_tiles", "inca_autotune",
+           "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -71,6 +71,7 @@ def load():
     L.inca_set_mixed_tail.argtypes = [vp, C.c_double]
     L.inca_set_tiles.argtypes = [vp, C.c_int32, C.c_int32]
     L.inca_autotune.argtypes = [vp, vp]
+    L.inca_get_prepared.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
@@ -276,6 +277,18 @@ class Engine:
         best = np.zeros(2, np.int32)
         self._ck(self._L.inca_autotune(self._h, best.ctypes.data), "inca_autotune")
         return int(best[0]), int(best[1])
+
+    def prepared(self, first=0, n=None):
+        """Prepared quartet records as a (n, stride) float64 array."""
+        st = C.c_int32()
+        self._ck(self._L.inca_get_prepared(self._h, 0, 0, None, C.addressof(st)),
+                 "inca_get_prepared")
+        if n is None:
+            n = self.mixed_split()[0] + self.mixed_split()[1] - first
+        out = np.zeros((max(n, 1), st.value))
+        self._ck(self._L.inca_get_prepared(self._h, int(first), int(n), out.ctypes.data,
+                                           C.addressof(st)), "inca_get_prepared")
+        return out[:n]
 
     def timings(self):
         t = np.zeros(3)

@@ -366,6 +366,17 @@ int inca_autotune(inca_ctx *ctx, int32_t *best) {
   return autotune(ctx, best);
 }
 
+int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out, int32_t *stride) {
+  CHECK_CTX(ctx);
+  if (stride) *stride = ctx->qstride;
+  if (first < 0 || n < 0 || first + n > ctx->q2)
+    return set_err(ctx, INCA_EINVAL, "prepared-record range out of bounds");
+  if (n > 0 && out)
+    INCA_HIP(hipMemcpy(out, ctx->d_q + first * ctx->qstride, n * ctx->qstride * sizeof(double),
+                       hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int inca_get_ivec(inca_ctx *ctx, double *ivec) {
   CHECK_CTX(ctx);
   if (!ctx->have_ivec) return set_err(ctx, INCA_EINVAL, "no Ivec: call inca_evaluate first");

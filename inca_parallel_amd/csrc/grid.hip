@@ -13,6 +13,11 @@
 
 #include "common.hpp"
 
+// Grid arithmetic keeps separate multiply and add roundings: the TFVC step
+// function cancels near nu = 1, and the w > 1e-106 filter must decide on the
+// same values as the FP64 restatement.
+#pragma clang fp contract(off)
+
 namespace inca {
 namespace {
 

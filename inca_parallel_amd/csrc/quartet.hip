@@ -19,6 +19,12 @@
 
 #include "common.hpp"
 
+// The pre-pass follows the reference's operation sequence (separate multiply
+// and add roundings, pow for integer powers) so that its coefficients agree
+// with the FP64 restatement to the last bits where the Vandermonde systems
+// are well conditioned and stay as close as roundoff allows where not.
+#pragma clang fp contract(off)
+
 namespace inca {
 namespace {
 
@@ -52,9 +58,12 @@ const double kGhRoot[11][11] = {
      -0.65680956688209979, 0, 0.65680956688209979, 1.3265570844949328, 2.0259480158257555,
      2.7832900997816514, 3.6684708465595826}};
 
+// Integer power x^e by e-1 successive multiplications.  The reference writes
+// x**dble(e) (libm pow); the FP64 restatement uses this same sequence, so the
+// Vandermonde systems the LU sees are bitwise those of the restatement.
 __device__ __forceinline__ double ipow(double x, int e) {
   double r = 1.0;
-  for (int k = 0; k < e; ++k) r *= x;
+  for (int k = 0; k < e; ++k) r = (k == 0) ? x : r * x;
   return r;
 }
 
@@ -84,7 +93,11 @@ __global__ __launch_bounds__(kT) void k_pair_table(int nprim, const double *__re
   const double x0 = gh_xmax(ti + tk + 1), x1 = gh_xmax(mi + mk + 1), x2 = gh_xmax(ni + nk + 1);
   const double L = (double)(ti + tk + mi + mk + ni + nk);
   const double adx = fabs(dx), ady = fabs(dy), adz = fabs(dz);
-  double J = pow(M_PI, 1.5) * pow(2.0 * aik, -(L + 1.5)) * exp(-2.0 * eik * rik2);
+  // pi^(3/2) (2 aik)^-(L+3/2) as a literal and a division by an integer
+  // power times a square root (no pow with a fractional exponent)
+  const double two_a = 2.0 * aik;
+  double J = 5.568327996831708 * (1.0 / (ipow(two_a, (int)L + 1) * sqrt(two_a))) *
+             exp(-2.0 * eik * rik2);
   J *= ipow(x0 + alfk * s * adx, 2 * ti) * ipow(x0 + alfi * s * adx, 2 * tk);
   J *= ipow(x1 + alfk * s * ady, 2 * mi) * ipow(x1 + alfi * s * ady, 2 * mk);
   J *= ipow(x2 + alfk * s * adz, 2 * ni) * ipow(x2 + alfi * s * adz, 2 * nk);
@@ -125,12 +138,8 @@ __device__ int polycoef(const double R[4], double Rik, double Rjl, double Rijkl,
   for (int i = 0; i < np; ++i) {
     const double xx = -0.5 + (double)(i + 1);
     const double y = sqe * (xx + Rik - Rjl);
-    // M(i, j) = y^(L - j): fill from the last column (power 0) leftwards
-    double pw = 1.0;
-    for (int j = L; j >= 0; --j) {
-      M[i + np * j] = pw;
-      pw *= y;
-    }
+    // M(i, j) = y^(L - j)  (mod_intrastuff.f90:151-157)
+    for (int j = 0; j <= L; ++j) M[i + np * j] = ipow(y, L - j);
     const double am = (alf - 0.5) * xx, ap = (alf + 0.5) * xx;
     double c = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(kT) void k_prep(
       const double sqe = sqrt(eijkl);
       const double invz = 1.0 / sqrt(zeta);
       const double alf = 0.5 * (aik - ajl) * (1.0 / zeta);
-      const double A = DMval * pow(zeta, -1.5) * exp(-pik.eR2 - pjl.eR2);
+      const double A = DMval * (1.0 / (zeta * sqrt(zeta))) * exp(-pik.eR2 - pjl.eR2);
       double *out = qout + n * (int64_t)qstride;
       int off = kQHead;
       int Ls[3];

@@ -143,6 +143,15 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
 int inca_set_tiles(inca_ctx *ctx, int32_t points_fp64, int32_t points_fp32);
 int inca_autotune(inca_ctx *ctx, int32_t *best);
 
+/* Diagnostics: copy prepared quartet records [first, first+n) of the last
+ * inca_evaluate (FP64 range first, then the FP32 range in mixed mode).
+ * Record layout: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
+ * Lx | Ly<<8 | Lz<<16 (as integer bits), [5] |A Ux Uy Uz|, then A*Cx(0..Lx),
+ * Cy(0..Ly), Cz(0..Lz); *stride receives the record length in doubles.
+ * No reference counterpart. */
+int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out,
+                      int32_t *stride);
+
 /* Copy Ivec(G) of the last evaluation. */
 int inca_get_ivec(inca_ctx *ctx, double *ivec);
 

@@ -6,19 +6,19 @@
  *
  * Reference: markelgtx/INCA_parallel, Fortran 90 + OpenMP.  Each function
  * cites the routine it restates.  The restatement keeps the reference's
- * arithmetic (pow() where the reference writes x**dble(n), the same
- * Vandermonde + partial-pivot LU as reference LAPACK dgesv, the same
- * thresholds and filters), so that on well-conditioned inputs it agrees
+ * arithmetic (the same Vandermonde + partial-pivot LU as reference LAPACK
+ * dgesv, the same thresholds and filters; integer powers by repeated
+ * multiplication, see ipow), so that on well-conditioned inputs it agrees
  * with the reference to rounding.
  *
- * How each part is pinned (tests/test_oracle_*.py):
+ * How each part is pinned (tests/test_oracle.py):
  *  - whole path: the reference's CH4 goldens (integral, I(0), two points,
  *    the 101-row .rad scan and all 8000 cube voxels) at <= 1e-10 relative;
  *  - normprim: unit self-overlap of every normalised primitive;
  *  - gauherm: numpy.polynomial.hermite.hermgauss nodes and weights;
- *  - polycoef + grid loop: closed form for s-only quartets, per-quartet
- *    sum rule  int f_q(u) d3u = S_ik S_jl,  brute-force 1D quadrature of
- *    each axis factor for p/d quartets;
+ *  - polycoef + grid loop: closed form for s-only quartets, the per-quartet
+ *    sum rule  int f_q(u) d3u = S_ik S_jl  for s/p quartets, and an exact
+ *    polynomial expansion for s/p/d quartets;
  *  - gauleg: numpy leggauss; lebedev (input): exactness on harmonics;
  *  - TFVC: partition of unity, inversion symmetry, integrals of normalised
  *    Gaussians;
@@ -38,6 +38,17 @@
 #endif
 
 static const double PI = 3.14159265358979323846;
+
+/* The reference writes integer powers as x**dble(n) (libm pow).  Here they
+ * are n-1 successive multiplications: libm pow is correctly rounded, so the
+ * two differ by a few ulp at most, which matters only where the Vandermonde
+ * fit is ill-conditioned -- and there the reference's own output depends on
+ * its libm, BLAS and compiler.  Fractional powers are written with sqrt. */
+static double ipow(double x, int e) {
+  double r = 1.0;
+  for (int k = 0; k < e; ++k) r = (k == 0) ? x : r * x;
+  return r;
+}
 
 /* Gauss-Hermite roots for n = 1..11 nodes (ascending).  Values are the
  * roots of H_n computed with numpy.polynomial.hermite.hermgauss; they are
@@ -138,12 +149,13 @@ double orc_ja8(int ti, int tk, int mi, int mk, int ni, int nk, double alfi,
   double s = sqrt(2.0 / aik);
   double L = (double)(ti + tk + mi + mk + ni + nk);
   double dx = fabs(xk - xi), dy = fabs(yk - yi), dz = fabs(zk - zi);
-  return pow(PI, 1.5) * pow(2.0 * aik, -(L + 1.5)) * exp(-2.0 * eik * rik2) *
-         pow(xm0 + alfk * s * dx, 2.0 * ti) *
-         pow(xm0 + alfi * s * dx, 2.0 * tk) *
-         pow(xm1 + alfk * s * dy, 2.0 * mi) *
-         pow(xm1 + alfi * s * dy, 2.0 * mk) *
-         pow(xm2 + alfk * s * dz, 2.0 * ni) * pow(xm2 + alfi * s * dz, 2.0 * nk);
+  const double two_a = 2.0 * aik;
+  double J = 5.568327996831708 * (1.0 / (ipow(two_a, (int)L + 1) * sqrt(two_a))) *
+             exp(-2.0 * eik * rik2);
+  J *= ipow(xm0 + alfk * s * dx, 2 * ti) * ipow(xm0 + alfi * s * dx, 2 * tk);
+  J *= ipow(xm1 + alfk * s * dy, 2 * mi) * ipow(xm1 + alfi * s * dy, 2 * mk);
+  J *= ipow(xm2 + alfk * s * dz, 2 * ni) * ipow(xm2 + alfi * s * dz, 2 * nk);
+  return J;
 }
 
 /* Partial-pivot LU solve of a column-major np x np system, in the operation
@@ -213,10 +225,8 @@ static double wijkl(double ri, double rj, double rk, double rl, double rijkl,
                     double invz, double alf) {
   double a = invz * rhat;
   double ap = (alf + 0.5) * r, am = (alf - 0.5) * r;
-  return pow(a + am + (rijkl - ri), (double)ti) *
-         pow(a + ap + (rijkl - rj), (double)tj) *
-         pow(a + am + (rijkl - rk), (double)tk) *
-         pow(a + ap + (rijkl - rl), (double)tl);
+  return ipow(a + am + (rijkl - ri), ti) * ipow(a + ap + (rijkl - rj), tj) *
+         ipow(a + am + (rijkl - rk), tk) * ipow(a + ap + (rijkl - rl), tl);
 }
 
 /* mod_intrastuff.f90:127-175 */
@@ -239,7 +249,7 @@ int orc_polycoef(double ri, double rj, double rk, double rl, double rik,
   for (int i = 0; i < np; ++i) {
     int l1 = L;
     for (int j = 0; j < np; ++j) {
-      M[i + np * j] = pow(sqe * (xx[i] + rik - rjl), (double)l1);
+      M[i + np * j] = ipow(sqe * (xx[i] + rik - rjl), l1);
       --l1;
     }
   }
@@ -312,7 +322,7 @@ void orc_prepare_quartet(int nprim, const double *alpha, const int32_t *tmn,
   }
   double invz = 1.0 / sqrt(zeta);
   double alfijkl = 0.5 * (aik - ajl) * (1.0 / zeta);
-  double A = DMval * pow(zeta, -1.5) * exp(-eik * rik2 - ejl * rjl2);
+  double A = DMval * (1.0 / (zeta * sqrt(zeta))) * exp(-eik * rik2 - ejl * rjl2);
   double U[3];
   for (int ax = 0; ax < 3; ++ax) {
     int L = t[0][ax] + t[1][ax] + t[2][ax] + t[3][ax];

@@ -76,6 +76,31 @@ def lib():
     return L
 
 
+class OrcQuartet(C.Structure):
+    _fields_ = [("A", C.c_double), ("sqe", C.c_double), ("dx", C.c_double),
+                ("dy", C.c_double), ("dz", C.c_double), ("screen1", C.c_double),
+                ("screen2", C.c_double), ("L", C.c_int * 3), ("pass1", C.c_int),
+                ("pass2", C.c_int), ("polyfail", C.c_int), ("C", (C.c_double * 22) * 3)]
+
+
+def prepare_quartet(prims, q, D, thresh, normalize=True):
+    """One record through screens + prefactors + polycoef (orc_prepare_quartet)."""
+    L = lib()
+    L.orc_prepare_quartet.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp, C.c_void_p,
+                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.POINTER(OrcQuartet)]
+    alpha = _f64(prims["alpha"])
+    n = len(alpha)
+    tmn = _i32(np.asarray(prims["tmn"]).reshape(n, 3).T.reshape(-1))
+    nrm = _f64(normprim(alpha, np.asarray(prims["tmn"]).reshape(n, 3))) if normalize else None
+    lim = thresh * (1.0 / (n * (n + 1.0) * 0.5))
+    out = OrcQuartet()
+    L.orc_prepare_quartet(n, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]), _f64(prims["z"]),
+                          None if nrm is None else nrm.ctypes.data, lim, int(q[0]), int(q[1]),
+                          int(q[2]), int(q[3]), float(D), C.byref(out))
+    return out
+
+
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
