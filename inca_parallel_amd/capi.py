@@ -25,6 +25,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
+           "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -72,7 +73,10 @@ def load():
     L.inca_set_tiles.argtypes = [vp, C.c_int32, C.c_int32]
     L.inca_autotune.argtypes = [vp, vp]
     L.inca_get_prepared.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
+    L.inca_set_polynomials.argtypes = [vp, C.c_int32]
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
+    L.inca_get_analytic_integral.argtypes = [vp, vp]
+    L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
     for s in SYMBOLS:
@@ -269,6 +273,22 @@ class Engine:
         self._ck(self._L.inca_get_mixed_split(self._h, C.addressof(a), C.addressof(b),
                                               C.addressof(t)), "inca_get_mixed_split")
         return a.value, b.value, t.value
+
+    def analytic_integral(self):
+        """Sum of the prepared terms' integrals over all space."""
+        t = C.c_double()
+        self._ck(self._L.inca_get_analytic_integral(self._h, C.addressof(t)),
+                 "inca_get_analytic_integral")
+        return t.value
+
+    def set_point_cutoff(self, log2_rel):
+        """0: evaluate every quartet at every point; else skip tiles beyond
+        the 2^-log2_rel radius of each quartet."""
+        self._ck(self._L.inca_set_point_cutoff(self._h, float(log2_rel)),
+                 "inca_set_point_cutoff")
+
+    def set_polynomials(self, exact):
+        self._ck(self._L.inca_set_polynomials(self._h, int(bool(exact))), "inca_set_polynomials")
 
     def set_tiles(self, p64, p32):
         self._ck(self._L.inca_set_tiles(self._h, int(p64), int(p32)), "inca_set_tiles")

@@ -76,7 +76,8 @@ int inca_destroy(inca_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->d_prim, c->d_tmn, c->d_pair, c->d_gx, c->d_gy, c->d_gz, c->d_w,
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
-                  c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec};
+                  c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
+                  c->d_gcent};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
@@ -348,6 +349,28 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32, double
   if (n_fp64) *n_fp64 = ctx->q_fp64;
   if (n_fp32) *n_fp32 = ctx->q2 - ctx->q_fp64;
   if (tau) *tau = ctx->mixed_tau;
+  return 0;
+}
+
+int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel) {
+  CHECK_CTX(ctx);
+  if (!(log2_rel == 0.0 || (log2_rel >= 64.0 && log2_rel <= 1000.0)))
+    return set_err(ctx, INCA_EINVAL, "point cutoff must be 0 (off) or in [64, 1000]");
+  ctx->cut_log2 = log2_rel;
+  return 0;
+}
+
+int inca_get_analytic_integral(inca_ctx *ctx, double *total) {
+  CHECK_CTX(ctx);
+  if (!total) return set_err(ctx, INCA_EINVAL, "null output");
+  *total = ctx->term_integral;
+  return 0;
+}
+
+int inca_set_polynomials(inca_ctx *ctx, int32_t mode) {
+  CHECK_CTX(ctx);
+  if (mode != 0 && mode != 1) return set_err(ctx, INCA_EINVAL, "polynomial mode must be 0 or 1");
+  ctx->poly_exact = mode;
   return 0;
 }
 

@@ -16,9 +16,12 @@ namespace inca {
 constexpr int kLmaxAxis = 21;
 constexpr int kNpMax = kLmaxAxis + 1;
 // Prepared-quartet record: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
-// Lx | Ly<<8 | Lz<<16, [5] |A Ux Uy Uz| (the screen-2 magnitude), then the
-// coefficients A*Cx(0..Lx), Cy(0..Ly), Cz(0..Lz) (C(0) = highest power).
-constexpr int kQHead = 6;
+// Lx | Ly<<8 | Lz<<16, [5] B = |A| prod_axis sum_k |C_k| Gamma((k+1)/2) / sqe,
+// a bound on the L1 norm of the term over all space, [6] rcut: beyond
+// |rp| = rcut the term stays below 2^-cut_log2 of its peak bound (+inf when
+// the cutoff is off), [7] unused, then the coefficients A*Cx(0..Lx),
+// Cy(0..Ly), Cz(0..Lz) (C(0) = highest power).
+constexpr int kQHead = 8;
 
 enum GridKind { GRID_NONE = 0, GRID_MULTI = 1, GRID_RADIAL = 2, GRID_CUBE = 3, GRID_POINTS = 4 };
 
@@ -73,9 +76,12 @@ struct inca_ctx {
   int qstride = 0;       // doubles per prepared record
   int64_t q2 = 0;
   int64_t q_fp64 = 0;         // mixed mode: leading records evaluated in FP64
-  double mixed_tail = 0.02;   // share of sum |U| left to the FP32 kernel
-  double mixed_tau = 0.0;     // |U| threshold chosen for the split
+  double mixed_tail = 0.02;   // FP32 tail budget: sum of its L1 bounds / |integral|
+  double mixed_tau = 0.0;     // L1-bound threshold chosen for the split
+  double term_integral = 0.0; // sum of the prepared terms' integrals over all space
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
+  int poly_exact = 0;         // 1: exact axis polynomials instead of the fits
+  double cut_log2 = 0.0;      // per-point cutoff 2^-cut_log2 of a term's peak (0: off)
   double *d_q = nullptr;      // compacted prepared records
   size_t q_cap = 0;
   double *d_qtmp = nullptr;   // uncompacted records (slot = record index)
@@ -91,6 +97,9 @@ struct inca_ctx {
   int64_t *d_bc = nullptr;
   size_t bc_cap = 0;
   unsigned long long *d_cnt = nullptr;  // device counters
+  double *d_tb = nullptr;     // per wave tile: anchor xyz, rho min/max, flag
+  size_t tb_cap = 0;
+  double *d_gcent = nullptr;  // multicentre grid: centre coordinates (3*nquad)
   double *d_partial = nullptr;
   size_t partial_cap = 0;
   double *d_ivec = nullptr;

@@ -55,7 +55,7 @@ __device__ __forceinline__ double exp_neg(double s) {
 }
 
 struct QHeader {
-  double sqe, ax, ay, az;
+  double sqe, ax, ay, az, rcut;
   int Lx, Ly, Lz;
 };
 
@@ -65,6 +65,7 @@ __device__ __forceinline__ QHeader load_header(const double *__restrict__ rec) {
   h.ax = rec[1];
   h.ay = rec[2];
   h.az = rec[3];
+  h.rcut = rec[6];
   const unsigned cls = (unsigned)__double_as_longlong(rec[4]);
   h.Lx = cls & 0xff;
   h.Ly = (cls >> 8) & 0xff;
@@ -72,18 +73,124 @@ __device__ __forceinline__ QHeader load_header(const double *__restrict__ rec) {
   return h;
 }
 
-template <int P>
+// Tile bounds.  Wave w of block b owns the 64 P consecutive grid points
+// from (4 b + w) 64 P on.  For each such tile: an anchor c and the range
+// [rho_min, rho_max] of |r - c| over its points (words 0..4), and whether
+// it holds any point (word 5).  Anchors: the origin for radial scans (the
+// points lie on spheres about it), the owning centre for multicentre grids
+// when the whole tile belongs to one centre, else the tile's centroid.
+constexpr int kTB = 8;  // doubles per tile record
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kT) void k_tile_bounds(const double *__restrict__ gx,
+                                                    const double *__restrict__ gy,
+                                                    const double *__restrict__ gz, int64_t G,
+                                                    int P, int mode,
+                                                    const int32_t *__restrict__ map,
+                                                    const double *__restrict__ gcent,
+                                                    int64_t ntile, double *__restrict__ tb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (t >= ntile) return;  // uniform per wave
+  const int64_t g0 = t * 64 * P;
+  const int first = (mode == 1 && g0 < G) ? map[g0] : 0;
+  double sx = 0.0, sy = 0.0, sz = 0.0, n = 0.0;
+  int same = 1;
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = g0 + (int64_t)k * 64 + lane;
+    if (g < G) {
+      sx += gx[g];
+      sy += gy[g];
+      sz += gz[g];
+      n += 1.0;
+      if (mode == 1 && map[g] != first) same = 0;
+    }
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  n = wave_sum(n);
+  same = __all(same);
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if (mode == 1 && same && first > 0) {
+    cx = gcent[3 * (first - 1)];
+    cy = gcent[3 * (first - 1) + 1];
+    cz = gcent[3 * (first - 1) + 2];
+  } else if (mode != 0 && n > 0.0) {
+    cx = sx / n;
+    cy = sy / n;
+    cz = sz / n;
+  }
+  double lo = HUGE_VAL, hi = 0.0;
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = g0 + (int64_t)k * 64 + lane;
+    if (g < G) {
+      const double dx = gx[g] - cx, dy = gy[g] - cy, dz = gz[g] - cz;
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      lo = fmin(lo, d);
+      hi = fmax(hi, d);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, d, 64));
+    hi = fmax(hi, __shfl_xor(hi, d, 64));
+  }
+  if (lane == 0) {
+    double *o = tb + t * kTB;
+    o[0] = cx;
+    o[1] = cy;
+    o[2] = cz;
+    // widened by far more than the roundoff of the distances
+    o[3] = n > 0.0 ? lo * (1.0 - 1e-12) : 0.0;
+    o[4] = n > 0.0 ? hi * (1.0 + 1e-12) + 1e-300 : 0.0;
+    o[5] = n;
+    o[6] = 0.0;
+    o[7] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_fill_tiles(int64_t ntile, double *__restrict__ tb) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ntile) return;
+  for (int w = 0; w < kTB; ++w) tb[t * kTB + w] = (w == 5) ? 1.0 : 0.0;
+}
+
+// True when every point of the tile lies beyond the quartet's cutoff radius:
+// |rp| = |sqe (r - c) + u|, u = sqe c + sqe d, is at least
+// max(|u| - sqe rho_max, sqe rho_min - |u|) on the tile.
+__device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy, double cz,
+                                         double rlo, double rhi) {
+  const double ux = fma(h.sqe, cx, h.ax), uy = fma(h.sqe, cy, h.ay), uz = fma(h.sqe, cz, h.az);
+  const double u2 = fma(ux, ux, fma(uy, uy, uz * uz));
+  const double o = fma(h.sqe, rhi, h.rcut);
+  const double i = fma(h.sqe, rlo, -h.rcut);
+  return (u2 >= o * o) || (i > 0.0 && u2 <= i * i);
+}
+
+template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
                                                   const double *__restrict__ gy,
                                                   const double *__restrict__ gz, int64_t G,
+                                                  const double *__restrict__ tb,
                                                   double *__restrict__ partial) {
-  const int64_t base = (int64_t)blockIdx.x * (kT * P) + threadIdx.x;
+  const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const double *__restrict__ tbw = tb + tile * kTB;
+  const double tcx = tbw[0], tcy = tbw[1], tcz = tbw[2], trlo = tbw[3], trhi = tbw[4];
+  const bool empty = !(tbw[5] > 0.0);
   double x[P], y[P], z[P], acc[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * kT;
+    const int64_t g = base + (int64_t)k * 64;
     const bool ok = g < G;
     x[k] = ok ? gx[g] : 0.0;
     y[k] = ok ? gy[g] : 0.0;
@@ -91,10 +198,12 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
     acc[k] = 0.0;
   }
   const int64_t q0 = (int64_t)blockIdx.y * qchunk;
-  const int64_t q1 = q0 + qchunk < nq ? q0 + qchunk : nq;
+  const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for (int64_t qi = q0; qi < q1; ++qi) {
     const double *__restrict__ rec = q + qi * qstride;
     const QHeader h = load_header(rec);
+    if (CUT && __builtin_amdgcn_readfirstlane((int)tile_far(h, tcx, tcy, tcz, trlo, trhi)))
+      continue;
     const double *__restrict__ cx = rec + kQHead;
     const double *__restrict__ cy = cx + h.Lx + 1;
     const double *__restrict__ cz = cy + h.Ly + 1;
@@ -131,7 +240,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   }
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * kT;
+    const int64_t g = base + (int64_t)k * 64;
     if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
   }
 }
@@ -140,18 +249,24 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
 // kMixTile quartets folded into FP64.
 constexpr int kMixTile = 128;
 
-template <int P>
+template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q, int qstride,
                                                    int64_t nq, int64_t qchunk,
                                                    const double *__restrict__ gx,
                                                    const double *__restrict__ gy,
                                                    const double *__restrict__ gz, int64_t G,
+                                                   const double *__restrict__ tb,
                                                    double *__restrict__ partial) {
-  const int64_t base = (int64_t)blockIdx.x * (kT * P) + threadIdx.x;
+  const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const double *__restrict__ tbw = tb + tile * kTB;
+  const double tcx = tbw[0], tcy = tbw[1], tcz = tbw[2], trlo = tbw[3], trhi = tbw[4];
+  const bool empty = !(tbw[5] > 0.0);
   double x[P], y[P], z[P], acc[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * kT;
+    const int64_t g = base + (int64_t)k * 64;
     const bool ok = g < G;
     x[k] = ok ? gx[g] : 0.0;
     y[k] = ok ? gy[g] : 0.0;
@@ -159,7 +274,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
     acc[k] = 0.0;
   }
   const int64_t q0 = (int64_t)blockIdx.y * qchunk;
-  const int64_t q1 = q0 + qchunk < nq ? q0 + qchunk : nq;
+  const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   // -log2(e) split so that s*log2(e) is formed to FP32 rounding without the
   // 1.3e-8 relative bias of a single-float constant
   const float kNL2eHi = -1.44269502162933349609375f;
@@ -175,6 +290,8 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
     for (int64_t qi = t0; qi < t1; ++qi) {
       const double *__restrict__ rec = q + qi * qstride;
       const QHeader h = load_header(rec);
+      if (CUT && __builtin_amdgcn_readfirstlane((int)tile_far(h, tcx, tcy, tcz, trlo, trhi)))
+        continue;
       const double *__restrict__ cx = rec + kQHead;
       const double *__restrict__ cy = cx + h.Lx + 1;
       const double *__restrict__ cz = cy + h.Ly + 1;
@@ -220,7 +337,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
   }
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * kT;
+    const int64_t g = base + (int64_t)k * 64;
     if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
   }
 }
@@ -288,13 +405,35 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
   dim3 grid((unsigned)tiles, (unsigned)nch);
   hipStream_t st = ctx->stream;
-#define INCA_EVAL(PP)                                                                          \
+  // bounds of the wave tiles of this P (a few microseconds; redone per launch)
+  const bool cut = ctx->cut_log2 > 0.0;
+  const int64_t ntile = tiles * (kT / 64);
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_tb, ctx->tb_cap, (size_t)ntile * kTB))) return rc;
+  if (cut) {
+    const int mode = ctx->grid_kind == GRID_RADIAL ? 0 : (ctx->grid_kind == GRID_MULTI ? 1 : 2);
+    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
+                       ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
+                       ctx->d_tb);
+  } else {
+    // only the point-count word is read: mark every tile non-empty
+    hipLaunchKernelGGL(k_fill_tiles, dim3((unsigned)((ntile + kT - 1) / kT)), dim3(kT), 0, st,
+                       ntile, ctx->d_tb);
+  }
+  INCA_HIP(hipGetLastError());
+#define INCA_EVAL2(PP, CC)                                                                     \
   if (mixed)                                                                                   \
-    hipLaunchKernelGGL(k_eval_mixed<PP>, grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,       \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, partial);                            \
+    hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch, \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, partial);                 \
   else                                                                                         \
-    hipLaunchKernelGGL(k_eval_fp64<PP>, grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,        \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, partial);
+    hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, partial);
+#define INCA_EVAL(PP)      \
+  if (cut) {               \
+    INCA_EVAL2(PP, true)   \
+  } else {                 \
+    INCA_EVAL2(PP, false)  \
+  }
   switch (P) {
     case 1: INCA_EVAL(1) break;
     case 2: INCA_EVAL(2) break;
@@ -302,6 +441,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     default: INCA_EVAL(4) break;
   }
 #undef INCA_EVAL
+#undef INCA_EVAL2
   INCA_HIP(hipGetLastError());
   return 0;
 }

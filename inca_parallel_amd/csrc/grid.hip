@@ -171,6 +171,8 @@ int alloc_grid(inca_ctx *ctx, int64_t G) {
   }
   if (ctx->d_map) (void)hipFree(ctx->d_map);
   ctx->d_map = nullptr;
+  if (ctx->d_gcent) (void)hipFree(ctx->d_gcent);
+  ctx->d_gcent = nullptr;
   size_t n = (size_t)(G > 0 ? G : 1);
   for (double **a : arrs)
     if (hipMalloc((void **)a, n * sizeof(double)) != hipSuccess)
@@ -302,6 +304,9 @@ int grid_multicenter(inca_ctx *ctx, int nquad, const double *cent,
   for (int64_t g = nf - 1; g >= 0; --g) ctx->centre_first[map[g] - 1] = g;
   for (int c = nquad - 1; c >= 0; --c)
     if (ctx->centre_first[c] > ctx->centre_first[c + 1]) ctx->centre_first[c] = ctx->centre_first[c + 1];
+  // centre coordinates: the anchors of the evaluation's tile bounds
+  INCA_HIP(hipMalloc((void **)&ctx->d_gcent, 3 * (size_t)nquad * sizeof(double)));
+  INCA_HIP(hipMemcpy(ctx->d_gcent, cent, 3 * (size_t)nquad * sizeof(double), hipMemcpyHostToDevice));
   *rrgrid = nf;
   ctx->nquad = nquad;
   ctx->grid_kind = GRID_MULTI;

@@ -36,6 +36,10 @@ __constant__ double c_gh_x[11][11];
 __constant__ double c_gh_w[11][11];
 // w_m of the second screen (mod_intracule.f90:136-143)
 __constant__ double c_wm[kNpMax];
+// Gamma((k+1)/2) = int |y|^k exp(-y^2) dy over the real line
+__constant__ double c_gmom[kNpMax];
+// max over y of |y|^k exp(-y^2) = (k/2)^(k/2) exp(-k/2)  (1 for k = 0)
+__constant__ double c_ymax[kNpMax];
 
 const double kGhRoot[11][11] = {
     {0},
@@ -210,15 +214,92 @@ __device__ int polycoef(const double R[4], double Rik, double Rjl, double Rijkl,
   return bad;
 }
 
+// Exact coefficients (descending powers of y = sqe (r + d), d = P_ik - P_jl)
+// of the polynomial polycoef fits: p(r) = sum_n w_n prod_a (invz t_n + s_a r +
+// (Rijkl - R_a))^{t_a}, s = alf -+ 1/2 (mod_intrastuff.f90:178-193); each
+// factor b + s r is expanded as (b - s d) + (s/sqe) y.  Same polynomial as
+// the fit in exact arithmetic, without the Vandermonde solve.
+template <int NPM>
+__device__ void exact_poly(const double R[4], double Rijkl, const int t[4], int L, double invz,
+                           double alf, double sqe, double d, double *E) {
+  const int n = L / 2 + 1;
+  double acc[NPM];
+  for (int m = 0; m <= L; ++m) acc[m] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    double poly[NPM];
+    poly[0] = 1.0;
+    int deg = 0;
+    const double a = invz * c_gh_x[n - 1][k];
+    for (int f = 0; f < 4; ++f) {
+      const double s0 = (f == 0 || f == 2) ? alf - 0.5 : alf + 0.5;
+      const double b = (a + (Rijkl - R[f])) - s0 * d;
+      const double sl = s0 / sqe;
+      for (int e = 0; e < t[f]; ++e) {
+        poly[deg + 1] = poly[deg] * sl;
+        for (int m = deg; m >= 1; --m) poly[m] = poly[m] * b + poly[m - 1] * sl;
+        poly[0] *= b;
+        ++deg;
+      }
+    }
+    for (int m = 0; m <= deg; ++m) acc[m] += c_gh_w[n - 1][k] * poly[m];
+  }
+  for (int q = 0; q <= L; ++q) E[q] = acc[L - q];
+}
+
+// Radius in rp beyond which the term A exp(-|rp|^2) Vx Vy Vz stays below
+// 2^-cut_log2 of its peak bound |A| prod_axis sum_k |c_k| ymax(k): with
+// |rp_axis| <= |rp| = sqrt(s), |term| <= |A| e^-s prod_axis H_axis(sqrt(s)),
+// H(t) = sum_k |c_k| t^k, so the smallest s with
+//   F(s) = sum_axis log(H_axis(sqrt s) / P_axis) - s + s0 <= 0,
+// s0 = cut_log2 ln 2, is wanted.  d/ds log H(sqrt s) <= L / (2 s), so F
+// falls at least at rate 1 - Lsum/(2s) > 0 for s >= s0 > Lsum/2, which makes
+// the fixed-point iterate plus one Newton-type correction a safe bound.
+// c holds the three axis polynomials back to back (highest power first,
+// the x axis scaled by A, which the ratios do not see).  s0 <= 0: no cutoff.
+__device__ double cutoff_radius(const double *c, const int Ls[3], double s0) {
+  if (!(s0 > 0.0)) return HUGE_VAL;
+  double Pax[3];
+  int off = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    const int L = Ls[ax];
+    double p = 0.0;
+    for (int q = 0; q <= L; ++q) p += fabs(c[off + q]) * c_ymax[L - q];
+    if (!(p > 0.0)) return 0.0;  // identically zero term
+    Pax[ax] = p;
+    off += L + 1;
+  }
+  const int Lsum = Ls[0] + Ls[1] + Ls[2];
+  double s = s0, F = 0.0;
+  for (int it = 0; it < 6; ++it) {
+    const double t = sqrt(s);
+    double ratio = 1.0;
+    off = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+      const int L = Ls[ax];
+      double h = 0.0;
+      for (int q = 0; q <= L; ++q) h = h * t + fabs(c[off + q]);
+      ratio *= h / Pax[ax];
+      off += L + 1;
+    }
+    F = log(ratio) - s + s0;
+    if (it < 5) s = s + F;
+  }
+  if (F > 0.0) s += F / (1.0 - Lsum / (2.0 * s));
+  const double r = sqrt(s) * (1.0 + 1e-12);
+  return (r >= 0.0) ? r : HUGE_VAL;
+}
+
 // One thread per record; the output record layout is described at kQHead.
 template <int NPM>
 __global__ __launch_bounds__(kT) void k_prep(
     const int32_t *__restrict__ rec, int64_t nvalid, int nprim, const double *__restrict__ prim,
     const int32_t *__restrict__ tmn, const PairEntry *__restrict__ pair, double lim,
-    int normalize, int qstride, double *__restrict__ qout, uint8_t *__restrict__ flag,
-    unsigned long long *__restrict__ cnt) {
+    int normalize, int qstride, int exact, double cut_s0, double *__restrict__ qout,
+    uint8_t *__restrict__ flag, unsigned long long *__restrict__ cnt,
+    double *__restrict__ msum) {
   const int64_t n = (int64_t)blockIdx.x * kT + threadIdx.x;
   int pass1 = 0, pass2 = 0, nfail = 0;
+  double mom = 0.0;  // signed whole-space integral of a surviving term
   if (n < nvalid) {
     const int32_t *p = rec + 6 * n;
     const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
@@ -243,6 +324,9 @@ __global__ __launch_bounds__(kT) void k_prep(
       int off = kQHead;
       int Ls[3];
       double U = A;
+      // B: bound on the L1 norm of the term, M: its integral over all space
+      // (per axis, int q(y) exp(-y^2) dr with dr = dy / sqe)
+      double B = fabs(A), M = A;
       double C[NPM];
       for (int ax = 0; ax < 3; ++ax) {
         const double *Rx = prim + (1 + ax) * nprim;
@@ -256,6 +340,15 @@ __global__ __launch_bounds__(kT) void k_prep(
         double u = 0.0;
         for (int q = 0; q <= L; ++q) u += C[q] * c_wm[q];
         U *= u;
+        if (exact) exact_poly<NPM>(R, Pijkl, t, L, invz, alf, sqe, pik.P[ax] - pjl.P[ax], C);
+        double bax = 0.0, max_ = 0.0;
+        for (int q = 0; q <= L; ++q) {
+          const double g = c_gmom[L - q];
+          bax += fabs(C[q]) * g;
+          if (((L - q) & 1) == 0) max_ += C[q] * g;
+        }
+        B *= bax * (1.0 / sqe);
+        M *= max_ * (1.0 / sqe);
         const double scale = (ax == 0) ? A : 1.0;
         for (int q = 0; q <= L; ++q) out[off + q] = scale * C[q];
         off += L + 1;
@@ -267,7 +360,10 @@ __global__ __launch_bounds__(kT) void k_prep(
         out[2] = sqe * (pik.P[1] - pjl.P[1]);
         out[3] = sqe * (pik.P[2] - pjl.P[2]);
         out[4] = __longlong_as_double((long long)(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16)));
-        out[5] = fabs(U);
+        out[5] = B;
+        out[6] = cutoff_radius(out + kQHead, Ls, cut_s0);
+        out[7] = 0.0;
+        mom = M;
       }
     }
     flag[n] = (uint8_t)pass2;
@@ -283,6 +379,33 @@ __global__ __launch_bounds__(kT) void k_prep(
     if (b1) atomicAdd(&cnt[1], (unsigned long long)__popcll(b1));
     if (wsum) atomicAdd(&cnt[3], (unsigned long long)wsum);
   }
+  if (msum) {  // fixed-order block sum of the term integrals
+    __shared__ double wred[kT / 64];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mom += __shfl_down(mom, d, 64);
+    if (__lane_id() == 0) wred[threadIdx.x / 64] = mom;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kT / 64; ++w) s += wred[w];
+      msum[blockIdx.x] = s;
+    }
+  }
+}
+
+// Sum of n block partials by one block in a fixed order.
+__global__ __launch_bounds__(kT) void k_sum_fixed(const double *__restrict__ x, int64_t n,
+                                                  double *__restrict__ out) {
+  __shared__ double red[kT];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kT) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict__ src,
@@ -297,8 +420,8 @@ __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict_
   dst[t] = src[idx[r] * stride + w];
 }
 
-// Histogram of the binary exponent of |A Ux Uy Uz| over records that passed
-// both screens (LDS per block, then one global atomic per non-empty bin).
+// Histogram of the binary exponent of B (header word 5) over records that
+// passed both screens (LDS per block, then one global atomic per non-empty bin).
 constexpr int kMagBins = 2100;  // ilogb in [-1075, 1024]
 constexpr int kMagOff = 1075;
 
@@ -320,7 +443,7 @@ __global__ __launch_bounds__(kT) void k_mag_hist(const double *__restrict__ qtmp
     if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
 }
 
-// heavy (|U| >= tau) or light (|U| < tau) among the records that passed
+// heavy (B >= tau) or light (B < tau) among the records that passed
 __global__ __launch_bounds__(kT) void k_split_flags(const double *__restrict__ qtmp, int stride,
                                                     const uint8_t *__restrict__ pass, int64_t n,
                                                     double tau, int heavy,
@@ -362,12 +485,18 @@ int upload_constants(inca_ctx *ctx) {
       w[n - 1][i] = cons * (1.0 / (h * h));
     }
   }
-  double wm[kNpMax];
+  double wm[kNpMax], gm[kNpMax];
   for (int p = 1; p <= kNpMax; ++p)
     wm[p - 1] = (p == 1) ? 1.0 : std::pow(p * 0.5, p * 0.5) * std::exp(-(double)p * 0.5);
+  for (int k = 0; k < kNpMax; ++k) gm[k] = std::tgamma(0.5 * (k + 1));
+  double ym[kNpMax];
+  for (int k = 0; k < kNpMax; ++k)
+    ym[k] = (k == 0) ? 1.0 : std::pow(0.5 * k, 0.5 * k) * std::exp(-0.5 * k);
   INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gh_x), kGhRoot, sizeof(kGhRoot)));
   INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gh_w), w, sizeof(w)));
   INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_wm), wm, sizeof(wm)));
+  INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gmom), gm, sizeof(gm)));
+  INCA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ymax), ym, sizeof(ym)));
   if (ctx->device >= 0 && ctx->device < 64) g_const_ready[ctx->device] = true;
   return 0;
 }
@@ -413,19 +542,38 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   if ((rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)(nvalid > 0 ? nvalid : 1)))) return rc;
   // mod_intracule.f90:163
   const double lim = thresh * (1.0 / ((double)nprim * ((double)nprim + 1.0) * 0.5));
+  const bool want_mom = nvalid > 0;
+  double *d_mom = nullptr;  // block partials of sum M, then the total after them
+  ctx->term_integral = 0.0;
+  if (want_mom) INCA_HIP(hipMalloc((void **)&d_mom, ((size_t)nblk(nvalid) + 1) * sizeof(double)));
   if (nvalid > 0) {
 #define INCA_PREP(NPM)                                                                        \
   hipLaunchKernelGGL(k_prep<NPM>, dim3(nblk(nvalid)), dim3(kT), 0, st,                       \
                      (const int32_t *)ctx->d_rec, nvalid, nprim, ctx->d_prim, ctx->d_tmn,     \
-                     ctx->d_pair, lim, ctx->normalize, stride, ctx->d_qtmp, ctx->d_flag,      \
-                     ctx->d_cnt)
+                     ctx->d_pair, lim, ctx->normalize, stride, ctx->poly_exact,               \
+                     ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp,                        \
+                     ctx->d_flag, ctx->d_cnt, d_mom)
     if (la <= 4) INCA_PREP(5);
     else if (la <= 8) INCA_PREP(9);
     else if (la <= 12) INCA_PREP(13);
     else if (la <= 16) INCA_PREP(17);
     else INCA_PREP(22);
 #undef INCA_PREP
-    INCA_HIP(hipGetLastError());
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && want_mom) {
+      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, d_mom, (int64_t)nblk(nvalid),
+                         d_mom + nblk(nvalid));
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && want_mom)
+      e = hipMemcpyAsync(&ctx->term_integral, d_mom + nblk(nvalid), sizeof(double),
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && want_mom) e = hipStreamSynchronize(st);
+    // in the grids' convention: their weights carry 2 pi where the volume
+    // element has 4 pi (mod_build_grid.f90:132)
+    ctx->term_integral *= 0.5;
+    if (d_mom) (void)hipFree(d_mom);
+    if (e != hipSuccess) return hip_fail(ctx, e, "quartet preparation", __LINE__);
   }
   int64_t q2 = 0;
   if ((rc = compact_flags(ctx, ctx->d_flag, nvalid, ctx->d_idx, &q2))) return rc;
@@ -433,8 +581,10 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   ctx->q_fp64 = q2;
   ctx->mixed_tau = 0.0;
   if (precision == INCA_MIXED && q2 > 0 && ctx->mixed_tail > 0.0) {
-    // Precision by magnitude: the quartets carrying all but `mixed_tail` of
-    // sum |U| stay FP64, the light tail goes to the FP32 kernel.
+    // Precision by magnitude.  FP32 roundoff in a term is bounded by a few
+    // ulps of its L1 norm, bounded by B; the lightest terms go to the FP32
+    // kernel while their summed B stays within mixed_tail * |sum M| (sum M
+    // being the integral of I over all space), the rest stays FP64.
     unsigned long long *d_h = nullptr;
     INCA_HIP(hipMalloc((void **)&d_h, kMagBins * sizeof(unsigned long long)));
     std::vector<unsigned long long> h(kMagBins, 0);
@@ -453,13 +603,17 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     std::vector<double> mass(kMagBins);
     double total = 0.0;
     for (int b = 0; b < kMagBins; ++b) {
-      // cap the bin scale so that 2^1024 never appears (empty bins stay 0)
-      mass[b] = h[b] ? (double)h[b] * std::ldexp(1.5, std::min(b - kMagOff, 1000)) : 0.0;
+      // upper bound of a bin's mass: count * 2^(exponent + 1), with the scale
+      // capped so that 2^1024 never appears (empty bins stay 0)
+      mass[b] = h[b] ? (double)h[b] * std::ldexp(2.0, std::min(b - kMagOff, 1000)) : 0.0;
       total += mass[b];
     }
+    const double budget = ctx->mixed_tail >= 1.0
+                              ? HUGE_VAL
+                              : ctx->mixed_tail * std::min(std::fabs(ctx->term_integral), total);
     double cum = 0.0;
     int cut = 0;
-    while (cut < kMagBins && cum + mass[cut] <= ctx->mixed_tail * total) cum += mass[cut++];
+    while (cut < kMagBins && cum + mass[cut] <= budget) cum += mass[cut++];
     const double tau = std::ldexp(1.0, cut - kMagOff);
     ctx->mixed_tau = tau;
     uint8_t *d_f2 = nullptr;

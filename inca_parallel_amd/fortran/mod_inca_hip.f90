@@ -91,6 +91,11 @@ module inca_hip
          type(c_ptr), value :: ctx
          real(c_double), value :: tail
       end function
+      integer(c_int) function inca_set_polynomials(ctx, mode) bind(C, name="inca_set_polynomials")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: mode
+      end function
       integer(c_int) function inca_integrate(ctx, integral, vee, ipc, vpc) &
             bind(C, name="inca_integrate")
          import :: c_int, c_ptr

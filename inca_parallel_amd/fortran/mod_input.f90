@@ -2,8 +2,9 @@
 !! src/mod_input_reader.f90: the blocks &files, &intracule_job and &cube_job
 !! (each optional; a syntax error stops the run), the same defaults, the
 !! calc_type dispatch and the default output names derived from the input
-!! file name.  &intracule_job also accepts two engine options of this build:
-!! engine_precision ('fp64' or 'mixed') and mixed_tail.
+!! file name.  &intracule_job also accepts three engine options of this
+!! build: engine_precision ('fp64' or 'mixed'), mixed_tail, and
+!! engine_polynomials ('reference' fits or 'exact' expansion).
 module input_reader
    use inputdat
    use intrainfo
@@ -19,6 +20,7 @@ module input_reader
    real(8), dimension(50) :: sfalpha_in = 1.0d0, Ps_in = 1.0d0
    logical :: multicenter = .false., manual_grid = .false.
    character(len=16) :: engine_precision = 'fp64'
+   character(len=16) :: engine_polynomials = 'reference'
 
    ! &cube_job fields (one-electron cubes are not part of this build; the
    ! block is parsed so existing inputs keep working)
@@ -39,7 +41,8 @@ contains
                                x_point2, y_point2, z_point2, multicenter, manual_grid, nrad, &
                                nang, nquad_in, cent_in, nradc_in, nangc_in, sfalpha_in, Ps_in, &
                                nblock, scan_start, scan_end, scan_step, scan_nang, center_i, &
-                               step_i, np_i, engine_precision, mixed_tail
+                               step_i, np_i, engine_precision, mixed_tail, &
+                               engine_polynomials
       namelist /cube_job/ cube, denscube, primcube, aocube, mocube, gradient, laplacian, npr, &
                           cao, mo, center, step, np
 
@@ -101,6 +104,18 @@ contains
       case default
          write(*,*) "ERROR: Unknown engine_precision '", trim(engine_precision), "'."
          write(*,*) "Valid values: fp64, mixed."
+         stop 1
+      end select
+
+      call to_lower(engine_polynomials)
+      select case (trim(engine_polynomials))
+      case ("reference")
+         exact_polynomials = .false.
+      case ("exact")
+         exact_polynomials = .true.
+      case default
+         write(*,*) "ERROR: Unknown engine_polynomials '", trim(engine_polynomials), "'."
+         write(*,*) "Valid values: reference, exact."
          stop 1
       end select
 

@@ -91,6 +91,8 @@ contains
       end if
       if (mixed_precision) call inca_check(inca_set_mixed_tail(inca_ctx, mixed_tail), &
                                            "inca_set_mixed_tail")
+      call inca_check(inca_set_polynomials(inca_ctx, merge(1_c_int32_t, 0_c_int32_t, &
+                      exact_polynomials)), "inca_set_polynomials")
       write(*,*) "Starting GPU evaluation over primitive quartets..."
 
       t3 = wall()
@@ -210,6 +212,11 @@ contains
          write(iout,*) "Engine precision: mixed (FP64 heavy quartets, FP32 tail ", mixed_tail, ")"
       else
          write(iout,*) "Engine precision: FP64"
+      end if
+      if (exact_polynomials) then
+         write(iout,*) "Axis polynomials: exact expansion in r"
+      else
+         write(iout,*) "Axis polynomials: Gauss-Hermite + Vandermonde fits (reference)"
       end if
       write(iout,*) "=========================================================="
       close(iout)

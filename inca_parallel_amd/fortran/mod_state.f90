@@ -66,6 +66,7 @@ module intrainfo
    ! engine options of this build (no reference counterpart)
    logical :: mixed_precision = .false.
    double precision :: mixed_tail = 0.02d0
+   logical :: exact_polynomials = .false.
 end module intrainfo
 
 module inputdat

@@ -123,16 +123,52 @@ int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
 int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh,
                   double *ivec, int64_t *counts);
 
-/* Mixed precision by magnitude: after screening, quartets are ranked by
- * their screen-2 magnitude |A Ux Uy Uz| (mod_intracule.f90:342); the heaviest
- * ones, carrying all but `tail` of the summed magnitude, are evaluated in
- * FP64 and the rest in FP32.  tail = 1 evaluates everything in FP32; the
- * default is 0.02.  No reference counterpart (the reference is FP64 only). */
+/* Mixed precision by magnitude: after screening, each quartet's term
+ * A exp(-|rp|^2) Vx Vy Vz gets a bound B on its L1 norm over all space
+ * (|A| prod_axis sum_k |C_k| Gamma((k+1)/2) / sqe) and its exact integral M.
+ * The lightest quartets go to the FP32 kernel as long as their summed B
+ * stays within tail * |sum M| (sum M = the integral of I over all space), so
+ * the FP32 roundoff is budgeted against the result, not the magnitudes of
+ * terms that cancel; the rest run in FP64.  tail >= 1 evaluates everything
+ * in FP32; the default is 0.02.  No reference counterpart (the reference is
+ * FP64 only). */
 int inca_set_mixed_tail(inca_ctx *ctx, double tail);
 /* Split used by the last INCA_MIXED evaluation: quartets in FP64 and in
- * FP32, and the magnitude threshold between them. */
+ * FP32, and the L1-bound threshold B between them. */
 int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
                          double *tau);
+
+/* Per-point cutoff.  The reference evaluates every quartet at every grid
+ * point (mod_intracule.f90:345-358).  With log2_rel > 0 each prepared
+ * quartet gets a radius in rp beyond which its term is provably below
+ * 2^-log2_rel of its peak bound (|A| prod_axis sum_k |C_k| max_y |y|^k
+ * e^-y^2, using |term| <= |A| e^-s prod_axis sum_k |C_k| s^(k/2) at
+ * s = |rp|^2), and a wavefront skips the quartet when all of its 64 P grid
+ * points lie beyond that radius.  The skipped contributions sum to at most
+ * 2^-log2_rel times the summed peak bounds.  0 turns the cutoff off (the
+ * reference's loop, the default); otherwise 64 <= log2_rel <= 1000 (100 is
+ * the tested setting).  Takes effect at the next inca_evaluate.  No
+ * reference counterpart. */
+int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
+
+/* Sum over the quartets prepared by the last inca_evaluate of each term's
+ * integral over all space, prod_axis sum_{k even} C_k Gamma((k+1)/2) A /
+ * sqe^3, times 1/2 (the grid weights carry 2 pi for the 4 pi of the volume
+ * element, mod_build_grid.f90:132): the analytic value of the integral that
+ * inca_integrate takes on the grid.  With exact polynomials it equals the
+ * pair sum rule N(N-1)/2 up to the screened terms.  No reference
+ * counterpart. */
+int inca_get_analytic_integral(inca_ctx *ctx, double *total);
+
+/* Axis polynomials.  mode 0 (default): the reference's Gauss-Hermite +
+ * Vandermonde fit in rp = sqe (r + P_ik - P_jl) (mod_intrastuff.f90:127-194),
+ * roundoff-dominated where the Vandermonde matrix is ill-conditioned (large
+ * exponents times high angular momentum).  mode 1: the same polynomials
+ * expanded exactly in rp (each linear factor of the product written in rp;
+ * no Vandermonde solve), well conditioned.  The screens
+ * and counters are those of the reference in both modes.  No reference
+ * counterpart for mode 1. */
+int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
 
 /* Evaluation tiles: grid points per lane (1, 2, 4 or 8) of the FP64 and of
  * the FP32 kernel.  inca_autotune times each choice on a slice of the
@@ -146,8 +182,10 @@ int inca_autotune(inca_ctx *ctx, int32_t *best);
 /* Diagnostics: copy prepared quartet records [first, first+n) of the last
  * inca_evaluate (FP64 range first, then the FP32 range in mixed mode).
  * Record layout: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
- * Lx | Ly<<8 | Lz<<16 (as integer bits), [5] |A Ux Uy Uz|, then A*Cx(0..Lx),
- * Cy(0..Ly), Cz(0..Lz); *stride receives the record length in doubles.
+ * Lx | Ly<<8 | Lz<<16 (as integer bits), [5] B (the L1 bound of the term,
+ * see inca_set_mixed_tail), [6] the cutoff radius in rp (+inf when off,
+ * see inca_set_point_cutoff), [7] unused, then A*Cx(0..Lx), Cy(0..Ly),
+ * Cz(0..Lz); *stride receives the record length in doubles.
  * No reference counterpart. */
 int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out,
                       int32_t *stride);

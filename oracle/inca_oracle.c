@@ -271,6 +271,40 @@ int orc_polycoef(double ri, double rj, double rk, double rl, double rik,
   return 0;
 }
 
+/* Exact coefficients of the polynomial polycoef fits: with
+ * p(r) = sum_n w_n prod_a (invz t_n + s_a r + (Rijkl - R_a))^{t_a},
+ * s = alfijkl - 1/2 for i,k and + 1/2 for j,l (mod_intrastuff.f90:178-193),
+ * the fit is q(y) = p(r) on y = sqe (r + d), d = P_ik - P_jl.  Each factor
+ * b + s r is (b - s d) + (s / sqe) y, so q is expanded directly in the
+ * reference's own variable instead of interpolated; E[0] multiplies y^L.
+ * No reference counterpart: the well-conditioned alternative to the
+ * Vandermonde fit (same polynomial in exact arithmetic). */
+static void exact_poly(const double R[4], double rijkl, const int t[4], int L,
+                       double invz, double alf, double sqe, double d,
+                       double *E) {
+  double rh[11], wh[11];
+  int n = orc_gauherm(L, rh, wh);
+  double acc[22] = {0.0};
+  for (int k = 0; k < n; ++k) {
+    double poly[22] = {1.0};
+    int deg = 0;
+    const double a = invz * rh[k];
+    for (int f = 0; f < 4; ++f) {
+      const double s0 = (f == 0 || f == 2) ? alf - 0.5 : alf + 0.5;
+      const double b = (a + (rijkl - R[f])) - s0 * d;
+      const double sl = s0 / sqe;
+      for (int e = 0; e < t[f]; ++e) {
+        poly[deg + 1] = poly[deg] * sl;
+        for (int m = deg; m >= 1; --m) poly[m] = poly[m] * b + poly[m - 1] * sl;
+        poly[0] *= b;
+        ++deg;
+      }
+    }
+    for (int m = 0; m <= deg; ++m) acc[m] += wh[k] * poly[m];
+  }
+  for (int q = 0; q <= L; ++q) E[q] = acc[L - q];
+}
+
 static inline int TMN(const int32_t *tmn, int nprim, int i, int ax) {
   return tmn[i + nprim * ax];
 }
@@ -332,6 +366,12 @@ void orc_prepare_quartet(int nprim, const double *alpha, const int32_t *tmn,
                      P_jl[ax], P[ax], t[0][ax], t[1][ax], t[2][ax], t[3][ax],
                      np, sqe, invz, alfijkl, q->C[ax]))
       q->polyfail++;
+    {
+      const double Rq[4] = {R[ax][0], R[ax][1], R[ax][2], R[ax][3]};
+      const int tq[4] = {t[0][ax], t[1][ax], t[2][ax], t[3][ax]};
+      exact_poly(Rq, P[ax], tq, L, invz, alfijkl, sqe, P_ik[ax] - P_jl[ax],
+                 q->E[ax]);
+    }
     /* wmarray, mod_intracule.f90:136-143 */
     double u = 0.0;
     for (int p = 0; p < np; ++p) {
@@ -351,29 +391,42 @@ void orc_prepare_quartet(int nprim, const double *alpha, const int32_t *tmn,
   q->pass2 = (q->screen2 >= lim);
 }
 
-/* mod_intracule.f90:345-358 for one prepared quartet over all points */
+/* mod_intracule.f90:345-358 for one prepared quartet over all points.
+ * exact != 0 evaluates the exact axis polynomials instead of the fits. */
 static void grid_loop(const orc_quartet *q, int64_t G, const double *r,
-                      double *acc) {
+                      double *acc, int exact) {
   const int nx = q->L[0] + 1, ny = q->L[1] + 1, nz = q->L[2] + 1;
+  const double(*Cc)[22] = exact ? q->E : q->C;
   for (int64_t g = 0; g < G; ++g) {
     double rx = q->sqe * (r[3 * g] + q->dx);
     double ry = q->sqe * (r[3 * g + 1] + q->dy);
     double rz = q->sqe * (r[3 * g + 2] + q->dz);
     double vx = 0.0, vy = 0.0, vz = 0.0;
-    for (int p = 0; p < nx; ++p) vx = vx * rx + q->C[0][p];
-    for (int p = 0; p < ny; ++p) vy = vy * ry + q->C[1][p];
-    for (int p = 0; p < nz; ++p) vz = vz * rz + q->C[2][p];
+    for (int p = 0; p < nx; ++p) vx = vx * rx + Cc[0][p];
+    for (int p = 0; p < ny; ++p) vy = vy * ry + Cc[1][p];
+    for (int p = 0; p < nz; ++p) vz = vz * rz + Cc[2][p];
     acc[g] += q->A * exp(-(rx * rx + ry * ry + rz * rz)) * vx * vy * vz;
   }
 }
 
-/* mod_intracule.f90:229-381 */
 int orc_intracule(int nprim, const double *alpha, const int32_t *tmn,
                   const double *x, const double *y, const double *z,
                   int normalize, double thresh, int64_t nrec,
                   const int32_t *rec_ijkl, const double *rec_d, int64_t G,
                   const double *r, double *ivec, int64_t *counts,
                   int nthreads, int dynamic) {
+  return orc_intracule_ex(nprim, alpha, tmn, x, y, z, normalize, thresh, nrec,
+                          rec_ijkl, rec_d, G, r, ivec, counts, nthreads,
+                          dynamic, 0);
+}
+
+/* mod_intracule.f90:229-381 */
+int orc_intracule_ex(int nprim, const double *alpha, const int32_t *tmn,
+                     const double *x, const double *y, const double *z,
+                     int normalize, double thresh, int64_t nrec,
+                     const int32_t *rec_ijkl, const double *rec_d, int64_t G,
+                     const double *r, double *ivec, int64_t *counts,
+                     int nthreads, int dynamic, int exact) {
   /* the reference stops at the first invalid record (:239-240) */
   int64_t nvalid = 0;
   for (; nvalid < nrec; ++nvalid) {
@@ -410,7 +463,7 @@ int orc_intracule(int nprim, const double *alpha, const int32_t *tmn,
         cf += q.polyfail;
         if (q.pass1 && q.pass2) {
           ++c2;
-          grid_loop(&q, G, r, acc);
+          grid_loop(&q, G, r, acc, exact);
         }
       }
     } else {
@@ -423,7 +476,7 @@ int orc_intracule(int nprim, const double *alpha, const int32_t *tmn,
         cf += q.polyfail;
         if (q.pass1 && q.pass2) {
           ++c2;
-          grid_loop(&q, G, r, acc);
+          grid_loop(&q, G, r, acc, exact);
         }
       }
     }

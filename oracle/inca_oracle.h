@@ -47,7 +47,8 @@ typedef struct {
   double screen1, screen2;
   int L[3];
   int pass1, pass2, polyfail;
-  double C[3][22];
+  double C[3][22];   /* fitted coefficients in rp (reference) */
+  double E[3][22];   /* exact coefficients in rp (see exact_poly) */
 } orc_quartet;
 
 /* mod_intracule.f90:282-344 for one record (1-based i,j,k,l). */
@@ -70,6 +71,14 @@ int orc_intracule(int nprim, const double *alpha, const int32_t *tmn,
                   const int32_t *rec_ijkl, const double *rec_d, int64_t G,
                   const double *r, double *ivec, int64_t *counts,
                   int nthreads, int dynamic);
+
+/* Same loop; exact != 0 evaluates the exact axis polynomials. */
+int orc_intracule_ex(int nprim, const double *alpha, const int32_t *tmn,
+                     const double *x, const double *y, const double *z,
+                     int normalize, double thresh, int64_t nrec,
+                     const int32_t *rec_ijkl, const double *rec_d, int64_t G,
+                     const double *r, double *ivec, int64_t *counts,
+                     int nthreads, int dynamic, int exact);
 
 /* mod_quadratures.f90:37-86 on [-1,1] */
 void orc_gauleg(int n, double *t, double *w);

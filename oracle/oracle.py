@@ -51,6 +51,8 @@ def lib():
                                 C.c_double, C.c_int64, _ip, _dp, C.c_int64,
                                 _dp, _dp, _lp, C.c_int, C.c_int]
     L.orc_intracule.restype = C.c_int
+    L.orc_intracule_ex.argtypes = L.orc_intracule.argtypes + [C.c_int]
+    L.orc_intracule_ex.restype = C.c_int
     L.orc_gauleg.argtypes = [C.c_int, _dp, _dp]
     L.orc_gridpoints.argtypes = [C.c_int, _dp, _ip, _ip, _dp, _dp, C.c_int,
                                  _dp, C.POINTER(C.c_int64),
@@ -80,7 +82,8 @@ class OrcQuartet(C.Structure):
     _fields_ = [("A", C.c_double), ("sqe", C.c_double), ("dx", C.c_double),
                 ("dy", C.c_double), ("dz", C.c_double), ("screen1", C.c_double),
                 ("screen2", C.c_double), ("L", C.c_int * 3), ("pass1", C.c_int),
-                ("pass2", C.c_int), ("polyfail", C.c_int), ("C", (C.c_double * 22) * 3)]
+                ("pass2", C.c_int), ("polyfail", C.c_int), ("C", (C.c_double * 22) * 3),
+                ("E", (C.c_double * 22) * 3)]
 
 
 def prepare_quartet(prims, q, D, thresh, normalize=True):
@@ -137,8 +140,9 @@ def normprim(alpha, tmn):
 
 def intracule(prims: dict, ijkl: np.ndarray, d: np.ndarray, r: np.ndarray,
               thresh: float, normalize: bool = True, nthreads: int = 0,
-              dynamic: bool = False):
-    """Quartet loop of the reference; returns (ivec, counts[4])."""
+              dynamic: bool = False, exact: bool = False):
+    """Quartet loop of the reference; returns (ivec, counts[4]).
+    exact=True evaluates the exact axis polynomials instead of the fits."""
     L = lib()
     alpha = _f64(prims["alpha"])
     nprim = len(alpha)
@@ -149,10 +153,10 @@ def intracule(prims: dict, ijkl: np.ndarray, d: np.ndarray, r: np.ndarray,
     G = r.size // 3
     ivec = np.zeros(max(G, 1))
     counts = np.zeros(4, dtype=np.int64)
-    L.orc_intracule(nprim, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]),
-                    _f64(prims["z"]), int(normalize), float(thresh),
-                    len(d), ijkl, d, G, r if G else np.zeros(3), ivec, counts,
-                    int(nthreads), int(dynamic))
+    L.orc_intracule_ex(nprim, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]),
+                       _f64(prims["z"]), int(normalize), float(thresh),
+                       len(d), ijkl, d, G, r if G else np.zeros(3), ivec, counts,
+                       int(nthreads), int(dynamic), int(exact))
     return ivec[:G], counts
 
 
