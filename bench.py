@@ -72,6 +72,68 @@ def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
                            n2, n_total, int(cnt[2]), grid_r.shape[0], t))
 
 
+def batch_bench(capi, synth, torch, nmol=48, reps=3):
+    """Batched launch on a set of small molecules (scaled-down batch workload:
+    3-6 atoms, s/p/d primitives, autogrid centres with nrad 30-50 x LD0110 or
+    LD0146, nosym, symmetry-unique records generated in HBM).  Compares one
+    replay of the captured persistent launch with the same molecules
+    evaluated one context at a time."""
+    engines, keep = [], []
+    evals = 0.0
+    t_unb_kernel = 0.0
+    t_unb_wall = 0.0
+    for i in range(nmol):
+        nat = 3 + i % 4
+        s = synth.config_cluster(seed=1000 + i, natoms=nat, nheavy=max(1, nat - 1),
+                                 nocc=2 + i % 4, shells_heavy={0: 2, 1: 1, 2: i % 2},
+                                 shells_h={0: 2})
+        recs = s.dm2_unique_torch(device="cuda")
+        keep.append(recs)
+        p = s.prims()
+        e = capi.Engine(0)
+        e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+        cent, Ps, sf = capi.autogrid(p["an"], p["cartes"], nohydro=True)
+        nq = len(Ps)
+        e.grid_multicenter(cent, [30 + 10 * (i % 3)] * nq, [110 if i % 2 else 146] * nq, sf, Ps,
+                           nosym=True)
+        e.set_quartets(device_ptr=recs.data_ptr(), n=recs.shape[0])
+        e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, cnt = e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+        t_unb_wall += time.perf_counter() - t0
+        tm = e.timings()
+        t_unb_kernel += (tm[1] + tm[2]) * 1e-3
+        evals += float(cnt[2]) * e.G
+        engines.append(e)
+    b = capi.Batch(engines)
+    for e in engines:
+        e.close()
+    del keep
+    best = b.autotune()
+    b.evaluate()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = []
+    for _ in range(reps):
+        b.evaluate()
+        ms.append(b.info()["last_ms"])
+    wall = (time.perf_counter() - t0) / reps
+    info = b.info()
+    b.close()
+    return {
+        "molecules": nmol, "evals_per_replay": evals, "work_items": info["items"],
+        "persistent_blocks": info["blocks"], "points_per_lane": best,
+        "value": evals / (np.mean(ms) * 1e-3), "unit": "evals/s",
+        "replay_ms": float(np.mean(ms)), "replay_wall_ms": wall * 1e3,
+        "unbatched_kernel_evals_per_s": evals / t_unb_kernel,
+        "unbatched_wall_evals_per_s": evals / t_unb_wall,
+        "note": "batched: one hipGraph replay (persistent class-sorted launch + reductions) over "
+                "prepared quartets; unbatched: the same molecules one inca_evaluate at a time "
+                "(kernel = eval + reduce time, wall = whole call incl. pre-pass and syncs)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +141,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--natoms", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -175,6 +238,9 @@ def main():
         "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
                       "reduce": float(r64["kern_ms"][2])},
     }
+    if not a.no_batch:
+        out["batch"] = batch_bench(capi, synth, torch)
+        log("batch: %s" % out["batch"])
     if not a.no_cpu:
         rec_host = recs.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(sysm, rec_host, eng.get_grid()["r"])

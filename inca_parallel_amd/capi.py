@@ -26,6 +26,9 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
+           "inca_prepare", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
+           "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
+           "inca_batch_autotune", "inca_batch_info",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -77,10 +80,21 @@ def load():
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_get_analytic_integral.argtypes = [vp, vp]
     L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
+    L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
+    L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
+                                vp, vp, vp]
+    L.inca_batch_create.argtypes = [C.c_int32, C.c_int32, vp, C.POINTER(vp)]
+    L.inca_batch_destroy.argtypes = [vp]
+    L.inca_batch_last_error.argtypes = [vp]
+    L.inca_batch_last_error.restype = C.c_char_p
+    L.inca_batch_evaluate.argtypes = [vp, vp, vp]
+    L.inca_batch_get_ivec.argtypes = [vp, C.c_int32, vp]
+    L.inca_batch_autotune.argtypes = [vp, vp]
+    L.inca_batch_info.argtypes = [vp, vp, vp, vp, vp, vp]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
     for s in SYMBOLS:
-        if s != "inca_last_error":
+        if s not in ("inca_last_error", "inca_batch_last_error"):
             getattr(L, s).restype = C.c_int
     _lib = L
     return L
@@ -105,6 +119,83 @@ def gauleg(n):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def autogrid(an, cartes, nohydro=True, betaone=False, radius=None):
+    """Integration centres (inca_autogrid): cent (nq,3), Ps (nq), sfalpha (nq).
+    cartes is (natoms,3) in bohr; radius optional per-atom radii in bohr."""
+    L = load()
+    an = np.ascontiguousarray(an, np.int32)
+    n = len(an)
+    cm = np.ascontiguousarray(np.asarray(cartes, np.float64).reshape(n, 3).T.reshape(-1))
+    rad = None if radius is None else np.ascontiguousarray(radius, np.float64)
+    nq = C.c_int32()
+    L.inca_autogrid(n, an.ctypes.data, cm.ctypes.data, int(bool(nohydro)), int(bool(betaone)),
+                    _ptr(rad), 0, C.addressof(nq), None, None, None)
+    cent, Ps, sf = np.zeros((nq.value, 3)), np.zeros(nq.value), np.zeros(nq.value)
+    rc = L.inca_autogrid(n, an.ctypes.data, cm.ctypes.data, int(bool(nohydro)),
+                         int(bool(betaone)), _ptr(rad), nq.value, C.addressof(nq),
+                         cent.ctypes.data, Ps.ctypes.data, sf.ctypes.data)
+    if rc != 0:
+        raise IncaError("inca_autogrid failed (%d)" % rc)
+    return cent, Ps, sf
+
+
+class Batch:
+    """Many prepared contexts evaluated by one captured persistent launch."""
+
+    def __init__(self, engines, device=0):
+        self._L = load()
+        self.engines_G = [e.G for e in engines]
+        arr = (C.c_void_p * len(engines))(*[e._h.value for e in engines])
+        self._h = C.c_void_p()
+        rc = self._L.inca_batch_create(int(device), len(engines), arr, C.byref(self._h))
+        if rc != 0:
+            msg = self._L.inca_batch_last_error(self._h).decode() if self._h else ""
+            self.close()
+            raise IncaError("inca_batch_create failed (%d): %s" % (rc, msg))
+        self.nmol = len(engines)
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise IncaError("%s failed (%d): %s" % (
+                what, rc, self._L.inca_batch_last_error(self._h).decode()))
+
+    def evaluate(self):
+        ti, tv = np.zeros(self.nmol), np.zeros(self.nmol)
+        self._ck(self._L.inca_batch_evaluate(self._h, ti.ctypes.data, tv.ctypes.data),
+                 "inca_batch_evaluate")
+        return ti, tv
+
+    def ivec(self, m):
+        iv = np.zeros(max(self.engines_G[m], 1))
+        self._ck(self._L.inca_batch_get_ivec(self._h, int(m), iv.ctypes.data),
+                 "inca_batch_get_ivec")
+        return iv[:self.engines_G[m]]
+
+    def autotune(self):
+        b = C.c_int32()
+        self._ck(self._L.inca_batch_autotune(self._h, C.addressof(b)), "inca_batch_autotune")
+        return b.value
+
+    def info(self):
+        n, p, nb, r, ms = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        self._ck(self._L.inca_batch_info(self._h, C.addressof(n), C.addressof(p),
+                                         C.addressof(nb), C.addressof(r), C.addressof(ms)),
+                 "inca_batch_info")
+        return dict(items=n.value, points_per_lane=p.value, blocks=nb.value,
+                    replays=r.value, last_ms=ms.value)
+
+    def close(self):
+        if self._h:
+            self._L.inca_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Engine:
@@ -234,6 +325,13 @@ class Engine:
         self._ck(self._L.inca_evaluate(self._h, int(precision), float(thresh), _ptr(iv),
                                        cnt.ctypes.data), "inca_evaluate")
         return (iv[:self.G] if want_ivec else None), cnt
+
+    def prepare(self, precision=INCA_FP64, thresh=1e-12):
+        """Pre-pass only (inca_prepare); returns the four counters."""
+        cnt = np.zeros(4, np.int64)
+        self._ck(self._L.inca_prepare(self._h, int(precision), float(thresh), cnt.ctypes.data),
+                 "inca_prepare")
+        return cnt
 
     def ivec(self):
         iv = np.zeros(max(self.G, 1))

@@ -337,6 +337,38 @@ int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh, double *ivec,
   return 0;
 }
 
+int inca_prepare(inca_ctx *ctx, int32_t precision, double thresh, int64_t *counts) {
+  CHECK_CTX(ctx);
+  if (ctx->nprim == 0) return set_err(ctx, INCA_EINVAL, "primitives not set");
+  if (!ctx->d_rec) return set_err(ctx, INCA_EINVAL, "DM2 records not set");
+  if (precision != INCA_FP64 && precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "unknown precision %d", precision);
+  INCA_HIP(hipSetDevice(ctx->device));
+  int64_t cnt[4] = {0, 0, 0, 0};
+  int rc = prepare_quartets(ctx, thresh, precision, cnt);
+  if (rc) return rc;
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->have_ivec = 0;
+  if (counts)
+    for (int i = 0; i < 4; ++i) counts[i] = cnt[i];
+  return 0;
+}
+
+int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes, int32_t nohydro,
+                  int32_t betaone, const double *radius, int32_t cap, int32_t *nquad,
+                  double *cent, double *Ps, double *sfalpha) {
+  if (natoms < 1 || !an || !cartes || !nquad) return INCA_EINVAL;
+  std::vector<double> c, p, a;
+  const int n = autogrid(natoms, an, cartes, nohydro, betaone, radius, c, p, a);
+  *nquad = n;
+  if (cap < n) return INCA_ERANGE;
+  if (!cent || !Ps || !sfalpha) return INCA_EINVAL;
+  std::memcpy(cent, c.data(), 3 * (size_t)n * sizeof(double));
+  std::memcpy(Ps, p.data(), (size_t)n * sizeof(double));
+  std::memcpy(sfalpha, a.data(), (size_t)n * sizeof(double));
+  return 0;
+}
+
 int inca_set_mixed_tail(inca_ctx *ctx, double tail) {
   CHECK_CTX(ctx);
   if (!(tail >= 0.0 && tail <= 1.0)) return set_err(ctx, INCA_EINVAL, "tail must be in [0,1]");

@@ -1,0 +1,528 @@
+// Batched evaluation: many molecules' intracules in one persistent launch.
+//
+// The reference evaluates one molecule per run (inca_main.f90:76 ->
+// calculate_intracule).  A batch takes contexts whose grids and quartets are
+// already prepared (inca_prepare) and lays them out in one HBM arena:
+//   * each molecule's quartets, stably sorted by angular-momentum class
+//     (Lsum, Lx, Ly, Lz), so that every work item holds quartets of a
+//     single class and its cost is known in advance;
+//   * all grid points back to back, with per-molecule offsets.
+// Work items are (molecule, block tile of 256 P points, class chunk of
+// quartets); they are ordered by estimated cost, largest first, and a
+// persistent grid of workgroups pulls them from an atomic counter until the
+// list is exhausted, so small and large molecules share the device without
+// one launch per molecule.  Each item writes its own partial-sum slot and a
+// fixed-order reduction follows, so results do not depend on which
+// workgroup took which item.  The per-item arithmetic is the single-context
+// FP64 kernel's (eval_core.hpp).  Points per lane are autotuned over
+// {1, 2, 4, 8}; the launch sequence (counter reset, persistent kernel,
+// reduction, per-molecule integrals) is captured once into a hipGraph and
+// replayed by every inca_batch_evaluate.  The counter is cleared by a kernel
+// node, not a memset node.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "eval_core.hpp"
+
+struct inca_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nmol = 0;
+  int P = 4;
+  int nblocks = 0;  // persistent grid size
+  // per molecule
+  std::vector<int64_t> G, g_off, nq, q_off, part_off, nch;
+  std::vector<int32_t> stride;
+  std::vector<std::vector<std::array<int64_t, 3>>> segs;  // (first, count, class key)
+  int64_t Gtot = 0, Qdoubles = 0;
+  // device arena
+  double *d_q = nullptr, *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
+  double *d_w = nullptr, *d_wvee = nullptr, *d_ivec = nullptr, *d_res = nullptr;
+  double *d_partial = nullptr;
+  size_t partial_cap = 0;
+  int64_t *d_items = nullptr;
+  size_t items_cap = 0;
+  int64_t *d_mol = nullptr;   // per molecule: g_off, G, part_off, nch
+  int32_t *d_pmol = nullptr;  // molecule of every point
+  unsigned int *d_counter = nullptr;
+  int64_t nitems = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t replays = 0;
+  double last_ms = 0.0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace inca {
+namespace {
+
+constexpr int kItem = 6;  // int64 words per work item: mol, tile, qoff, n, stride, partial base
+
+int bfail(inca_batch *b, int code, const std::string &msg) {
+  b->err = msg;
+  return code;
+}
+
+#define B_HIP(call)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return bfail(b, INCA_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+__global__ void k_batch_reset(unsigned int *counter) {
+  if (threadIdx.x == 0) counter[0] = 0u;
+}
+
+template <int P>
+__global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ items,
+                                                   int64_t nitems, const double *__restrict__ q,
+                                                   const double *__restrict__ gx,
+                                                   const double *__restrict__ gy,
+                                                   const double *__restrict__ gz,
+                                                   const int64_t *__restrict__ mol,
+                                                   double *__restrict__ partial,
+                                                   unsigned int *__restrict__ counter) {
+  __shared__ unsigned int s_it;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (;;) {
+    if (threadIdx.x == 0) s_it = atomicAdd(counter, 1u);
+    __syncthreads();
+    const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane(s_it);
+    __syncthreads();  // s_it is read by every wave before the next fetch
+    if (it >= nitems) break;  // uniform over the block: every wave leaves together
+    const int64_t *__restrict__ d = items + it * kItem;
+    const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5];
+    const int stride = (int)d[4];
+    const int64_t goff = mol[4 * m], G = mol[4 * m + 1];
+    const int64_t base = bt * (kT * P) + (int64_t)wv * (64 * P) + lane;
+    double x[P], y[P], z[P], acc[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int64_t g = base + (int64_t)k * 64;
+      const bool ok = g < G;
+      x[k] = ok ? gx[goff + g] : 0.0;
+      y[k] = ok ? gy[goff + g] : 0.0;
+      z[k] = ok ? gz[goff + g] : 0.0;
+      acc[k] = 0.0;
+    }
+    const double *__restrict__ rec = q + qo;
+    for (int64_t i = 0; i < n; ++i, rec += stride) {
+      const QHeader h = load_header(rec);
+      accumulate_fp64<P>(rec, h, x, y, z, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int64_t g = base + (int64_t)k * 64;
+      if (g < G) partial[pb + g] = acc[k];
+    }
+  }
+}
+
+// Ivec of every point: its molecule's chunk partials in chunk order.
+__global__ __launch_bounds__(kT) void k_batch_reduce(const double *__restrict__ partial,
+                                                     const int32_t *__restrict__ pmol,
+                                                     const int64_t *__restrict__ mol,
+                                                     int64_t Gtot, double *__restrict__ ivec) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= Gtot) return;
+  const int m = pmol[g];
+  const int64_t goff = mol[4 * m], G = mol[4 * m + 1], po = mol[4 * m + 2], nc = mol[4 * m + 3];
+  const int64_t local = g - goff;
+  double s = 0.0;
+  for (int64_t c = 0; c < nc; ++c) s += partial[po + c * G + local];
+  ivec[g] = s;
+}
+
+// One block per molecule: sum w I and sum w_vee I in a fixed order
+// (mod_intracule.f90:441-457).
+__global__ __launch_bounds__(kT) void k_batch_integrate(const double *__restrict__ ivec,
+                                                        const double *__restrict__ w,
+                                                        const double *__restrict__ wvee,
+                                                        const int64_t *__restrict__ mol,
+                                                        double *__restrict__ res) {
+  __shared__ double r1[kT], r2[kT];
+  const int64_t m = blockIdx.x;
+  const int64_t goff = mol[4 * m], G = mol[4 * m + 1];
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t g = threadIdx.x; g < G; g += kT) {
+    s1 += w[goff + g] * ivec[goff + g];
+    s2 += wvee[goff + g] * ivec[goff + g];
+  }
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      r1[threadIdx.x] += r1[threadIdx.x + h];
+      r2[threadIdx.x] += r2[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[2 * m] = r1[0];
+    res[2 * m + 1] = r2[0];
+  }
+}
+
+// dst[r] = src[perm[r]] for whole records of `stride` doubles
+__global__ __launch_bounds__(kT) void k_batch_gather(const double *__restrict__ src,
+                                                     const int64_t *__restrict__ perm, int64_t n,
+                                                     int stride, double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= n * stride) return;
+  const int64_t r = t / stride;
+  const int w = (int)(t - r * stride);
+  dst[t] = src[perm[r] * stride + w];
+}
+
+__global__ __launch_bounds__(kT) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+template <class T>
+int bensure(inca_batch *b, T *&ptr, size_t &cap, size_t n) {
+  if (n <= cap && ptr) return 0;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  if (hipMalloc((void **)&ptr, n * sizeof(T)) != hipSuccess)
+    return bfail(b, INCA_ENOMEM, "batch allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  cap = n;
+  return 0;
+}
+
+// Items for points-per-lane P: chunks of at most qchunk quartets that never
+// straddle a class boundary, one item per (chunk, block tile), ordered by
+// estimated cost (quartets x (16 + 2 Lsum) x points), largest first.
+int build_items(inca_batch *b, int P) {
+  const int64_t tile_pts = (int64_t)kT * P;
+  double work = 0.0;
+  for (int m = 0; m < b->nmol; ++m)
+    work += (double)b->nq[m] * (double)((b->G[m] + tile_pts - 1) / tile_pts);
+  const double target = 16.0 * std::max(1, b->nblocks);
+  int64_t qchunk = (int64_t)std::ceil(work / target);
+  qchunk = std::max<int64_t>(32, std::min<int64_t>(qchunk, 1 << 20));
+  struct It {
+    std::array<int64_t, kItem> w;
+    double cost;
+  };
+  std::vector<It> items;
+  int64_t part = 0;
+  for (int m = 0; m < b->nmol; ++m) {
+    const int64_t tiles = (b->G[m] + tile_pts - 1) / tile_pts;
+    b->part_off[m] = part;
+    int64_t c = 0;
+    for (const auto &sg : b->segs[m]) {
+      const int lsum = (int)(sg[2] >> 24);
+      for (int64_t s = 0; s < sg[1]; s += qchunk, ++c) {
+        const int64_t n = std::min(qchunk, sg[1] - s);
+        const int64_t qo = b->q_off[m] + (sg[0] + s) * b->stride[m];
+        for (int64_t t = 0; t < tiles; ++t) {
+          const int64_t pts = std::min(tile_pts, b->G[m] - t * tile_pts);
+          It it;
+          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m]};
+          it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts;
+          items.push_back(it);
+        }
+      }
+    }
+    b->nch[m] = c;
+    part += c * b->G[m];
+  }
+  std::stable_sort(items.begin(), items.end(),
+                   [](const It &a, const It &c) { return a.cost > c.cost; });
+  b->nitems = (int64_t)items.size();
+  std::vector<int64_t> flat((size_t)std::max<int64_t>(1, b->nitems) * kItem);
+  for (size_t i = 0; i < items.size(); ++i)
+    std::copy(items[i].w.begin(), items[i].w.end(), flat.begin() + i * kItem);
+  int rc;
+  if ((rc = bensure(b, b->d_items, b->items_cap, flat.size()))) return rc;
+  if ((rc = bensure(b, b->d_partial, b->partial_cap, (size_t)std::max<int64_t>(1, part)))) return rc;
+  B_HIP(hipMemcpy(b->d_items, flat.data(), flat.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  std::vector<int64_t> mi(4 * (size_t)b->nmol);
+  for (int m = 0; m < b->nmol; ++m) {
+    mi[4 * m] = b->g_off[m];
+    mi[4 * m + 1] = b->G[m];
+    mi[4 * m + 2] = b->part_off[m];
+    mi[4 * m + 3] = b->nch[m];
+  }
+  B_HIP(hipMemcpy(b->d_mol, mi.data(), mi.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// the evaluation sequence on b->stream (captured into the graph)
+int enqueue(inca_batch *b, int P) {
+  hipStream_t st = b->stream;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counter);
+  if (b->nitems > 0) {
+#define INCA_BEVAL(PP)                                                                        \
+  hipLaunchKernelGGL(k_batch_eval<PP>, dim3(b->nblocks), dim3(kT), 0, st, b->d_items,         \
+                     b->nitems, b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_mol, b->d_partial,     \
+                     b->d_counter)
+    switch (P) {
+      case 1: INCA_BEVAL(1); break;
+      case 2: INCA_BEVAL(2); break;
+      case 8: INCA_BEVAL(8); break;
+      default: INCA_BEVAL(4); break;
+    }
+#undef INCA_BEVAL
+  }
+  if (b->Gtot > 0)
+    hipLaunchKernelGGL(k_batch_reduce, dim3(nblk(b->Gtot)), dim3(kT), 0, st, b->d_partial,
+                       b->d_pmol, b->d_mol, b->Gtot, b->d_ivec);
+  hipLaunchKernelGGL(k_batch_integrate, dim3(b->nmol), dim3(kT), 0, st, b->d_ivec, b->d_w,
+                     b->d_wvee, b->d_mol, b->d_res);
+  B_HIP(hipGetLastError());
+  return 0;
+}
+
+int persistent_blocks(inca_batch *b, int P) {
+  int dev = b->device, ncu = 0, per = 0;
+  B_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  hipError_t e = hipSuccess;
+  switch (P) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_eval<1>, kT, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_eval<2>, kT, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_eval<8>, kT, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_eval<4>, kT, 0); break;
+  }
+  B_HIP(e);
+  b->nblocks = std::max(1, ncu * std::max(1, per));
+  return 0;
+}
+
+int capture(inca_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue(b, b->P);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  B_HIP(e);
+  b->graph = g;
+  B_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
+  b->nmol = nmol;
+  b->G.assign(nmol, 0);
+  b->g_off.assign(nmol, 0);
+  b->nq.assign(nmol, 0);
+  b->q_off.assign(nmol, 0);
+  b->part_off.assign(nmol, 0);
+  b->nch.assign(nmol, 0);
+  b->stride.assign(nmol, 0);
+  b->segs.assign(nmol, {});
+  for (int m = 0; m < nmol; ++m) {
+    const inca_ctx *c = mols[m];
+    if (!c) return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": null context");
+    if (c->device != b->device)
+      return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + " is on another device");
+    if (c->grid_kind == GRID_NONE)
+      return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": no grid");
+    if (c->qstride == 0)
+      return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": quartets not prepared");
+    b->G[m] = c->G;
+    b->g_off[m] = b->Gtot;
+    b->Gtot += c->G;
+    b->nq[m] = c->q2;
+    b->stride[m] = c->qstride;
+    b->q_off[m] = b->Qdoubles;
+    b->Qdoubles += c->q2 * c->qstride;
+  }
+  const size_t gt = (size_t)std::max<int64_t>(1, b->Gtot);
+  const size_t qd = (size_t)std::max<int64_t>(1, b->Qdoubles);
+  size_t cap = 0;
+  int rc;
+  double **garr[6] = {&b->d_gx, &b->d_gy, &b->d_gz, &b->d_w, &b->d_wvee, &b->d_ivec};
+  for (double **a : garr)
+    if ((cap = 0, rc = bensure(b, *a, cap, gt))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_q, cap, qd))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_res, cap, 2 * (size_t)nmol))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_mol, cap, 4 * (size_t)nmol))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_pmol, cap, gt))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_counter, cap, 1))) return rc;
+  hipStream_t st = b->stream;
+  int64_t *d_perm = nullptr;
+  size_t perm_cap = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const inca_ctx *c = mols[m];
+    const int64_t G = c->G, n = c->q2;
+    if (G > 0) {
+      const double *src[5] = {c->d_gx, c->d_gy, c->d_gz, c->d_w, c->d_wvee};
+      double *dst[5] = {b->d_gx, b->d_gy, b->d_gz, b->d_w, b->d_wvee};
+      for (int a = 0; a < 5; ++a)
+        B_HIP(hipMemcpyAsync(dst[a] + b->g_off[m], src[a], G * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(kT), 0, st, b->d_pmol + b->g_off[m], G,
+                         (int32_t)m);
+      B_HIP(hipGetLastError());
+    }
+    if (n == 0) continue;
+    // class of every prepared record (header word 4), sorted stably on the host
+    std::vector<double> cls((size_t)n);
+    B_HIP(hipStreamSynchronize(c->stream));
+    B_HIP(hipMemcpy2D(cls.data(), sizeof(double), c->d_q + 4, c->qstride * sizeof(double),
+                      sizeof(double), (size_t)n, hipMemcpyDeviceToHost));
+    std::vector<int64_t> key((size_t)n), perm((size_t)n);
+    for (int64_t r = 0; r < n; ++r) {
+      uint64_t bits;
+      std::memcpy(&bits, &cls[r], sizeof(bits));
+      const int64_t lx = bits & 0xff, ly = (bits >> 8) & 0xff, lz = (bits >> 16) & 0xff;
+      key[r] = ((lx + ly + lz) << 24) | (int64_t)(bits & 0xffffff);
+    }
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t c2) { return key[a] < key[c2]; });
+    for (int64_t r = 0; r < n;) {
+      int64_t e = r;
+      while (e < n && key[perm[e]] == key[perm[r]]) ++e;
+      b->segs[m].push_back({r, e - r, key[perm[r]]});
+      r = e;
+    }
+    if ((rc = bensure(b, d_perm, perm_cap, (size_t)n))) return rc;
+    B_HIP(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_batch_gather, dim3(nblk(n * c->qstride)), dim3(kT), 0, st, c->d_q, d_perm,
+                       n, c->qstride, b->d_q + b->q_off[m]);
+    B_HIP(hipGetLastError());
+    B_HIP(hipStreamSynchronize(st));  // d_perm is reused by the next molecule
+  }
+  if (d_perm) (void)hipFree(d_perm);
+  B_HIP(hipStreamSynchronize(st));
+  if ((rc = persistent_blocks(b, b->P))) return rc;
+  if ((rc = build_items(b, b->P))) return rc;
+  return capture(b);
+}
+
+}  // namespace
+}  // namespace inca
+
+using namespace inca;
+
+extern "C" {
+
+int inca_batch_create(int32_t device, int32_t nmol, inca_ctx *const *mols, inca_batch **out) {
+  if (!out) return INCA_EINVAL;
+  *out = nullptr;
+  if (nmol < 1 || !mols) return INCA_EINVAL;
+  inca_batch *b = new inca_batch();
+  b->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev[1]);
+  int rc = (e == hipSuccess) ? setup(b, nmol, mols)
+                             : bfail(b, INCA_EHIP, std::string("batch set-up: ") + hipGetErrorString(e));
+  *out = b;  // returned even on failure so that the message can be read
+  return rc;
+}
+
+int inca_batch_destroy(inca_batch *b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void *bufs[] = {b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_w, b->d_wvee, b->d_ivec, b->d_res,
+                  b->d_partial, b->d_items, b->d_mol, b->d_pmol, b->d_counter};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  for (auto &ev : b->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+const char *inca_batch_last_error(const inca_batch *b) { return b ? b->err.c_str() : ""; }
+
+int inca_batch_evaluate(inca_batch *b, double *integral, double *vee) {
+  if (!b || !b->exec) return INCA_EINVAL;
+  B_HIP(hipSetDevice(b->device));
+  B_HIP(hipEventRecord(b->ev[0], b->stream));
+  B_HIP(hipGraphLaunch(b->exec, b->stream));
+  B_HIP(hipEventRecord(b->ev[1], b->stream));
+  std::vector<double> res(2 * (size_t)b->nmol);
+  B_HIP(hipMemcpyAsync(res.data(), b->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       b->stream));
+  B_HIP(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->last_ms = ms;
+  ++b->replays;
+  for (int m = 0; m < b->nmol; ++m) {
+    if (integral) integral[m] = res[2 * m];
+    if (vee) vee[m] = res[2 * m + 1];
+  }
+  return 0;
+}
+
+int inca_batch_get_ivec(inca_batch *b, int32_t mol, double *ivec) {
+  if (!b || !ivec) return INCA_EINVAL;
+  if (mol < 0 || mol >= b->nmol) return bfail(b, INCA_EINVAL, "molecule index out of range");
+  if (b->G[mol] == 0) return 0;
+  B_HIP(hipSetDevice(b->device));
+  B_HIP(hipMemcpy(ivec, b->d_ivec + b->g_off[mol], b->G[mol] * sizeof(double),
+                  hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int inca_batch_autotune(inca_batch *b, int32_t *best) {
+  if (!b) return INCA_EINVAL;
+  B_HIP(hipSetDevice(b->device));
+  const int cand[4] = {1, 2, 4, 8};
+  float best_ms = 1e30f;
+  int best_p = b->P, rc;
+  for (int c = 0; c < 4; ++c) {
+    const int P = cand[c];
+    if ((rc = persistent_blocks(b, P)) || (rc = build_items(b, P))) return rc;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {  // the first launch warms up
+      B_HIP(hipEventRecord(b->ev[0], b->stream));
+      if ((rc = enqueue(b, P))) return rc;
+      B_HIP(hipEventRecord(b->ev[1], b->stream));
+      B_HIP(hipEventSynchronize(b->ev[1]));
+      B_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_p = P;
+    }
+  }
+  b->P = best_p;
+  if ((rc = persistent_blocks(b, b->P)) || (rc = build_items(b, b->P)) || (rc = capture(b)))
+    return rc;
+  if (best) *best = best_p;
+  return 0;
+}
+
+int inca_batch_info(inca_batch *b, int64_t *nitems, int32_t *points_per_lane, int32_t *blocks,
+                    int64_t *replays, double *last_ms) {
+  if (!b) return INCA_EINVAL;
+  if (nitems) *nitems = b->nitems;
+  if (points_per_lane) *points_per_lane = b->P;
+  if (blocks) *blocks = b->nblocks;
+  if (replays) *replays = b->replays;
+  if (last_ms) *last_ms = b->last_ms;
+  return 0;
+}
+
+}  // extern "C"

@@ -170,6 +170,12 @@ int build_pair_table(inca_ctx *ctx);
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision,
                      int64_t *counts);
 
+// autogrid.hip
+double bragg_slater_bohr(int z);
+int autogrid(int natoms, const int32_t *an, const double *cartes, int nohydro, int betaone,
+             const double *radius, std::vector<double> &cent, std::vector<double> &Ps,
+             std::vector<double> &sfalpha);
+
 // launchers (eval.hip)
 int evaluate_grid(inca_ctx *ctx, int precision);
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist);

@@ -28,6 +28,7 @@ extern "C" {
 #endif
 
 typedef struct inca_ctx inca_ctx;
+typedef struct inca_batch inca_batch;
 
 enum {
   INCA_OK = 0,
@@ -122,6 +123,57 @@ int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
  * or j,k,l outside 1..nprim) as the reference's reader does (:239-240). */
 int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh,
                   double *ivec, int64_t *counts);
+
+/* The pre-pass of inca_evaluate alone (decode, screens, prefactors,
+ * polynomials, compaction: mod_intracule.f90:229-344) without the grid
+ * loop; the prepared quartets stay on the device for inca_batch_create.
+ * counts as in inca_evaluate.  No reference counterpart as a separate
+ * call. */
+int inca_prepare(inca_ctx *ctx, int32_t precision, double thresh,
+                 int64_t *counts);
+
+/* Integration centres of the multicentre grid, replacing centercalc
+ * (mod_autogrid.f90:12-131): the origin, +-(R_j - R_k) for heavy-atom pairs
+ * (all pairs when nohydro = 0), z snapped to 0 below 0.5 bohr, centres
+ * within 0.1 bohr merged.  cartes(natoms,3) column-major as geninfo::cartes
+ * (bohr).  radius: per-atom radii in bohr, or NULL for the Bragg-Slater
+ * table of get_bs_radius (mod_autogrid.f90:142-185).  *nquad receives the
+ * number of centres; if cap < *nquad nothing is written and INCA_ERANGE is
+ * returned (call with cap = 0 to size the arrays).  cent(3,nquad), Ps and
+ * sfalpha(nquad) as intrainfo::cent/Ps/sfalpha. */
+int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes,
+                  int32_t nohydro, int32_t betaone, const double *radius,
+                  int32_t cap, int32_t *nquad, double *cent, double *Ps,
+                  double *sfalpha);
+
+/* Batched evaluation of many molecules (no reference counterpart: the
+ * reference runs one molecule per process, inca_main.f90:76).  Every
+ * context must be on `device` and hold a grid and prepared quartets
+ * (inca_prepare or inca_evaluate); the batch copies both into its own
+ * arena, so the contexts may be destroyed afterwards.  Quartets are sorted
+ * by class (Lsum, Lx, Ly, Lz) per molecule; work items (molecule, 256 P
+ * point tile, single-class chunk) are ordered largest first and consumed by
+ * a persistent kernel from an atomic work counter; the whole evaluation is
+ * captured into a hipGraph at creation and replayed by inca_batch_evaluate.
+ * FP64 only.  On failure *out may still be set: read inca_batch_last_error
+ * and call inca_batch_destroy. */
+int inca_batch_create(int32_t device, int32_t nmol, inca_ctx *const *mols,
+                      inca_batch **out);
+int inca_batch_destroy(inca_batch *b);
+const char *inca_batch_last_error(const inca_batch *b);
+/* One replay of the captured graph: integral[m] = sum w I and vee[m] =
+ * sum w/|u| I per molecule (mod_intracule.f90:441-457); either may be
+ * NULL. */
+int inca_batch_evaluate(inca_batch *b, double *integral, double *vee);
+/* Ivec of molecule `mol` after the last inca_batch_evaluate. */
+int inca_batch_get_ivec(inca_batch *b, int32_t mol, double *ivec);
+/* Times points-per-lane 1, 2, 4, 8 on the batch, keeps the fastest in
+ * *best and re-captures the graph with it. */
+int inca_batch_autotune(inca_batch *b, int32_t *best);
+/* Work items, points per lane, persistent workgroups, graph replays and
+ * the device time of the last replay in ms. */
+int inca_batch_info(inca_batch *b, int64_t *nitems, int32_t *points_per_lane,
+                    int32_t *blocks, int64_t *replays, double *last_ms);
 
 /* Mixed precision by magnitude: after screening, each quartet's term
  * A exp(-|rp|^2) Vx Vy Vz gets a bound B on its L1 norm over all space

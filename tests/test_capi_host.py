@@ -80,3 +80,51 @@ def test_create_fails_cleanly_without_gpu():
         pass
     with pytest.raises(capi.IncaError):
         capi.Engine(0)
+
+
+def _molecule(seed):
+    rng = np.random.default_rng(seed)
+    nat = int(rng.integers(2, 9))
+    an = rng.choice([1, 1, 6, 7, 8, 9, 16], size=nat).astype(np.int32)
+    return an, rng.normal(scale=2.0, size=(nat, 3))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_autogrid_matches_oracle_centercalc(seed):
+    """inca_autogrid (the product's centercalc) against the oracle's."""
+    an, cartes = _molecule(seed)
+    for nohydro in (True, False):
+        for betaone in (False, True):
+            got = capi.autogrid(an, cartes, nohydro, betaone)
+            ref = O.centercalc(an, cartes, nohydro, betaone)
+            for g, r in zip(got, ref):
+                assert g.shape == r.shape and np.array_equal(g, r)
+
+
+def test_autogrid_merges_repeated_displacements():
+    """A chain of four equally spaced heavy atoms: +-d, +-2d repeat and merge
+    with population-weighted radii (mod_autogrid.f90:83-99)."""
+    an = np.array([6, 6, 8, 8], np.int32)
+    cartes = np.array([[0, 0, 0], [0, 0, 2.5], [0, 0, 5.0], [0, 0, 7.5]], float)
+    cent, Ps, sf = capi.autogrid(an, cartes)
+    ref = O.centercalc(an, cartes)
+    assert len(Ps) == 7 and np.array_equal(cent, ref[0])
+    assert np.array_equal(Ps, ref[1]) and np.array_equal(sf, ref[2])
+    # origin: sum Z(Z-1)/2; the +-2.5 centres merge three pairs (6*6, 6*8, 8*8)/2
+    assert Ps[0] == 2 * 15 + 2 * 28
+    i = int(np.argmin(np.abs(cent[:, 2] - 2.5) + np.abs(cent[:, :2]).sum(axis=1)))
+    assert Ps[i] == (36 + 48 + 64) / 2
+
+
+def test_autogrid_radius_hook():
+    """Explicit per-atom radii equal to the Bragg-Slater table reproduce the
+    default; other radii change only the radial scales."""
+    an, cartes = _molecule(3)
+    bs = np.array([O.bs_radius(z) for z in an])
+    base = capi.autogrid(an, cartes, False, False)
+    same = capi.autogrid(an, cartes, False, False, radius=bs)
+    for a, b in zip(base, same):
+        assert np.array_equal(a, b)
+    other = capi.autogrid(an, cartes, False, False, radius=bs * 1.3)
+    assert np.array_equal(other[0], base[0]) and np.array_equal(other[1], base[1])
+    assert np.allclose(other[2], base[2] * 1.3, rtol=1e-14)
