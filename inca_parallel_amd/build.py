@@ -30,6 +30,7 @@ HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 FC = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
 
 HIP_SOURCES = ["util.hip", "grid.hip", "quartet.hip", "eval.hip", "autogrid.hip", "batch.hip",
+               "mlp.hip",
                "api.hip"]
 # the Fortran modules in dependency order
 F_SOURCES = ["mod_inca_hip.f90", "mod_state.f90", "mod_locate.f90",
@@ -62,7 +63,7 @@ def build_hip(verbose=False):
             print(out)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(7, os.cpu_count() or 1)) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(one, HIP_SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libinca_hip.so")
 
 INCA_FP64 = 0
 INCA_MIXED = 1
+INCA_ERANGE = -6
 
 # every symbol include/inca_hip.h declares
 SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitives",
@@ -29,6 +30,9 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_prepare", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
+           "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
+           "inca_mlp_set_params", "inca_mlp_forward", "inca_mlp_loss_grad", "inca_mlp_adam_step",
+           "inca_atom_features", "inca_learned_autogrid",
            "inca_lebedev", "inca_gauleg"]
 
 _lib = None
@@ -91,10 +95,22 @@ def load():
     L.inca_batch_get_ivec.argtypes = [vp, C.c_int32, vp]
     L.inca_batch_autotune.argtypes = [vp, vp]
     L.inca_batch_info.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.inca_mlp_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(vp)]
+    L.inca_mlp_destroy.argtypes = [vp]
+    L.inca_mlp_last_error.argtypes = [vp]
+    L.inca_mlp_last_error.restype = C.c_char_p
+    L.inca_mlp_get_params.argtypes = [vp, vp]
+    L.inca_mlp_set_params.argtypes = [vp, vp]
+    L.inca_mlp_forward.argtypes = [vp, C.c_int32, vp, vp]
+    L.inca_mlp_loss_grad.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+    L.inca_mlp_adam_step.argtypes = [vp, C.c_float]
+    L.inca_atom_features.argtypes = [C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp]
+    L.inca_learned_autogrid.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp, vp, vp, C.c_int32,
+                                        C.c_int32, vp, vp, vp, vp, vp, vp, vp]
     L.inca_lebedev.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
     L.inca_gauleg.argtypes = [C.c_int32, _dp, _dp]
     for s in SYMBOLS:
-        if s not in ("inca_last_error", "inca_batch_last_error"):
+        if s not in ("inca_last_error", "inca_batch_last_error", "inca_mlp_last_error"):
             getattr(L, s).restype = C.c_int
     _lib = L
     return L
@@ -139,6 +155,112 @@ def autogrid(an, cartes, nohydro=True, betaone=False, radius=None):
     if rc != 0:
         raise IncaError("inca_autogrid failed (%d)" % rc)
     return cent, Ps, sf
+
+
+MLP_IN, MLP_OUT, MLP_NPARAM = 16, 9, 9993
+NRAD_CLASSES = (50, 75, 100, 150)
+NANG_CLASSES = (110, 194, 302, 590)
+
+
+def _molecule_arrays(an, cartes, alpha, tmn, prim_atom):
+    an = np.ascontiguousarray(an, np.int32)
+    n = len(an)
+    cm = np.ascontiguousarray(np.asarray(cartes, np.float64).reshape(n, 3).T.reshape(-1))
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    npr = len(alpha)
+    tm = np.ascontiguousarray(np.asarray(tmn, np.int32).reshape(npr, 3).T.reshape(-1))
+    ra = np.ascontiguousarray(prim_atom, np.int32)
+    return an, cm, alpha, tm, ra
+
+
+def atom_features(an, cartes, alpha, tmn, prim_atom):
+    """(natoms,16) float32 inputs of the learned autogrid (inca_atom_features)."""
+    L = load()
+    an, cm, alpha, tm, ra = _molecule_arrays(an, cartes, alpha, tmn, prim_atom)
+    f = np.zeros((len(an), MLP_IN), np.float32)
+    rc = L.inca_atom_features(len(an), an.ctypes.data, cm.ctypes.data, len(alpha),
+                              alpha.ctypes.data, tm.ctypes.data, ra.ctypes.data, f.ctypes.data)
+    if rc != 0:
+        raise IncaError("inca_atom_features failed (%d)" % rc)
+    return f
+
+
+class MLP:
+    """The learned-autogrid network on one device (inca_mlp_*)."""
+
+    def __init__(self, device=0, seed=1):
+        self._L = load()
+        self._h = C.c_void_p()
+        rc = self._L.inca_mlp_create(int(device), int(seed), C.byref(self._h))
+        if rc != 0:
+            msg = self._L.inca_mlp_last_error(self._h).decode() if self._h else ""
+            self.close()
+            raise IncaError("inca_mlp_create failed (%d): %s" % (rc, msg))
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise IncaError("%s failed (%d): %s" % (
+                what, rc, self._L.inca_mlp_last_error(self._h).decode()))
+
+    def params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        self._ck(self._L.inca_mlp_get_params(self._h, p.ctypes.data), "inca_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_NPARAM
+        self._ck(self._L.inca_mlp_set_params(self._h, p.ctypes.data), "inca_mlp_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros((x.shape[0], MLP_OUT), np.float32)
+        self._ck(self._L.inca_mlp_forward(self._h, x.shape[0], x.ctypes.data, out.ctypes.data),
+                 "inca_mlp_forward")
+        return out
+
+    def loss_grad(self, x, target, want_grad=True):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_double()
+        g = np.zeros(MLP_NPARAM, np.float32) if want_grad else None
+        self._ck(self._L.inca_mlp_loss_grad(self._h, x.shape[0], x.ctypes.data, t.ctypes.data,
+                                            C.addressof(loss), _ptr(g)), "inca_mlp_loss_grad")
+        return loss.value, g
+
+    def adam_step(self, lr):
+        self._ck(self._L.inca_mlp_adam_step(self._h, float(lr)), "inca_mlp_adam_step")
+
+    def learned_autogrid(self, an, cartes, alpha, tmn, prim_atom, nohydro=True):
+        """cent (nq,3), Ps, sfalpha, nradc, nangc, per-atom radii."""
+        an, cm, alpha, tm, ra = _molecule_arrays(an, cartes, alpha, tmn, prim_atom)
+        nq = C.c_int32()
+        args = (self._h, len(an), an.ctypes.data, cm.ctypes.data, len(alpha), alpha.ctypes.data,
+                tm.ctypes.data, ra.ctypes.data, int(bool(nohydro)))
+        rc = self._L.inca_learned_autogrid(*args, 0, C.addressof(nq), None, None, None, None,
+                                           None, None)
+        if rc not in (0, INCA_ERANGE):
+            self._ck(rc, "inca_learned_autogrid")
+        n = nq.value
+        cent, Ps, sf = np.zeros((n, 3)), np.zeros(n), np.zeros(n)
+        nr, na = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        rad = np.zeros(len(an))
+        self._ck(self._L.inca_learned_autogrid(*args, n, C.addressof(nq), cent.ctypes.data,
+                                               Ps.ctypes.data, sf.ctypes.data, nr.ctypes.data,
+                                               na.ctypes.data, rad.ctypes.data),
+                 "inca_learned_autogrid")
+        return cent, Ps, sf, nr, na, rad
+
+    def close(self):
+        if self._h:
+            self._L.inca_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -8,6 +8,7 @@
 // and Bragg-Slater radii (angstrom x 1.8897) scale the radial maps.  A
 // caller may pass its own per-atom radii instead of the Bragg-Slater table:
 // the centre formulas stay those of the reference.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -88,6 +89,68 @@ int autogrid(int natoms, const int32_t *an, const double *cartes, int nohydro, i
     sfalpha.push_back(betaone ? 1.0 : cs[i].alpha);
   }
   return (int)Ps.size();
+}
+
+// Per-atom descriptors of the learned autogrid (16 values, each scaled to
+// order one): element, exponent range and angular momentum of the atom's
+// primitives, its neighbourhood, and molecule-wide exponent range.  prim_atom
+// is 1-based (geninfo::Ra); tmn(nprim,3) column-major.
+void atom_features(int natoms, const int32_t *an, const double *cartes, int nprim,
+                   const double *alpha, const int32_t *tmn, const int32_t *prim_atom, float *f) {
+  auto xyz = [&](int a, int ax) { return cartes[a + (size_t)natoms * ax]; };
+  double gmin = HUGE_VAL, gmax = 0.0;
+  for (int i = 0; i < nprim; ++i) {
+    gmin = std::fmin(gmin, alpha[i]);
+    gmax = std::fmax(gmax, alpha[i]);
+  }
+  if (nprim == 0) gmin = gmax = 1.0;
+  for (int a = 0; a < natoms; ++a) {
+    double amin = HUGE_VAL, amax = 0.0, cnt = 0.0;
+    int lmax = 0;
+    for (int i = 0; i < nprim; ++i) {
+      if (prim_atom[i] != a + 1) continue;
+      amin = std::fmin(amin, alpha[i]);
+      amax = std::fmax(amax, alpha[i]);
+      cnt += 1.0;
+      lmax = std::max(lmax, tmn[i] + tmn[i + nprim] + tmn[i + 2 * nprim]);
+    }
+    if (cnt == 0.0) amin = amax = 1.0;
+    double dmin = 0.0, dsum = 0.0, near = 0.0, coul = 0.0;
+    bool first = true;
+    for (int b = 0; b < natoms; ++b) {
+      if (b == a) continue;
+      double d2 = 0.0;
+      for (int ax = 0; ax < 3; ++ax) {
+        const double d = xyz(a, ax) - xyz(b, ax);
+        d2 += d * d;
+      }
+      const double d = std::sqrt(d2);
+      dmin = first ? d : std::fmin(dmin, d);
+      first = false;
+      dsum += d;
+      near += (d < 4.0) ? 1.0 : 0.0;
+      coul += (d > 1e-8) ? an[b] / d : 0.0;
+    }
+    const int z = an[a];
+    const int period = z <= 2 ? 1 : (z <= 10 ? 2 : (z <= 18 ? 3 : 4));
+    float *o = f + (size_t)a * 16;
+    o[0] = (float)(z / 10.0);
+    o[1] = (float)(z == 1 ? 1.0 : 0.0);
+    o[2] = (float)(period / 3.0);
+    o[3] = (float)(std::log10(amin) / 3.0);
+    o[4] = (float)(std::log10(amax) / 3.0);
+    o[5] = (float)(cnt / 20.0);
+    o[6] = (float)(lmax / 2.0);
+    o[7] = (float)(dmin / 5.0);
+    o[8] = (float)(natoms > 1 ? dsum / (natoms - 1) / 5.0 : 0.0);
+    o[9] = (float)(near / 10.0);
+    o[10] = (float)(natoms / 20.0);
+    o[11] = (float)(bragg_slater_bohr(z) / 3.0);
+    o[12] = (float)(coul / 10.0);
+    o[13] = (float)(std::log10(gmin) / 3.0);
+    o[14] = (float)(std::log10(gmax) / 3.0);
+    o[15] = 1.0f;
+  }
 }
 
 }  // namespace inca

@@ -175,6 +175,8 @@ double bragg_slater_bohr(int z);
 int autogrid(int natoms, const int32_t *an, const double *cartes, int nohydro, int betaone,
              const double *radius, std::vector<double> &cent, std::vector<double> &Ps,
              std::vector<double> &sfalpha);
+void atom_features(int natoms, const int32_t *an, const double *cartes, int nprim,
+                   const double *alpha, const int32_t *tmn, const int32_t *prim_atom, float *f);
 
 // launchers (eval.hip)
 int evaluate_grid(inca_ctx *ctx, int precision);

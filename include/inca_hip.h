@@ -175,6 +175,54 @@ int inca_batch_autotune(inca_batch *b, int32_t *best);
 int inca_batch_info(inca_batch *b, int64_t *nitems, int32_t *points_per_lane,
                     int32_t *blocks, int64_t *replays, double *last_ms);
 
+/* Learned autogrid (no reference counterpart).  A per-atom network
+ * 16 -> 64 -> 64 -> 64 -> 9 (GELU, FP32; hand-written HIP kernels for
+ * inference and training) predicts a factor on the Bragg-Slater radius that
+ * get_bs_radius would return (mod_autogrid.f90:36, 61-63) and classes for
+ * nrad in {50, 75, 100, 150} and nang in {110, 194, 302, 590}, which replace
+ * the one nrad/nang broadcast to every centre (:127-131).  Parameters are a
+ * flat FP32 array of INCA_MLP_NPARAM values: W1 (16x64, [in][out]), b1, W2
+ * (64x64), b2, W3 (64x64), b3, W4 (64x9), b4. */
+#define INCA_MLP_IN 16
+#define INCA_MLP_OUT 9
+#define INCA_MLP_NPARAM 9993
+typedef struct inca_mlp inca_mlp;
+/* Deterministic initial weights from `seed` (uniform, variance-scaled). */
+int inca_mlp_create(int32_t device, uint64_t seed, inca_mlp **out);
+int inca_mlp_destroy(inca_mlp *m);
+const char *inca_mlp_last_error(const inca_mlp *m);
+int inca_mlp_get_params(inca_mlp *m, float *params);
+/* Also resets the optimiser state. */
+int inca_mlp_set_params(inca_mlp *m, const float *params);
+/* out(n,9) = network(x(n,16)), row-major. */
+int inca_mlp_forward(inca_mlp *m, int32_t n, const float *x, float *out);
+/* Loss over n atoms with targets (n,3) = [log radius factor, nrad class
+ * 0..3, nang class 0..3]: mean of (out0 - t0)^2 + the two softmax
+ * cross-entropies.  Computes the gradient on the device (kept for
+ * inca_mlp_adam_step) and copies it to `grad` (INCA_MLP_NPARAM) unless
+ * NULL. */
+int inca_mlp_loss_grad(inca_mlp *m, int32_t n, const float *x, const float *target,
+                       double *loss, float *grad);
+/* One Adam step (beta 0.9/0.999, eps 1e-8, bias-corrected) with the last
+ * gradient. */
+int inca_mlp_adam_step(inca_mlp *m, float lr);
+/* The network's 16 inputs per atom: feat(natoms,16) row-major.  an,
+ * cartes(natoms,3) column-major in bohr; primitives as geninfo::Alpha,
+ * TMN(nprim,3) column-major and Ra (1-based atom of each primitive). */
+int inca_atom_features(int32_t natoms, const int32_t *an, const double *cartes, int32_t nprim,
+                       const double *alpha, const int32_t *tmn, const int32_t *prim_atom,
+                       float *feat);
+/* Centres as inca_autogrid with the predicted radii (betaone off), plus
+ * per-centre nradc/nangc: a centre takes the largest classes among the
+ * atoms it stands for (the origin: all atoms; a pair centre: its pair).
+ * radius (natoms, may be NULL) receives the radii used.  cap/ERANGE as in
+ * inca_autogrid. */
+int inca_learned_autogrid(inca_mlp *m, int32_t natoms, const int32_t *an, const double *cartes,
+                          int32_t nprim, const double *alpha, const int32_t *tmn,
+                          const int32_t *prim_atom, int32_t nohydro, int32_t cap, int32_t *nquad,
+                          double *cent, double *Ps, double *sfalpha, int32_t *nradc,
+                          int32_t *nangc, double *radius);
+
 /* Mixed precision by magnitude: after screening, each quartet's term
  * A exp(-|rp|^2) Vx Vy Vz gets a bound B on its L1 norm over all space
  * (|A| prod_axis sum_k |C_k| Gamma((k+1)/2) / sqe) and its exact integral M.
