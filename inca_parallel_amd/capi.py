@@ -27,7 +27,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
-           "inca_prepare", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
+           "inca_prepare", "inca_evaluate_prepared", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
@@ -85,6 +85,7 @@ def load():
     L.inca_get_analytic_integral.argtypes = [vp, vp]
     L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
+    L.inca_evaluate_prepared.argtypes = [vp, C.c_int32, vp]
     L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
                                 vp, vp, vp]
     L.inca_batch_create.argtypes = [C.c_int32, C.c_int32, vp, C.POINTER(vp)]
@@ -454,6 +455,13 @@ class Engine:
         self._ck(self._L.inca_prepare(self._h, int(precision), float(thresh), cnt.ctypes.data),
                  "inca_prepare")
         return cnt
+
+    def evaluate_prepared(self, precision=INCA_FP64, want_ivec=True):
+        """Grid loop only, over the quartets already prepared (inca_evaluate_prepared)."""
+        iv = np.zeros(max(self.G, 1)) if want_ivec else None
+        self._ck(self._L.inca_evaluate_prepared(self._h, int(precision), _ptr(iv)),
+                 "inca_evaluate_prepared")
+        return iv[:self.G] if want_ivec else None
 
     def ivec(self):
         iv = np.zeros(max(self.G, 1))

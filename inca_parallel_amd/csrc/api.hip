@@ -354,6 +354,30 @@ int inca_prepare(inca_ctx *ctx, int32_t precision, double thresh, int64_t *count
   return 0;
 }
 
+int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec) {
+  CHECK_CTX(ctx);
+  if (ctx->grid_kind == GRID_NONE) return set_err(ctx, INCA_EINVAL, "grid not set");
+  if (ctx->qstride == 0) return set_err(ctx, INCA_EINVAL, "quartets not prepared");
+  if (precision != INCA_FP64 && precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "unknown precision %d", precision);
+  if (precision == INCA_MIXED && ctx->prep_precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "quartets were prepared for FP64: prepare with INCA_MIXED");
+  INCA_HIP(hipSetDevice(ctx->device));
+  INCA_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  int rc = evaluate_grid(ctx, precision);
+  if (rc) return rc;
+  INCA_HIP(hipEventSynchronize(ctx->ev[3]));
+  float b = 0, c = 0;
+  INCA_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+  INCA_HIP(hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]));
+  ctx->t_ms[0] = 0.0;
+  ctx->t_ms[1] = b;
+  ctx->t_ms[2] = c;
+  if (ivec && ctx->G > 0)
+    INCA_HIP(hipMemcpy(ivec, ctx->d_ivec, ctx->G * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes, int32_t nohydro,
                   int32_t betaone, const double *radius, int32_t cap, int32_t *nquad,
                   double *cent, double *Ps, double *sfalpha) {

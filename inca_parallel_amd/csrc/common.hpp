@@ -74,6 +74,7 @@ struct inca_ctx {
 
   // prepared quartets
   int qstride = 0;       // doubles per prepared record
+  int prep_precision = 0; // precision the records were prepared for (mixed: split done)
   int64_t q2 = 0;
   int64_t q_fp64 = 0;         // mixed mode: leading records evaluated in FP64
   double mixed_tail = 0.02;   // FP32 tail budget: sum of its L1 bounds / |integral|

@@ -580,6 +580,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride))) return rc;
   ctx->q_fp64 = q2;
   ctx->mixed_tau = 0.0;
+  ctx->prep_precision = precision;
   if (precision == INCA_MIXED && q2 > 0 && ctx->mixed_tail > 0.0) {
     // Precision by magnitude.  FP32 roundoff in a term is bounded by a few
     // ulps of its L1 norm, bounded by B; the lightest terms go to the FP32

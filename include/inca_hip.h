@@ -132,6 +132,12 @@ int inca_evaluate(inca_ctx *ctx, int32_t precision, double thresh,
 int inca_prepare(inca_ctx *ctx, int32_t precision, double thresh,
                  int64_t *counts);
 
+/* The grid loop alone (mod_intracule.f90:345-381) over the quartets the
+ * last inca_prepare / inca_evaluate prepared, on the current grid: for
+ * evaluating one DM2 on several grids.  INCA_MIXED needs quartets prepared
+ * with INCA_MIXED (the FP64/FP32 split is made in the pre-pass). */
+int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec);
+
 /* Integration centres of the multicentre grid, replacing centercalc
  * (mod_autogrid.f90:12-131): the origin, +-(R_j - R_k) for heavy-atom pairs
  * (all pairs when nohydro = 0), z snapped to 0 below 0.5 bohr, centres
