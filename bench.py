@@ -134,6 +134,21 @@ def batch_bench(capi, synth, torch, nmol=48, reps=3):
     }
 
 
+PROFILE = os.path.join(ROOT, "profiles", "eval_fp64_radial_plot_cluster151.md")
+
+
+def profiled_traffic():
+    """HBM bytes per k_eval_fp64 launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy)."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch k_eval_fp64"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +157,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--natoms", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--tiles", default=None,
+                    help="points per lane 'fp64,fp32' instead of autotuning (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one device copy of 1 GiB and exit: the reference run for the "
+                         "rocprofv3 byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -150,6 +170,15 @@ def main():
 
     import torch
     from inca_parallel_amd import capi, synth
+
+    if a.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes" % (src.numel() * 8))
+        return 0
 
     t0 = time.perf_counter()
     sysm = synth.config_cluster(seed=2, natoms=a.natoms)
@@ -166,9 +195,14 @@ def main():
     log("grid: %d radii, %d points" % (nradi, G))
 
     # tile autotuning on this grid and these quartets (outside the timed steps)
-    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
-    tiles = eng.autotune()
-    log("autotuned points per lane: fp64 %d, fp32 %d" % tiles)
+    if a.tiles:
+        tiles = tuple(int(v) for v in a.tiles.split(","))
+        eng.set_tiles(*tiles)
+        log("points per lane set: fp64 %d, fp32 %d" % tiles)
+    else:
+        eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+        tiles = eng.autotune()
+        log("autotuned points per lane: fp64 %d, fp32 %d" % tiles)
 
     results = {}
     for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
@@ -224,9 +258,12 @@ def main():
         "roofline": {
             "bound": "valu", "kernel": "k_eval_fp64",
             "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_VALU, "traffic": None,
-            "note": "FLOP = 16 + 2*Lsum per eval (FMA = 2), exp not counted; "
-                    "eval kernel %.1f ms per launch" % (eval_s * 1e3),
+            "frac": achieved / PEAK_FP64_VALU, "traffic": profiled_traffic(),
+            "note": "FLOP = 16 + 2*Lsum per eval (FMA = 2), exp not counted; eval kernel %.1f "
+                    "ms per launch (HIP events on the engine stream); traffic = HBM bytes per "
+                    "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s); the kernel is bound by "
+                    "FP64 VALU issue (39.4 VALU lane-instructions per eval, issue busy 0.86)"
+                    % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
         "mixed": {
             "value": evals / rm["step_s"], "unit": "evals/s",
@@ -235,6 +272,9 @@ def main():
             "eval_kernel_ms": float(rm["kern_ms"][1]),
             "quartets_fp64": int(rm["split"][0]), "quartets_fp32": int(rm["split"][1]),
         },
+        "dm2_stream": {"bytes": 24 * Q, "prep_ms": float(r64["kern_ms"][0]),
+                       "GB_per_s": 24 * Q / (float(r64["kern_ms"][0]) * 1e-3) / 1e9,
+                       "note": "24-byte .dm2p records decoded, screened and prepared per step"},
         "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
                       "reduce": float(r64["kern_ms"][2])},
     }
