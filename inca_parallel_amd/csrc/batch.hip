@@ -91,6 +91,8 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
                                                    double *__restrict__ partial,
                                                    unsigned int *__restrict__ counter) {
   __shared__ unsigned int s_it;
+  __shared__ double s_exp2[64];
+  stage_exp_table(s_exp2);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (;;) {
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
     const double *__restrict__ rec = q + qo;
     for (int64_t i = 0; i < n; ++i, rec += stride) {
       const QHeader h = load_header(rec);
-      accumulate_fp64<P>(rec, h, x, y, z, acc);
+      accumulate_fp64<P>(rec, h, x, y, z, acc, s_exp2);
     }
 #pragma unroll
     for (int k = 0; k < P; ++k) {

@@ -344,3 +344,27 @@ def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     print("max |term| / peak bound beyond the cutoff radius: %.3e (2^-100 = %.3e)" % (
         worst, 2.0 ** -100))
     assert worst <= 2.0 ** -100
+
+
+def test_fp64_exp_accuracy_through_one_s_primitive(engine):
+    """One s primitive (alpha = 1) and the record (1,1,1,1): sqe = 1 and
+    d = 0, so I(r) / I(0) = exp(-|r|^2) exactly.  Points on the x axis with
+    x^2 exact in double sweep s = |r|^2 over [0, 700]; the device exp must
+    agree with numpy's to 1e-15 relative (a few ulp)."""
+    engine.set_primitives(np.array([1.0]), np.zeros((1, 3), np.int32), np.zeros(1), np.zeros(1),
+                          np.zeros(1))
+    engine.set_quartets(ijkl=np.array([[1, 1, 1, 1]], np.int32), d=np.array([1.0]))
+    m = np.unique(np.concatenate([np.arange(0, 1 << 12), np.linspace(0, 27093, 4001).astype(np.int64)
+                                  * 1000]))
+    x = m / 1024.0
+    x = x[x * x <= 700.0]
+    pts = np.zeros((len(x), 3))
+    pts[:, 0] = x
+    engine.grid_points(pts)
+    iv, _ = engine.evaluate(capi.INCA_FP64, 0.0)
+    ref = np.exp(-(x * x))
+    keep = ref > 1e-300
+    err = np.abs(iv[keep] / iv[0] - ref[keep]) / ref[keep]
+    print("exp over s in [0, %.1f]: max rel err %.2e at s = %.3f (%d points)" % (
+        (x * x).max(), err.max(), (x[keep] ** 2)[np.argmax(err)], keep.sum()))
+    assert iv[0] > 0 and err.max() < 1e-15
