@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
                                                    double *__restrict__ partial,
                                                    unsigned int *__restrict__ counter) {
   __shared__ unsigned int s_it;
-  __shared__ double s_exp2[64];
+  __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -13,9 +13,9 @@
 //
 // Arithmetic.  This is exp-bound VALU work (no MFMA: nothing here is a
 // matrix product).  FP64 path: rp by one FMA per axis, exp(-s) by
-// reduction by ln2/64 to |r| <= ln2/128, a degree-5 Taylor polynomial of e^r
-// (truncation error 3.5e-17), a 64-entry table of 2^(j/64) in LDS and an
-// exponent shift (eval_core.hpp).  Mixed path: rp in FP64 (the cancellation-prone step),
+// reduction by ln2/256 to |r| <= ln2/512, a degree-4 Taylor polynomial of
+// e^r (truncation error 3.8e-17), a 256-entry table of 2^(j/256) in LDS and
+// an exponent shift (eval_core.hpp).  Mixed path: rp in FP64 (the cancellation-prone step),
 // everything after it in FP32 -- hardware exp2, FP32 Horner -- with a
 // compensated (Kahan) FP32 sum over tiles of quartets that is folded into
 // an FP64 accumulator at the end of every tile.
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   const double *__restrict__ tbw = tb + tile * kTB;
   const double tcx = tbw[0], tcy = tbw[1], tcz = tbw[2], trlo = tbw[3], trhi = tbw[4];
   const bool empty = !(tbw[5] > 0.0);
-  __shared__ double s_exp2[64];
+  __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
   double x[P], y[P], z[P], acc[P];
 #pragma unroll

@@ -10,51 +10,105 @@ namespace inca {
 
 constexpr int kT = 256;  // lanes per block in every evaluation kernel
 
-// 2^(j/64), j = 0..63, each the double nearest the exact value (computed
+// 2^(j/256), j = 0..255, each the double nearest the exact value (computed
 // with 80-digit decimal arithmetic).  Kernels that evaluate exp_neg stage it
-// in LDS (stage_exp_table) and read it with a per-lane index.
-static __constant__ double c_exp2_64[64] = {
-    1, 1.0108892860517005, 1.0218971486541166, 1.0330248790212284,
-    1.0442737824274138, 1.0556451783605572, 1.0671404006768237, 1.0787607977571199,
-    1.0905077326652577, 1.1023825833078409, 1.1143867425958924, 1.1265216186082418,
-    1.1387886347566916, 1.1511892299529827, 1.1637248587775775, 1.1763969916502812,
-    1.189207115002721, 1.2021567314527031, 1.215247359980469, 1.22848053610687,
-    1.241857812073484, 1.2553807570246911, 1.2690509571917332, 1.2828700160787783,
-    1.2968395546510096, 1.3109612115247644, 1.3252366431597413, 1.3396675240533029,
-    1.3542555469368927, 1.3690024229745905, 1.383909881963832, 1.3989796725383112,
-    1.4142135623730951, 1.42961333839197, 1.4451808069770467, 1.460917794180647,
-    1.4768261459394993, 1.4929077282912648, 1.5091644275934228, 1.5255981507445384,
-    1.5422108254079407, 1.5590044002378369, 1.5759808451078865, 1.593142151342267,
-    1.6104903319492543, 1.6280274218573478, 1.6457554781539649, 1.6636765803267364,
-    1.681792830507429, 1.7001063537185235, 1.7186192981224779, 1.7373338352737062,
-    1.7562521603732995, 1.7753764925265212, 1.7947090750031072, 1.8142521755003989,
-    1.8340080864093424, 1.8539791250833855, 1.8741676341103, 1.8945759815869656,
-    1.9152065613971474, 1.9360617934922943, 1.9571441241754002, 1.9784560263879509};
+// in LDS (stage_exp_table, 2 KB) and read it with a per-lane index.
+constexpr int kExpTable = 256;
+static __constant__ double c_exp2_256[kExpTable] = {
+    1, 1.0027112750502025, 1.0054299011128027, 1.0081558981184175,
+    1.0108892860517005, 1.0136300849514894, 1.0163783149109531, 1.0191339960777379,
+    1.0218971486541166, 1.0246677928971357, 1.0274459491187637, 1.030231637686041,
+    1.0330248790212284, 1.0358256936019572, 1.0386341019613787, 1.0414501246883161,
+    1.0442737824274138, 1.0471050958792898, 1.0499440858006872, 1.0527907730046264,
+    1.0556451783605572, 1.0585073227945128, 1.0613772272892621, 1.0642549128844645,
+    1.0671404006768237, 1.0700337118202419, 1.0729348675259756, 1.075843889062791,
+    1.0787607977571199, 1.0816856149932152, 1.0846183622133092, 1.0875590609177697,
+    1.0905077326652577, 1.0934643990728858, 1.0964290818163769, 1.0994018026302219,
+    1.1023825833078409, 1.1053714457017412, 1.1083684117236787, 1.1113735033448175,
+    1.1143867425958924, 1.1174081515673693, 1.1204377524096067, 1.1234755673330199,
+    1.1265216186082418, 1.1295759285662881, 1.1326385195987192, 1.1357094141578055,
+    1.1387886347566916, 1.1418762039695616, 1.1449721444318042, 1.1480764788401789,
+    1.1511892299529827, 1.1543104205902159, 1.1574400736337511, 1.1605782120274988,
+    1.1637248587775775, 1.1668800369524817, 1.1700437696832502, 1.1732160801636373,
+    1.1763969916502812, 1.1795865274628758, 1.182784710984341, 1.1859915656609938,
+    1.189207115002721, 1.1924313825831512, 1.1956643920398273, 1.1989061670743806,
+    1.2021567314527031, 1.2054161090051239, 1.2086843236265816, 1.2119613992768012,
+    1.215247359980469, 1.2185422298274085, 1.2218460329727576, 1.2251587936371455,
+    1.22848053610687, 1.2318112847340759, 1.2351510639369334, 1.2384998981998165,
+    1.241857812073484, 1.245224830175258, 1.2486009771892048, 1.2519862778663162,
+    1.2553807570246911, 1.2587844395497165, 1.2621973503942507, 1.2656195145788063,
+    1.2690509571917332, 1.2724917033894028, 1.275941778396392, 1.2794012075056693,
+    1.2828700160787783, 1.2863482295460256, 1.2898358734066657, 1.2933329732290895,
+    1.2968395546510096, 1.3003556433796506, 1.3038812651919358, 1.3074164459346773,
+    1.3109612115247644, 1.3145155879493546, 1.318079601266064, 1.3216532776031575,
+    1.3252366431597413, 1.3288297242059544, 1.3324325470831615, 1.3360451382041458,
+    1.3396675240533029, 1.3432997311868353, 1.3469417862329458, 1.3505937158920345,
+    1.3542555469368927, 1.3579273062129011, 1.3616090206382248, 1.3653007172040119,
+    1.3690024229745905, 1.3727141650876684, 1.3764359707545302, 1.380167867260238,
+    1.383909881963832, 1.3876620422985291, 1.3914243757719262, 1.3951969099662003,
+    1.3989796725383112, 1.4027726912202048, 1.4065759938190154, 1.4103896082172707,
+    1.4142135623730951, 1.4180478843204152, 1.4218926021691656, 1.4257477441054942,
+    1.42961333839197, 1.4334894133677889, 1.4373759974489824, 1.4412731191286257,
+    1.4451808069770467, 1.449099089642035, 1.4530279958490526, 1.4569675544014438,
+    1.460917794180647, 1.4648787441464057, 1.4688504333369818, 1.4728328908693675,
+    1.4768261459394993, 1.4808302278224719, 1.4848451658727524, 1.488870989524397,
+    1.4929077282912648, 1.4969554117672355, 1.5010140696264256, 1.5050837316234065,
+    1.5091644275934228, 1.5132561874526098, 1.5173590411982147, 1.5214730189088146,
+    1.5255981507445384, 1.529734466947287, 1.5338819978409559, 1.5380407738316568,
+    1.5422108254079407, 1.5463921831410214, 1.550584877685, 1.5547889397770887,
+    1.5590044002378369, 1.5632312899713576, 1.567469639965553, 1.5717194812923414,
+    1.5759808451078865, 1.5802537626528246, 1.5845382652524937, 1.588834384317164,
+    1.593142151342267, 1.5974615979086271, 1.6017927556826934, 1.606135656416771,
+    1.6104903319492543, 1.6148568142048607, 1.6192351351948637, 1.6236253270173289,
+    1.6280274218573478, 1.632441451987275, 1.6368674497669644, 1.6413054476440063,
+    1.6457554781539649, 1.6502175739206177, 1.6546917676561943, 1.6591780921616162,
+    1.6636765803267364, 1.6681872651305825, 1.6727101796415966, 1.6772453570178785,
+    1.681792830507429, 1.6863526334483934, 1.6909247992693053, 1.6955093614893326,
+    1.7001063537185235, 1.7047158096580513, 1.7093377631004629, 1.713972247929926,
+    1.7186192981224779, 1.723278947746274, 1.7279512309618377, 1.7326361820223111,
+    1.7373338352737062, 1.7420442251551564, 1.746767386199169, 1.7515033530318782,
+    1.7562521603732995, 1.7610138430375839, 1.7657884359332727, 1.7705759740635547,
+    1.7753764925265212, 1.7801900265154245, 1.785016611318935, 1.789856282321401,
+    1.7947090750031072, 1.7995750249405351, 1.8044541678066239, 1.809346539371032,
+    1.8142521755003989, 1.8191711121586085, 1.8241033854070534, 1.8290490314048973,
+    1.8340080864093424, 1.8389805867758937, 1.843966568958626, 1.8489660695104508,
+    1.8539791250833855, 1.8590057724288205, 1.864046048397789, 1.8690999899412386,
+    1.8741676341103, 1.8792490180565602, 1.8843441790323345, 1.8894531543909392,
+    1.8945759815869656, 1.8997126981765553, 1.9048633418176741, 1.9100279502703899,
+    1.9152065613971474, 1.9203992131630474, 1.925605943636125, 1.9308267909876271,
+    1.9360617934922943, 1.9413109895286405, 1.9465744175792332, 1.9518521162309783,
+    1.9571441241754002, 1.9624504802089273, 1.9677712232331759, 1.9731063922552343,
+    1.9784560263879509, 1.9838201648502194, 1.9891988469672663, 1.9945921121709402};
 
 // Copy the table into the block's LDS; every thread of the block must call it.
 __device__ __forceinline__ void stage_exp_table(double *lds) {
-  if (threadIdx.x < 64) lds[threadIdx.x] = c_exp2_64[threadIdx.x];
+  for (int i = threadIdx.x; i < kExpTable; i += blockDim.x) lds[i] = c_exp2_256[i];
   __syncthreads();
 }
 
-// exp(-s) for s >= 0 in FP64.  k = rint(-64 s / ln 2) (s clamped at 1000,
-// where exp(-s) is 0 in double), r = -s - k ln2/64 by a Cody-Waite split of
-// ln2/64 (hi: 22 significant bits, so k hi is exact for |k| < 2^31), |r| <=
-// ln2/128; e^r by its degree-5 Taylor polynomial (truncation error
-// (ln2/128)^6/720 = 3.5e-17); then e^-s = e^r 2^(j/64) 2^n with k = 64 n + j,
-// j in [0, 63], from the LDS table.  13 FP64 operations.
+// exp(-s) for s >= 0 in FP64, 11 FP64 operations:
+//   s is clamped at 1000 (exp(-1000) is 0 in double);
+//   t = -256 s / ln 2 + 1.5 2^52 rounds to an integer k in one FMA (k < 0,
+//     |k| < 2^19) and leaves k in the low word of t; kd = t - 1.5 2^52;
+//   r = -s - k ln2/256 by a Cody-Waite split of ln2/256 (hi: 22 significant
+//     bits, so k hi is exact), |r| <= ln2/512;
+//   e^r by its degree-4 Taylor polynomial (truncation error
+//     (ln2/512)^5/120 = 3.8e-17);
+//   e^-s = e^r 2^(j/256) 2^n with k = 256 n + j, j in [0, 255], the table
+//     entry from LDS and 2^n by ldexp.
 __device__ __forceinline__ double exp_neg(double s, const double *__restrict__ tbl) {
+  const double kMagic = 6755399441055744.0;  // 1.5 2^52
   s = fmin(s, 1000.0);
-  const double kd = rint(s * -92.33248261689366);  // -64 / ln 2
-  const int k = (int)kd;
-  double r = fma(kd, -0.01083042100071907, -s);    // ln2/64, high part
-  r = fma(kd, -3.695530075025074e-09, r);          // ln2/64, low part
-  double p = fma(r, 1.0 / 120.0, 1.0 / 24.0);
-  p = fma(p, r, 1.0 / 6.0);
+  const double t = fma(s, -369.3299304675746, kMagic);  // -256 / ln 2
+  const int k = __double2loint(t);
+  const double kd = t - kMagic;
+  double r = fma(kd, -0.0027076052501797676, -s);    // ln2/256, high part
+  r = fma(kd, -9.238825187562685e-10, r);          // ln2/256, low part
+  double p = fma(r, 1.0 / 24.0, 1.0 / 6.0);
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  return ldexp(p * tbl[k & 63], k >> 6);
+  return ldexp(p * tbl[k & (kExpTable - 1)], k >> 8);
 }
 
 struct QHeader {
