@@ -218,7 +218,8 @@ def main():
         dt = (time.perf_counter() - t0) / a.steps
         kern = np.array(kern)
         results[name] = dict(step_s=dt, counts=cnt, kern_ms=kern.mean(axis=0),
-                             rintra=eng.radial_values(), split=eng.mixed_split())
+                             rintra=eng.radial_values(), split=eng.mixed_split(),
+                             evals=eng.eval_counts())
         log("%s: %.3f s/step, kernels (prep, eval, reduce) ms = %s, counts %s" % (
             name, dt, np.round(kern.mean(axis=0), 2), cnt))
 
@@ -230,7 +231,10 @@ def main():
     r64 = results["fp64"]
     rm = results["mixed"]
     eval_s = r64["kern_ms"][1] * 1e-3
-    achieved = evals * W / eval_s
+    executed = float(r64["evals"][1])
+    # the roofline counts the work the kernel executed (the cutoff skips
+    # (quartet, tile) pairs whose terms are below 2^-100 of their peak)
+    achieved = executed * W / eval_s
     ref = r64["rintra"]
     mixed_err = float(np.max(np.abs(rm["rintra"] - ref) / np.maximum(np.abs(ref), 1e-300)))
 
@@ -253,17 +257,19 @@ def main():
                         "(%d points), thresh 1e-12" % (sysm.nprim, Q, G),
             "records": Q, "quartets_after_screen1": int(cnt[1]),
             "quartets_computed": q2, "grid_points": G, "flop_per_eval": W,
+            "reference_evals": evals, "executed_evals": executed,
+            "point_cutoff": "2^-100 of each term's peak (executed/reference = %.3f)"
+                            % (executed / evals),
             "points_per_lane": {"fp64": tiles[0], "fp32": tiles[1]},
         },
         "roofline": {
             "bound": "valu", "kernel": "k_eval_fp64",
             "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU, "traffic": profiled_traffic(),
-            "note": "FLOP = 16 + 2*Lsum per eval (FMA = 2), exp not counted; eval kernel %.1f "
-                    "ms per launch (HIP events on the engine stream); traffic = HBM bytes per "
-                    "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s); the kernel is bound by "
-                    "FP64 VALU issue (39.4 VALU lane-instructions per eval, issue busy 0.86)"
-                    % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
+            "note": "FLOP = 16 + 2*Lsum per executed eval (FMA = 2), exp not counted; eval "
+                    "kernel %.1f ms per launch (HIP events on the engine stream); traffic = HBM "
+                    "bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s); the kernel is "
+                    "bound by FP64 VALU issue" % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
         "mixed": {
             "value": evals / rm["step_s"], "unit": "evals/s",

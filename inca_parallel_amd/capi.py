@@ -27,6 +27,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
+           "inca_get_eval_counts",
            "inca_prepare", "inca_evaluate_prepared", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
@@ -84,6 +85,7 @@ def load():
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_get_analytic_integral.argtypes = [vp, vp]
     L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
+    L.inca_get_eval_counts.argtypes = [vp, vp, vp]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
     L.inca_evaluate_prepared.argtypes = [vp, C.c_int32, vp]
     L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
@@ -514,6 +516,13 @@ class Engine:
         the 2^-log2_rel radius of each quartet."""
         self._ck(self._L.inca_set_point_cutoff(self._h, float(log2_rel)),
                  "inca_set_point_cutoff")
+
+    def eval_counts(self):
+        """(reference evals, executed evals) of the last grid loop."""
+        a, b = C.c_int64(), C.c_int64()
+        self._ck(self._L.inca_get_eval_counts(self._h, C.addressof(a), C.addressof(b)),
+                 "inca_get_eval_counts")
+        return a.value, b.value
 
     def set_polynomials(self, exact):
         self._ck(self._L.inca_set_polynomials(self._h, int(bool(exact))), "inca_set_polynomials")

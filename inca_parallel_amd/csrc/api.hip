@@ -77,7 +77,7 @@ int inca_destroy(inca_ctx *c) {
   void *bufs[] = {c->d_prim, c->d_tmn, c->d_pair, c->d_gx, c->d_gy, c->d_gz, c->d_w,
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
-                  c->d_gcent};
+                  c->d_gcent, c->d_mask, c->d_evals};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
@@ -413,6 +413,13 @@ int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel) {
   if (!(log2_rel == 0.0 || (log2_rel >= 64.0 && log2_rel <= 1000.0)))
     return set_err(ctx, INCA_EINVAL, "point cutoff must be 0 (off) or in [64, 1000]");
   ctx->cut_log2 = log2_rel;
+  return 0;
+}
+
+int inca_get_eval_counts(inca_ctx *ctx, int64_t *reference, int64_t *executed) {
+  CHECK_CTX(ctx);
+  if (reference) *reference = ctx->evals_reference;
+  if (executed) *executed = ctx->evals_executed;
   return 0;
 }
 

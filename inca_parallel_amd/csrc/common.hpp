@@ -82,7 +82,7 @@ struct inca_ctx {
   double term_integral = 0.0; // sum of the prepared terms' integrals over all space
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   int poly_exact = 0;         // 1: exact axis polynomials instead of the fits
-  double cut_log2 = 0.0;      // per-point cutoff 2^-cut_log2 of a term's peak (0: off)
+  double cut_log2 = 100.0;    // per-point cutoff 2^-cut_log2 of a term's peak (0: off)
   double *d_q = nullptr;      // compacted prepared records
   size_t q_cap = 0;
   double *d_qtmp = nullptr;   // uncompacted records (slot = record index)
@@ -100,6 +100,11 @@ struct inca_ctx {
   unsigned long long *d_cnt = nullptr;  // device counters
   double *d_tb = nullptr;     // per wave tile: anchor xyz, rho min/max, flag
   size_t tb_cap = 0;
+  unsigned long long *d_mask = nullptr;  // cutoff skip bits [64-quartet word][wave tile]
+  size_t mask_cap = 0;
+  unsigned long long *d_evals = nullptr;  // executed evaluations of the last grid loop
+  size_t evals_cap = 0;
+  int64_t evals_reference = 0, evals_executed = 0;
   double *d_gcent = nullptr;  // multicentre grid: centre coordinates (3*nquad)
   double *d_partial = nullptr;
   size_t partial_cap = 0;

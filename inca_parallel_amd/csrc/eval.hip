@@ -127,6 +127,77 @@ __device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy,
   return (u2 >= o * o) || (i > 0.0 && u2 <= i * i);
 }
 
+// Skip mask: bit i of mask[w][t] is set when quartet 64 w + i is beyond its
+// cutoff radius at every point of wave tile t.  One wave per 64-quartet word:
+// lane i holds quartet 64 w + i, a ballot per tile gives the word, and the
+// words of 64 consecutive tiles are stored by the 64 lanes at once.
+__global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, int qstride,
+                                                  int64_t nq, const double *__restrict__ tb,
+                                                  int64_t ntile, int64_t nwords,
+                                                  unsigned long long *__restrict__ mask) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (w >= nwords) return;  // uniform per wave
+  const int64_t qi = w * 64 + lane;
+  const bool valid = qi < nq;
+  QHeader h{};
+  if (valid) h = load_header(q + qi * qstride);
+  for (int64_t t0 = 0; t0 < ntile; t0 += 64) {
+    unsigned long long keep = 0;
+    const int nt = (int)((ntile - t0) < 64 ? (ntile - t0) : 64);
+    for (int j = 0; j < nt; ++j) {
+      const double *__restrict__ tbw = tb + (t0 + j) * kTB;
+      const bool far = valid && tbw[5] > 0.0 && tile_far(h, tbw[0], tbw[1], tbw[2], tbw[3], tbw[4]);
+      const unsigned long long m = __ballot(far);
+      if (lane == j) keep = m;
+    }
+    if (lane < nt) mask[w * ntile + t0 + lane] = keep;
+  }
+}
+
+// Executed evaluations under the skip mask: for every word and tile, the
+// kept quartets times the tile's points (integer atomics: exact, order-free).
+__global__ __launch_bounds__(kT) void k_mask_evals(const unsigned long long *__restrict__ mask,
+                                                   const double *__restrict__ tb, int64_t ntile,
+                                                   int64_t nwords, int64_t nq,
+                                                   unsigned long long *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < nwords * ntile) {
+    const int64_t w = i / ntile, t = i - w * ntile;
+    unsigned long long live = ~mask[i];
+    const int64_t left = nq - w * 64;
+    if (left < 64) live &= (1ull << left) - 1ull;
+    v = (unsigned long long)__popcll(live) * (unsigned long long)tb[t * kTB + 5];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+// Calls f(qi) for the quartets of [a, b) that the skip mask keeps for
+// `tile` (all of them when mask is null).  The mask words are wave-uniform,
+// so the iteration runs on scalar bit scans.
+template <class F>
+__device__ __forceinline__ void for_live(const unsigned long long *__restrict__ mask, int64_t ntile,
+                                         int64_t tile, int64_t a, int64_t b, F &&f) {
+  if (!mask) {
+    for (int64_t qi = a; qi < b; ++qi) f(qi);
+    return;
+  }
+  for (int64_t wd = a >> 6; (wd << 6) < b; ++wd) {
+    const int64_t base = wd << 6;
+    unsigned long long live = ~mask[wd * ntile + tile];
+    if (base < a) live &= ~0ull << (a - base);
+    if (b - base < 64) live &= (1ull << (b - base)) - 1ull;
+    while (live) {
+      const int i = __builtin_ctzll(live);
+      live &= live - 1ull;
+      f(base + i);
+    }
+  }
+}
+
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
@@ -134,13 +205,12 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
                                                   const double *__restrict__ gy,
                                                   const double *__restrict__ gz, int64_t G,
                                                   const double *__restrict__ tb,
-                                                  double *__restrict__ partial) {
+                                                  const unsigned long long *__restrict__ mask,
+                                                  int64_t ntile, double *__restrict__ partial) {
   const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
                        __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
-  const double *__restrict__ tbw = tb + tile * kTB;
-  const double tcx = tbw[0], tcy = tbw[1], tcz = tbw[2], trlo = tbw[3], trhi = tbw[4];
-  const bool empty = !(tbw[5] > 0.0);
+  const bool empty = !(tb[tile * kTB + 5] > 0.0);
   __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
   double x[P], y[P], z[P], acc[P];
@@ -155,13 +225,11 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   }
   const int64_t q0 = (int64_t)blockIdx.y * qchunk;
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
-  for (int64_t qi = q0; qi < q1; ++qi) {
+  for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
     const QHeader h = load_header(rec);
-    if (CUT && __builtin_amdgcn_readfirstlane((int)tile_far(h, tcx, tcy, tcz, trlo, trhi)))
-      continue;
     accumulate_fp64<P>(rec, h, x, y, z, acc, s_exp2);
-  }
+  });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const int64_t g = base + (int64_t)k * 64;
@@ -180,13 +248,12 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
                                                    const double *__restrict__ gy,
                                                    const double *__restrict__ gz, int64_t G,
                                                    const double *__restrict__ tb,
-                                                   double *__restrict__ partial) {
+                                                   const unsigned long long *__restrict__ mask,
+                                                   int64_t ntile, double *__restrict__ partial) {
   const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
                        __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
-  const double *__restrict__ tbw = tb + tile * kTB;
-  const double tcx = tbw[0], tcy = tbw[1], tcz = tbw[2], trlo = tbw[3], trhi = tbw[4];
-  const bool empty = !(tbw[5] > 0.0);
+  const bool empty = !(tb[tile * kTB + 5] > 0.0);
   double x[P], y[P], z[P], acc[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -211,11 +278,9 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
       sum[k] = 0.0f;
       comp[k] = 0.0f;
     }
-    for (int64_t qi = t0; qi < t1; ++qi) {
+    for_live(CUT ? mask : nullptr, ntile, tile, t0, t1, [&](int64_t qi) {
       const double *__restrict__ rec = q + qi * qstride;
       const QHeader h = load_header(rec);
-      if (CUT && __builtin_amdgcn_readfirstlane((int)tile_far(h, tcx, tcy, tcz, trlo, trhi)))
-        continue;
       const double *__restrict__ cx = rec + kQHead;
       const double *__restrict__ cy = cx + h.Lx + 1;
       const double *__restrict__ cz = cy + h.Ly + 1;
@@ -255,7 +320,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
         comp[k] = (tk - sum[k]) - yk;
         sum[k] = tk;
       }
-    }
+    });
 #pragma unroll
     for (int k = 0; k < P; ++k) acc[k] += (double)sum[k] - (double)comp[k];
   }
@@ -290,6 +355,10 @@ __global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q,
   __syncthreads();
   if (threadIdx.x < 64 && threadIdx.x < nbins && h[threadIdx.x])
     atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void k_zero_u64(unsigned long long *p) {
+  if (threadIdx.x == 0) p[0] = 0ull;
 }
 
 __global__ void k_zero(double *p, int64_t n) {
@@ -339,6 +408,15 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
                        ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
                        ctx->d_tb);
+    const int64_t nwords = (n + 63) / 64;
+    if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, (size_t)(nwords > 0 ? nwords : 1) * ntile)))
+      return rc;
+    if (nwords > 0) {
+      hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((nwords + 3) / 4)), dim3(kT), 0, st, q,
+                         ctx->qstride, n, ctx->d_tb, ntile, nwords, ctx->d_mask);
+      hipLaunchKernelGGL(k_mask_evals, dim3((unsigned)((nwords * ntile + kT - 1) / kT)), dim3(kT),
+                         0, st, ctx->d_mask, ctx->d_tb, ntile, nwords, n, ctx->d_evals);
+    }
   } else {
     // only the point-count word is read: mark every tile non-empty
     hipLaunchKernelGGL(k_fill_tiles, dim3((unsigned)((ntile + kT - 1) / kT)), dim3(kT), 0, st,
@@ -348,10 +426,12 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
 #define INCA_EVAL2(PP, CC)                                                                     \
   if (mixed)                                                                                   \
     hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch, \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, partial);                 \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
+                       partial);                                                                \
   else                                                                                         \
     hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, partial);
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
+                       partial);
 #define INCA_EVAL(PP)      \
   if (cut) {               \
     INCA_EVAL2(PP, true)   \
@@ -400,6 +480,8 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
     ctx->have_ivec = 1;
     return 0;
   }
+  if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ctx->d_evals);
   // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
   const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
   const int64_t nr[2] = {n64, nq - n64};
@@ -426,6 +508,14 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   INCA_HIP(hipGetLastError());
   INCA_HIP(hipEventRecord(ctx->ev[3], st));
   ctx->have_ivec = 1;
+  ctx->evals_reference = (int64_t)nq * G;
+  ctx->evals_executed = ctx->evals_reference;
+  if (ctx->cut_log2 > 0.0) {
+    unsigned long long ex = 0;
+    INCA_HIP(hipMemcpyAsync(&ex, ctx->d_evals, sizeof(ex), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipStreamSynchronize(st));
+    ctx->evals_executed = (int64_t)ex;
+  }
   return 0;
 }
 

@@ -96,6 +96,18 @@ module inca_hip
          type(c_ptr), value :: ctx
          integer(c_int32_t), value :: mode
       end function
+      integer(c_int) function inca_set_point_cutoff(ctx, log2_rel) &
+            bind(C, name="inca_set_point_cutoff")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: log2_rel
+      end function
+      integer(c_int) function inca_get_eval_counts(ctx, reference, executed) &
+            bind(C, name="inca_get_eval_counts")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), intent(out) :: reference, executed
+      end function
       integer(c_int) function inca_integrate(ctx, integral, vee, ipc, vpc) &
             bind(C, name="inca_integrate")
          import :: c_int, c_ptr

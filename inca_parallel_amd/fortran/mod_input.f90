@@ -2,9 +2,10 @@
 !! src/mod_input_reader.f90: the blocks &files, &intracule_job and &cube_job
 !! (each optional; a syntax error stops the run), the same defaults, the
 !! calc_type dispatch and the default output names derived from the input
-!! file name.  &intracule_job also accepts three engine options of this
-!! build: engine_precision ('fp64' or 'mixed'), mixed_tail, and
-!! engine_polynomials ('reference' fits or 'exact' expansion).
+!! file name.  &intracule_job also accepts four engine options of this
+!! build: engine_precision ('fp64' or 'mixed'), mixed_tail,
+!! engine_polynomials ('reference' fits or 'exact' expansion) and
+!! point_cutoff (log2 of the per-point cutoff, 0 = evaluate every term).
 module input_reader
    use inputdat
    use intrainfo
@@ -42,7 +43,7 @@ contains
                                nang, nquad_in, cent_in, nradc_in, nangc_in, sfalpha_in, Ps_in, &
                                nblock, scan_start, scan_end, scan_step, scan_nang, center_i, &
                                step_i, np_i, engine_precision, mixed_tail, &
-                               engine_polynomials
+                               engine_polynomials, point_cutoff
       namelist /cube_job/ cube, denscube, primcube, aocube, mocube, gradient, laplacian, npr, &
                           cao, mo, center, step, np
 
@@ -118,6 +119,11 @@ contains
          write(*,*) "Valid values: reference, exact."
          stop 1
       end select
+
+      if (.not. (point_cutoff == 0.0d0 .or. (point_cutoff >= 64.0d0 .and. point_cutoff <= 1000.0d0))) then
+         write(*,*) "ERROR: point_cutoff must be 0 (off) or between 64 and 1000."
+         stop 1
+      end if
 
       if (.not. readwfx .and. .not. readfchk) then
          write(*,*) "ERROR: No wavefunction file specified. Please provide a .wfx or .fchk file."

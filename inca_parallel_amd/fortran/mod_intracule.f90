@@ -15,6 +15,7 @@ module intracule
 
    integer(c_int64_t), private :: counts(4) = 0
    double precision, private :: kern_ms(3) = 0.d0
+   integer(c_int64_t), private :: evals_ref = 0, evals_exec = 0
 
 contains
 
@@ -93,11 +94,14 @@ contains
                                            "inca_set_mixed_tail")
       call inca_check(inca_set_polynomials(inca_ctx, merge(1_c_int32_t, 0_c_int32_t, &
                       exact_polynomials)), "inca_set_polynomials")
+      call inca_check(inca_set_point_cutoff(inca_ctx, point_cutoff), "inca_set_point_cutoff")
       write(*,*) "Starting GPU evaluation over primitive quartets..."
 
       t3 = wall()
       if (radial_integral .or. radial_plot .or. cubeintra) then
          call evaluate(ivec)
+         call inca_check(inca_get_eval_counts(inca_ctx, evals_ref, evals_exec), &
+                         "inca_get_eval_counts")
          ! reductions on the main grid before any other grid replaces it
          if (radial_integral) then
             allocate(ipc(nquad), vpc(nquad))
@@ -214,9 +218,15 @@ contains
          write(iout,*) "Engine precision: FP64"
       end if
       if (exact_polynomials) then
-         write(iout,*) "Axis polynomials: exact expansion in r"
+         write(iout,*) "Axis polynomials: exact expansion in rp"
       else
          write(iout,*) "Axis polynomials: Gauss-Hermite + Vandermonde fits (reference)"
+      end if
+      if (point_cutoff > 0.d0) then
+         write(iout,'(A, F6.1, A, I0, A, I0)') " Per-point cutoff: 2^-", point_cutoff, &
+              " of each term's peak; evaluations executed ", evals_exec, " of ", evals_ref
+      else
+         write(iout,*) "Per-point cutoff: off (every term at every point)"
       end if
       write(iout,*) "=========================================================="
       close(iout)

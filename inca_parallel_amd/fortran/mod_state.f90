@@ -67,6 +67,7 @@ module intrainfo
    logical :: mixed_precision = .false.
    double precision :: mixed_tail = 0.02d0
    logical :: exact_polynomials = .false.
+   double precision :: point_cutoff = 100.0d0   ! log2 of the per-point cutoff; 0 = off
 end module intrainfo
 
 module inputdat

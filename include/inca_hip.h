@@ -250,12 +250,16 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
  * 2^-log2_rel of its peak bound (|A| prod_axis sum_k |C_k| max_y |y|^k
  * e^-y^2, using |term| <= |A| e^-s prod_axis sum_k |C_k| s^(k/2) at
  * s = |rp|^2), and a wavefront skips the quartet when all of its 64 P grid
- * points lie beyond that radius.  The skipped contributions sum to at most
+ * points lie beyond that radius (a bit mask per 64 quartets and wave tile,
+ * built before the grid loop).  The skipped contributions sum to at most
  * 2^-log2_rel times the summed peak bounds.  0 turns the cutoff off (the
- * reference's loop, the default); otherwise 64 <= log2_rel <= 1000 (100 is
- * the tested setting).  Takes effect at the next inca_evaluate.  No
- * reference counterpart. */
+ * reference's loop); otherwise 64 <= log2_rel <= 1000; default 100.  Takes
+ * effect at the next inca_evaluate / inca_prepare.  No reference
+ * counterpart. */
 int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
+/* Evaluations of the last grid loop: reference = quartets x points (the
+ * reference's count), executed = those the cutoff kept (equal when off). */
+int inca_get_eval_counts(inca_ctx *ctx, int64_t *reference, int64_t *executed);
 
 /* Sum over the quartets prepared by the last inca_evaluate of each term's
  * integral over all space, prod_axis sum_{k even} C_k Gamma((k+1)/2) A /

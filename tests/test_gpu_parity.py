@@ -270,8 +270,9 @@ def test_exact_polynomials_ch4_matches_reference_mode(engine, ch4):
 
 
 def test_point_cutoff_matches_full_loop(engine, small_cluster):
-    """The per-point cutoff (2^-100 of each term's peak) against the
-    reference's full quartet x point loop, on the three grid kinds."""
+    """The per-point cutoff (2^-100 of each term's peak, the default) against
+    the reference's full quartet x point loop, on the three grid kinds; the
+    executed evaluations are counted."""
     s, ijkl, d = small_cluster
     p = s.prims()
     engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
@@ -289,8 +290,13 @@ def test_point_cutoff_matches_full_loop(engine, small_cluster):
             for prec in (capi.INCA_FP64, capi.INCA_MIXED):
                 engine.set_point_cutoff(0)
                 full, _ = engine.evaluate(prec, THRESH)
+                ref_n, ex_n = engine.eval_counts()
+                assert ex_n == ref_n
                 engine.set_point_cutoff(100)
                 cut, _ = engine.evaluate(prec, THRESH)
+                ref_c, ex_c = engine.eval_counts()
+                assert ref_c == ref_n and 0 < ex_c <= ref_c
+                print("executed %d of %d evaluations" % (ex_c, ref_c))
                 er, ea = _close(cut, full)
                 print("grid G=%d prec %d: cutoff vs full rel %.2e abs %.2e" % (
                     engine.G, prec, er, ea))
@@ -299,7 +305,7 @@ def test_point_cutoff_matches_full_loop(engine, small_cluster):
                 else:  # skipped terms shift the FP32 tile roundoff
                     assert er < 1e-10 and ea < 1e-12
     finally:
-        engine.set_point_cutoff(0)
+        engine.set_point_cutoff(100)
 
 
 def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
@@ -312,10 +318,7 @@ def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     engine.set_quartets(ijkl=ijkl, d=d)
     engine.grid_points(np.zeros((1, 3)))
     engine.set_point_cutoff(100)
-    try:
-        _, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
-    finally:
-        engine.set_point_cutoff(0)
+    _, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
     n = int(cnt[2])
     recs = engine.prepared(0, n)
     rng = np.random.default_rng(5)
