@@ -18,7 +18,9 @@
 // {1, 2, 4, 8}; the launch sequence (counter reset, persistent kernel,
 // reduction, per-molecule integrals) is captured once into a hipGraph and
 // replayed by every inca_batch_evaluate.  The counter is cleared by a kernel
-// node, not a memset node.
+// node, not a memset node.  When every context had the per-point cutoff on,
+// the batch builds each molecule's skip mask (eval_core.hpp) whenever the
+// items are rebuilt, outside the graph: grids and quartets are fixed.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -49,8 +51,19 @@ struct inca_batch {
   size_t partial_cap = 0;
   int64_t *d_items = nullptr;
   size_t items_cap = 0;
-  int64_t *d_mol = nullptr;   // per molecule: g_off, G, part_off, nch
+  int64_t *d_mol = nullptr;   // per molecule: kMol words (see build_items)
   int32_t *d_pmol = nullptr;  // molecule of every point
+  // per-point cutoff (when every context had it on): tile anchors come from
+  // each molecule's grid kind, centre map and centres, as in one context
+  bool cut = false;
+  std::vector<int> mode;
+  std::vector<int64_t> c_off, ntile, nword, moff;
+  int32_t *d_pmap = nullptr;  // molecule-local 1-based centre of every point
+  double *d_cent = nullptr;   // all molecules' centres (3 per centre)
+  double *d_tb = nullptr;
+  size_t tb_cap = 0;
+  unsigned long long *d_mask = nullptr;
+  size_t mask_cap = 0;
   unsigned int *d_counter = nullptr;
   int64_t nitems = 0;
   hipGraph_t graph = nullptr;
@@ -63,7 +76,8 @@ struct inca_batch {
 namespace inca {
 namespace {
 
-constexpr int kItem = 6;  // int64 words per work item: mol, tile, qoff, n, stride, partial base
+constexpr int kItem = 7;  // int64 words per item: mol, tile, qoff, n, stride, partial, local q
+constexpr int kMol = 6;   // int64 words per molecule: g_off, G, part_off, nch, mask off, tiles
 
 int bfail(inca_batch *b, int code, const std::string &msg) {
   b->err = msg;
@@ -88,6 +102,7 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
                                                    const double *__restrict__ gy,
                                                    const double *__restrict__ gz,
                                                    const int64_t *__restrict__ mol,
+                                                   const unsigned long long *__restrict__ mask,
                                                    double *__restrict__ partial,
                                                    unsigned int *__restrict__ counter) {
   __shared__ unsigned int s_it;
@@ -102,9 +117,10 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
     __syncthreads();  // s_it is read by every wave before the next fetch
     if (it >= nitems) break;  // uniform over the block: every wave leaves together
     const int64_t *__restrict__ d = items + it * kItem;
-    const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5];
+    const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5], lq = d[6];
     const int stride = (int)d[4];
-    const int64_t goff = mol[4 * m], G = mol[4 * m + 1];
+    const int64_t goff = mol[kMol * m], G = mol[kMol * m + 1];
+    const int64_t moff = mol[kMol * m + 4], nt = mol[kMol * m + 5];
     const int64_t base = bt * (kT * P) + (int64_t)wv * (64 * P) + lane;
     double x[P], y[P], z[P], acc[P];
 #pragma unroll
@@ -116,11 +132,12 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
       z[k] = ok ? gz[goff + g] : 0.0;
       acc[k] = 0.0;
     }
-    const double *__restrict__ rec = q + qo;
-    for (int64_t i = 0; i < n; ++i, rec += stride) {
+    const double *__restrict__ q0 = q + qo - lq * stride;  // the molecule's first quartet
+    for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
+      const double *__restrict__ rec = q0 + l * stride;
       const QHeader h = load_header(rec);
       accumulate_fp64<P>(rec, h, x, y, z, acc, s_exp2);
-    }
+    });
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       const int64_t g = base + (int64_t)k * 64;
@@ -137,7 +154,8 @@ __global__ __launch_bounds__(kT) void k_batch_reduce(const double *__restrict__ 
   const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (g >= Gtot) return;
   const int m = pmol[g];
-  const int64_t goff = mol[4 * m], G = mol[4 * m + 1], po = mol[4 * m + 2], nc = mol[4 * m + 3];
+  const int64_t goff = mol[kMol * m], G = mol[kMol * m + 1], po = mol[kMol * m + 2],
+                nc = mol[kMol * m + 3];
   const int64_t local = g - goff;
   double s = 0.0;
   for (int64_t c = 0; c < nc; ++c) s += partial[po + c * G + local];
@@ -153,7 +171,7 @@ __global__ __launch_bounds__(kT) void k_batch_integrate(const double *__restrict
                                                         double *__restrict__ res) {
   __shared__ double r1[kT], r2[kT];
   const int64_t m = blockIdx.x;
-  const int64_t goff = mol[4 * m], G = mol[4 * m + 1];
+  const int64_t goff = mol[kMol * m], G = mol[kMol * m + 1];
   double s1 = 0.0, s2 = 0.0;
   for (int64_t g = threadIdx.x; g < G; g += kT) {
     s1 += w[goff + g] * ivec[goff + g];
@@ -234,7 +252,7 @@ int build_items(inca_batch *b, int P) {
         for (int64_t t = 0; t < tiles; ++t) {
           const int64_t pts = std::min(tile_pts, b->G[m] - t * tile_pts);
           It it;
-          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m]};
+          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m], sg[0] + s};
           it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts;
           items.push_back(it);
         }
@@ -253,12 +271,45 @@ int build_items(inca_batch *b, int P) {
   if ((rc = bensure(b, b->d_items, b->items_cap, flat.size()))) return rc;
   if ((rc = bensure(b, b->d_partial, b->partial_cap, (size_t)std::max<int64_t>(1, part)))) return rc;
   B_HIP(hipMemcpy(b->d_items, flat.data(), flat.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  std::vector<int64_t> mi(4 * (size_t)b->nmol);
+  // cutoff masks for this P: wave tiles of 64 P points per molecule
+  int64_t tb_total = 0, mask_total = 0;
   for (int m = 0; m < b->nmol; ++m) {
-    mi[4 * m] = b->g_off[m];
-    mi[4 * m + 1] = b->G[m];
-    mi[4 * m + 2] = b->part_off[m];
-    mi[4 * m + 3] = b->nch[m];
+    b->ntile[m] = ((b->G[m] + tile_pts - 1) / tile_pts) * (kT / 64);
+    b->nword[m] = (b->nq[m] + 63) / 64;
+    b->moff[m] = mask_total;
+    tb_total += b->ntile[m];
+    mask_total += b->ntile[m] * b->nword[m];
+  }
+  if (b->cut) {
+    if ((rc = bensure(b, b->d_tb, b->tb_cap, (size_t)std::max<int64_t>(1, tb_total) * kTB)) ||
+        (rc = bensure(b, b->d_mask, b->mask_cap, (size_t)std::max<int64_t>(1, mask_total))))
+      return rc;
+    int64_t toff = 0;
+    for (int m = 0; m < b->nmol; ++m) {
+      const int64_t nt = b->ntile[m];
+      if (nt > 0 && b->G[m] > 0) {
+        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(nt / (kT / 64))), dim3(kT), 0, b->stream,
+                           b->d_gx + b->g_off[m], b->d_gy + b->g_off[m], b->d_gz + b->g_off[m],
+                           b->G[m], P, b->mode[m], b->d_pmap + b->g_off[m],
+                           b->d_cent + 3 * b->c_off[m], nt, b->d_tb + toff * kTB);
+        if (b->nword[m] > 0)
+          hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((b->nword[m] + 3) / 4)), dim3(kT), 0,
+                             b->stream, b->d_q + b->q_off[m], b->stride[m], b->nq[m],
+                             b->d_tb + toff * kTB, nt, b->nword[m], b->d_mask + b->moff[m]);
+      }
+      toff += nt;
+    }
+    B_HIP(hipGetLastError());
+    B_HIP(hipStreamSynchronize(b->stream));
+  }
+  std::vector<int64_t> mi(kMol * (size_t)b->nmol);
+  for (int m = 0; m < b->nmol; ++m) {
+    mi[kMol * m] = b->g_off[m];
+    mi[kMol * m + 1] = b->G[m];
+    mi[kMol * m + 2] = b->part_off[m];
+    mi[kMol * m + 3] = b->nch[m];
+    mi[kMol * m + 4] = b->moff[m];
+    mi[kMol * m + 5] = b->ntile[m];
   }
   B_HIP(hipMemcpy(b->d_mol, mi.data(), mi.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   return 0;
@@ -271,8 +322,8 @@ int enqueue(inca_batch *b, int P) {
   if (b->nitems > 0) {
 #define INCA_BEVAL(PP)                                                                        \
   hipLaunchKernelGGL(k_batch_eval<PP>, dim3(b->nblocks), dim3(kT), 0, st, b->d_items,         \
-                     b->nitems, b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_mol, b->d_partial,     \
-                     b->d_counter)
+                     b->nitems, b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_mol,                   \
+                     b->cut ? b->d_mask : nullptr, b->d_partial, b->d_counter)
     switch (P) {
       case 1: INCA_BEVAL(1); break;
       case 2: INCA_BEVAL(2); break;
@@ -334,6 +385,13 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   b->nch.assign(nmol, 0);
   b->stride.assign(nmol, 0);
   b->segs.assign(nmol, {});
+  b->mode.assign(nmol, 2);
+  b->c_off.assign(nmol, 0);
+  b->ntile.assign(nmol, 0);
+  b->nword.assign(nmol, 0);
+  b->moff.assign(nmol, 0);
+  b->cut = true;
+  int64_t ncent = 0;
   for (int m = 0; m < nmol; ++m) {
     const inca_ctx *c = mols[m];
     if (!c) return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": null context");
@@ -350,6 +408,10 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     b->stride[m] = c->qstride;
     b->q_off[m] = b->Qdoubles;
     b->Qdoubles += c->q2 * c->qstride;
+    b->cut = b->cut && c->cut_log2 > 0.0;
+    b->mode[m] = c->grid_kind == GRID_RADIAL ? 0 : (c->grid_kind == GRID_MULTI ? 1 : 2);
+    b->c_off[m] = ncent;
+    if (b->mode[m] == 1) ncent += c->nquad;
   }
   const size_t gt = (size_t)std::max<int64_t>(1, b->Gtot);
   const size_t qd = (size_t)std::max<int64_t>(1, b->Qdoubles);
@@ -360,9 +422,11 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     if ((cap = 0, rc = bensure(b, *a, cap, gt))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_q, cap, qd))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_res, cap, 2 * (size_t)nmol))) return rc;
-  if ((cap = 0, rc = bensure(b, b->d_mol, cap, 4 * (size_t)nmol))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_mol, cap, kMol * (size_t)nmol))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_pmol, cap, gt))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_counter, cap, 1))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_pmap, cap, gt))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_cent, cap, 3 * (size_t)std::max<int64_t>(1, ncent)))) return rc;
   hipStream_t st = b->stream;
   int64_t *d_perm = nullptr;
   size_t perm_cap = 0;
@@ -378,6 +442,12 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
       hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(kT), 0, st, b->d_pmol + b->g_off[m], G,
                          (int32_t)m);
       B_HIP(hipGetLastError());
+      if (b->mode[m] == 1) {
+        B_HIP(hipMemcpyAsync(b->d_pmap + b->g_off[m], c->d_map, G * sizeof(int32_t),
+                             hipMemcpyDeviceToDevice, st));
+        B_HIP(hipMemcpyAsync(b->d_cent + 3 * b->c_off[m], c->d_gcent, 3 * c->nquad * sizeof(double),
+                             hipMemcpyDeviceToDevice, st));
+      }
     }
     if (n == 0) continue;
     // class of every prepared record (header word 4), sorted stably on the host
@@ -443,8 +513,10 @@ int inca_batch_destroy(inca_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void *bufs[] = {b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_w, b->d_wvee, b->d_ivec, b->d_res,
-                  b->d_partial, b->d_items, b->d_mol, b->d_pmol, b->d_counter};
+  void *bufs[] = {b->d_q,     b->d_gx,    b->d_gy,   b->d_gz,    b->d_w,
+                  b->d_wvee,  b->d_ivec,  b->d_res,  b->d_partial, b->d_items,
+                  b->d_mol,   b->d_pmol,  b->d_counter, b->d_pmap, b->d_cent,
+                  b->d_tb,    b->d_mask};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : b->ev)

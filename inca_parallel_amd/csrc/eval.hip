@@ -16,9 +16,8 @@
 // reduction by ln2/256 to |r| <= ln2/512, a degree-4 Taylor polynomial of
 // e^r (truncation error 3.8e-17), a 256-entry table of 2^(j/256) in LDS and
 // an exponent shift (eval_core.hpp).  Mixed path: rp in FP64 (the cancellation-prone step),
-// everything after it in FP32 -- hardware exp2, FP32 Horner -- with a
-// compensated (Kahan) FP32 sum over tiles of quartets that is folded into
-// an FP64 accumulator at the end of every tile.
+// everything after it in FP32 -- hardware exp2, FP32 Horner, FP32 products
+// -- and every term added to an FP64 accumulator.
 #include <cmath>
 
 #include "common.hpp"
@@ -26,177 +25,6 @@
 
 namespace inca {
 namespace {
-
-// Tile bounds.  Wave w of block b owns the 64 P consecutive grid points
-// from (4 b + w) 64 P on.  For each such tile: an anchor c and the range
-// [rho_min, rho_max] of |r - c| over its points (words 0..4), and whether
-// it holds any point (word 5).  Anchors: the origin for radial scans (the
-// points lie on spheres about it), the owning centre for multicentre grids
-// when the whole tile belongs to one centre, else the tile's centroid.
-constexpr int kTB = 8;  // doubles per tile record
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-__global__ __launch_bounds__(kT) void k_tile_bounds(const double *__restrict__ gx,
-                                                    const double *__restrict__ gy,
-                                                    const double *__restrict__ gz, int64_t G,
-                                                    int P, int mode,
-                                                    const int32_t *__restrict__ map,
-                                                    const double *__restrict__ gcent,
-                                                    int64_t ntile, double *__restrict__ tb) {
-  const int lane = threadIdx.x & 63;
-  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (t >= ntile) return;  // uniform per wave
-  const int64_t g0 = t * 64 * P;
-  const int first = (mode == 1 && g0 < G) ? map[g0] : 0;
-  double sx = 0.0, sy = 0.0, sz = 0.0, n = 0.0;
-  int same = 1;
-  for (int k = 0; k < P; ++k) {
-    const int64_t g = g0 + (int64_t)k * 64 + lane;
-    if (g < G) {
-      sx += gx[g];
-      sy += gy[g];
-      sz += gz[g];
-      n += 1.0;
-      if (mode == 1 && map[g] != first) same = 0;
-    }
-  }
-  sx = wave_sum(sx);
-  sy = wave_sum(sy);
-  sz = wave_sum(sz);
-  n = wave_sum(n);
-  same = __all(same);
-  double cx = 0.0, cy = 0.0, cz = 0.0;
-  if (mode == 1 && same && first > 0) {
-    cx = gcent[3 * (first - 1)];
-    cy = gcent[3 * (first - 1) + 1];
-    cz = gcent[3 * (first - 1) + 2];
-  } else if (mode != 0 && n > 0.0) {
-    cx = sx / n;
-    cy = sy / n;
-    cz = sz / n;
-  }
-  double lo = HUGE_VAL, hi = 0.0;
-  for (int k = 0; k < P; ++k) {
-    const int64_t g = g0 + (int64_t)k * 64 + lane;
-    if (g < G) {
-      const double dx = gx[g] - cx, dy = gy[g] - cy, dz = gz[g] - cz;
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      lo = fmin(lo, d);
-      hi = fmax(hi, d);
-    }
-  }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    lo = fmin(lo, __shfl_xor(lo, d, 64));
-    hi = fmax(hi, __shfl_xor(hi, d, 64));
-  }
-  if (lane == 0) {
-    double *o = tb + t * kTB;
-    o[0] = cx;
-    o[1] = cy;
-    o[2] = cz;
-    // widened by far more than the roundoff of the distances
-    o[3] = n > 0.0 ? lo * (1.0 - 1e-12) : 0.0;
-    o[4] = n > 0.0 ? hi * (1.0 + 1e-12) + 1e-300 : 0.0;
-    o[5] = n;
-    o[6] = 0.0;
-    o[7] = 0.0;
-  }
-}
-
-__global__ __launch_bounds__(kT) void k_fill_tiles(int64_t ntile, double *__restrict__ tb) {
-  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (t >= ntile) return;
-  for (int w = 0; w < kTB; ++w) tb[t * kTB + w] = (w == 5) ? 1.0 : 0.0;
-}
-
-// True when every point of the tile lies beyond the quartet's cutoff radius:
-// |rp| = |sqe (r - c) + u|, u = sqe c + sqe d, is at least
-// max(|u| - sqe rho_max, sqe rho_min - |u|) on the tile.
-__device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy, double cz,
-                                         double rlo, double rhi) {
-  const double ux = fma(h.sqe, cx, h.ax), uy = fma(h.sqe, cy, h.ay), uz = fma(h.sqe, cz, h.az);
-  const double u2 = fma(ux, ux, fma(uy, uy, uz * uz));
-  const double o = fma(h.sqe, rhi, h.rcut);
-  const double i = fma(h.sqe, rlo, -h.rcut);
-  return (u2 >= o * o) || (i > 0.0 && u2 <= i * i);
-}
-
-// Skip mask: bit i of mask[w][t] is set when quartet 64 w + i is beyond its
-// cutoff radius at every point of wave tile t.  One wave per 64-quartet word:
-// lane i holds quartet 64 w + i, a ballot per tile gives the word, and the
-// words of 64 consecutive tiles are stored by the 64 lanes at once.
-__global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, int qstride,
-                                                  int64_t nq, const double *__restrict__ tb,
-                                                  int64_t ntile, int64_t nwords,
-                                                  unsigned long long *__restrict__ mask) {
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (w >= nwords) return;  // uniform per wave
-  const int64_t qi = w * 64 + lane;
-  const bool valid = qi < nq;
-  QHeader h{};
-  if (valid) h = load_header(q + qi * qstride);
-  for (int64_t t0 = 0; t0 < ntile; t0 += 64) {
-    unsigned long long keep = 0;
-    const int nt = (int)((ntile - t0) < 64 ? (ntile - t0) : 64);
-    for (int j = 0; j < nt; ++j) {
-      const double *__restrict__ tbw = tb + (t0 + j) * kTB;
-      const bool far = valid && tbw[5] > 0.0 && tile_far(h, tbw[0], tbw[1], tbw[2], tbw[3], tbw[4]);
-      const unsigned long long m = __ballot(far);
-      if (lane == j) keep = m;
-    }
-    if (lane < nt) mask[w * ntile + t0 + lane] = keep;
-  }
-}
-
-// Executed evaluations under the skip mask: for every word and tile, the
-// kept quartets times the tile's points (integer atomics: exact, order-free).
-__global__ __launch_bounds__(kT) void k_mask_evals(const unsigned long long *__restrict__ mask,
-                                                   const double *__restrict__ tb, int64_t ntile,
-                                                   int64_t nwords, int64_t nq,
-                                                   unsigned long long *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  unsigned long long v = 0;
-  if (i < nwords * ntile) {
-    const int64_t w = i / ntile, t = i - w * ntile;
-    unsigned long long live = ~mask[i];
-    const int64_t left = nq - w * 64;
-    if (left < 64) live &= (1ull << left) - 1ull;
-    v = (unsigned long long)__popcll(live) * (unsigned long long)tb[t * kTB + 5];
-  }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
-}
-
-// Calls f(qi) for the quartets of [a, b) that the skip mask keeps for
-// `tile` (all of them when mask is null).  The mask words are wave-uniform,
-// so the iteration runs on scalar bit scans.
-template <class F>
-__device__ __forceinline__ void for_live(const unsigned long long *__restrict__ mask, int64_t ntile,
-                                         int64_t tile, int64_t a, int64_t b, F &&f) {
-  if (!mask) {
-    for (int64_t qi = a; qi < b; ++qi) f(qi);
-    return;
-  }
-  for (int64_t wd = a >> 6; (wd << 6) < b; ++wd) {
-    const int64_t base = wd << 6;
-    unsigned long long live = ~mask[wd * ntile + tile];
-    if (base < a) live &= ~0ull << (a - base);
-    if (b - base < 64) live &= (1ull << (b - base)) - 1ull;
-    while (live) {
-      const int i = __builtin_ctzll(live);
-      live &= live - 1ull;
-      f(base + i);
-    }
-  }
-}
 
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
@@ -237,9 +65,74 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   }
 }
 
-// Mixed precision: FP64 rp, FP32 exp/polynomials, Kahan FP32 per tile of
-// kMixTile quartets folded into FP64.
-constexpr int kMixTile = 128;
+// Mixed precision: rp in FP64 (sqe r + sqe d cancels), everything after it
+// in FP32 -- the hardware exp2 and FP32 Horner -- and each term added to an
+// FP64 accumulator.  The body is specialised on the axes of nonzero degree
+// like the FP64 kernel's.
+template <int P, bool MX, bool MY, bool MZ>
+__device__ __forceinline__ void accumulate_mixed(const double *__restrict__ rec, const QHeader &h,
+                                                 const double (&x)[P], const double (&y)[P],
+                                                 const double (&z)[P], double (&acc)[P]) {
+  // -log2(e) split so that s*log2(e) is formed to FP32 rounding without the
+  // 1.3e-8 relative bias of a single-float constant
+  const float kNL2eHi = -1.44269502162933349609375f;
+  const float kNL2eLo = -1.925963033500011e-08f;
+  const double *__restrict__ cx = rec + kQHead;
+  const double *__restrict__ cy = cx + h.Lx + 1;
+  const double *__restrict__ cz = cy + h.Ly + 1;
+  float px[P], py[P], pz[P], vx[P], vy[P], vz[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    px[k] = (float)fma(h.sqe, x[k], h.ax);
+    py[k] = (float)fma(h.sqe, y[k], h.ay);
+    pz[k] = (float)fma(h.sqe, z[k], h.az);
+  }
+  auto horner32 = [&](const double *__restrict__ c, int L, const float(&p)[P], float(&v)[P]) {
+    const float c0 = (float)c[0], c1 = (float)c[1];
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = fmaf(p[k], c0, c1);
+    for (int j = 2; j <= L; ++j) {
+      const float cj = (float)c[j];
+#pragma unroll
+      for (int k = 0; k < P; ++k) v[k] = fmaf(v[k], p[k], cj);
+    }
+  };
+  if (MX) horner32(cx, h.Lx, px, vx);
+  if (MY) horner32(cy, h.Ly, py, vy);
+  if (MZ) horner32(cz, h.Lz, pz, vz);
+  float S = 1.0f;
+  if (!MX) S = (float)cx[0];
+  if (!MY) S = MX ? (float)cy[0] : S * (float)cy[0];
+  if (!MZ) S = (MX && MY) ? (float)cz[0] : S * (float)cz[0];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const float s = fmaf(px[k], px[k], fmaf(py[k], py[k], pz[k] * pz[k]));
+    float f = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo));
+    if (!(MX && MY && MZ)) f *= S;
+    if (MX) f *= vx[k];
+    if (MY) f *= vy[k];
+    if (MZ) f *= vz[k];
+    acc[k] += (double)f;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void accumulate_mixed_any(const double *__restrict__ rec,
+                                                     const QHeader &h, const double (&x)[P],
+                                                     const double (&y)[P], const double (&z)[P],
+                                                     double (&acc)[P]) {
+  const int mask = (h.Lx > 0) | ((h.Ly > 0) << 1) | ((h.Lz > 0) << 2);
+  switch (mask) {
+    case 0: accumulate_mixed<P, false, false, false>(rec, h, x, y, z, acc); break;
+    case 1: accumulate_mixed<P, true, false, false>(rec, h, x, y, z, acc); break;
+    case 2: accumulate_mixed<P, false, true, false>(rec, h, x, y, z, acc); break;
+    case 3: accumulate_mixed<P, true, true, false>(rec, h, x, y, z, acc); break;
+    case 4: accumulate_mixed<P, false, false, true>(rec, h, x, y, z, acc); break;
+    case 5: accumulate_mixed<P, true, false, true>(rec, h, x, y, z, acc); break;
+    case 6: accumulate_mixed<P, false, true, true>(rec, h, x, y, z, acc); break;
+    default: accumulate_mixed<P, true, true, true>(rec, h, x, y, z, acc); break;
+  }
+}
 
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q, int qstride,
@@ -266,64 +159,11 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
   }
   const int64_t q0 = (int64_t)blockIdx.y * qchunk;
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
-  // -log2(e) split so that s*log2(e) is formed to FP32 rounding without the
-  // 1.3e-8 relative bias of a single-float constant
-  const float kNL2eHi = -1.44269502162933349609375f;
-  const float kNL2eLo = -1.925963033500011e-08f;
-  for (int64_t t0 = q0; t0 < q1; t0 += kMixTile) {
-    const int64_t t1 = t0 + kMixTile < q1 ? t0 + kMixTile : q1;
-    float sum[P], comp[P];
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-      sum[k] = 0.0f;
-      comp[k] = 0.0f;
-    }
-    for_live(CUT ? mask : nullptr, ntile, tile, t0, t1, [&](int64_t qi) {
-      const double *__restrict__ rec = q + qi * qstride;
-      const QHeader h = load_header(rec);
-      const double *__restrict__ cx = rec + kQHead;
-      const double *__restrict__ cy = cx + h.Lx + 1;
-      const double *__restrict__ cz = cy + h.Ly + 1;
-      float px[P], py[P], pz[P], vx[P], vy[P], vz[P];
-      const float c0x = (float)cx[0], c0y = (float)cy[0], c0z = (float)cz[0];
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        px[k] = (float)fma(h.sqe, x[k], h.ax);
-        py[k] = (float)fma(h.sqe, y[k], h.ay);
-        pz[k] = (float)fma(h.sqe, z[k], h.az);
-        vx[k] = c0x;
-        vy[k] = c0y;
-        vz[k] = c0z;
-      }
-      for (int p = 1; p <= h.Lx; ++p) {
-        const float c = (float)cx[p];
-#pragma unroll
-        for (int k = 0; k < P; ++k) vx[k] = fmaf(vx[k], px[k], c);
-      }
-      for (int p = 1; p <= h.Ly; ++p) {
-        const float c = (float)cy[p];
-#pragma unroll
-        for (int k = 0; k < P; ++k) vy[k] = fmaf(vy[k], py[k], c);
-      }
-      for (int p = 1; p <= h.Lz; ++p) {
-        const float c = (float)cz[p];
-#pragma unroll
-        for (int k = 0; k < P; ++k) vz[k] = fmaf(vz[k], pz[k], c);
-      }
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const float s = fmaf(px[k], px[k], fmaf(py[k], py[k], pz[k] * pz[k]));
-        const float e = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo));
-        const float term = e * vx[k] * vy[k] * vz[k];
-        const float yk = term - comp[k];
-        const float tk = sum[k] + yk;
-        comp[k] = (tk - sum[k]) - yk;
-        sum[k] = tk;
-      }
-    });
-#pragma unroll
-    for (int k = 0; k < P; ++k) acc[k] += (double)sum[k] - (double)comp[k];
-  }
+  for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
+    const double *__restrict__ rec = q + qi * qstride;
+    const QHeader h = load_header(rec);
+    accumulate_mixed_any<P>(rec, h, x, y, z, acc);
+  });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const int64_t g = base + (int64_t)k * 64;
@@ -355,6 +195,17 @@ __global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q,
   __syncthreads();
   if (threadIdx.x < 64 && threadIdx.x < nbins && h[threadIdx.x])
     atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// dst record r = src record floor(r nq / ns), whole records of `stride` doubles
+__global__ __launch_bounds__(kT) void k_gather_strided(const double *__restrict__ src, int64_t nq,
+                                                       int64_t ns, int stride,
+                                                       double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ns * stride) return;
+  const int64_t r = t / stride;
+  const int w = (int)(t - r * stride);
+  dst[t] = src[((r * nq) / ns) * stride + w];
 }
 
 __global__ void k_zero_u64(unsigned long long *p) {
@@ -525,7 +376,15 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   const int64_t G = ctx->G, nq = ctx->q2;
   if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
   const int cand[4] = {1, 2, 4, 8};
-  const int64_t n = nq < 262144 ? nq : 262144;
+  // an evenly strided sample of the prepared quartets (file order groups
+  // quartets by primitive, and with the cutoff their cost depends on where
+  // they sit), gathered into the pre-pass scratch, which is free after it
+  const int64_t n = nq < (1 << 20) ? nq : (1 << 20);
+  if ((size_t)n * ctx->qstride > ctx->qtmp_cap)
+    return set_err(ctx, INCA_EINVAL, "autotune needs prepared quartets");
+  hipLaunchKernelGGL(k_gather_strided, dim3((unsigned)((n * ctx->qstride + kT - 1) / kT)), dim3(kT),
+                     0, ctx->stream, ctx->d_q, nq, n, ctx->qstride, ctx->d_qtmp);
+  INCA_HIP(hipGetLastError());
   hipEvent_t a, b;
   INCA_HIP(hipEventCreate(&a));
   INCA_HIP(hipEventCreate(&b));
@@ -542,7 +401,7 @@ int autotune(inca_ctx *ctx, int32_t *best) {
       float ms = 0.f;
       for (int rep = 0; rep < 2 && !rc; ++rep) {  // first launch warms up
         INCA_HIP(hipEventRecord(a, ctx->stream));
-        rc = launch_eval(ctx, kind == 1, P, ctx->d_q, n, qch, nch, ctx->d_partial);
+        rc = launch_eval(ctx, kind == 1, P, ctx->d_qtmp, n, qch, nch, ctx->d_partial);
         INCA_HIP(hipEventRecord(b, ctx->stream));
         INCA_HIP(hipEventSynchronize(b));
         INCA_HIP(hipEventElapsedTime(&ms, a, b));

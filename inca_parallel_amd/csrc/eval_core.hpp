@@ -219,4 +219,181 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ rec, 
   }
 }
 
+// ---- per-point cutoff: tile bounds and skip masks (shared by the
+// single-context and batched evaluations) ----
+namespace {
+
+// Tile bounds.  Wave w of block b owns the 64 P consecutive grid points
+// from (4 b + w) 64 P on.  For each such tile: an anchor c and the range
+// [rho_min, rho_max] of |r - c| over its points (words 0..4), and whether
+// it holds any point (word 5).  Anchors: the origin for radial scans (the
+// points lie on spheres about it), the owning centre for multicentre grids
+// when the whole tile belongs to one centre, else the tile's centroid.
+constexpr int kTB = 8;  // doubles per tile record
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kT) void k_tile_bounds(const double *__restrict__ gx,
+                                                    const double *__restrict__ gy,
+                                                    const double *__restrict__ gz, int64_t G,
+                                                    int P, int mode,
+                                                    const int32_t *__restrict__ map,
+                                                    const double *__restrict__ gcent,
+                                                    int64_t ntile, double *__restrict__ tb) {
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (t >= ntile) return;  // uniform per wave
+  const int64_t g0 = t * 64 * P;
+  const int first = (mode == 1 && g0 < G) ? map[g0] : 0;
+  double sx = 0.0, sy = 0.0, sz = 0.0, n = 0.0;
+  int same = 1;
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = g0 + (int64_t)k * 64 + lane;
+    if (g < G) {
+      sx += gx[g];
+      sy += gy[g];
+      sz += gz[g];
+      n += 1.0;
+      if (mode == 1 && map[g] != first) same = 0;
+    }
+  }
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  n = wave_sum(n);
+  same = __all(same);
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if (mode == 1 && same && first > 0) {
+    cx = gcent[3 * (first - 1)];
+    cy = gcent[3 * (first - 1) + 1];
+    cz = gcent[3 * (first - 1) + 2];
+  } else if (mode != 0 && n > 0.0) {
+    cx = sx / n;
+    cy = sy / n;
+    cz = sz / n;
+  }
+  double lo = HUGE_VAL, hi = 0.0;
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = g0 + (int64_t)k * 64 + lane;
+    if (g < G) {
+      const double dx = gx[g] - cx, dy = gy[g] - cy, dz = gz[g] - cz;
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      lo = fmin(lo, d);
+      hi = fmax(hi, d);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, d, 64));
+    hi = fmax(hi, __shfl_xor(hi, d, 64));
+  }
+  if (lane == 0) {
+    double *o = tb + t * kTB;
+    o[0] = cx;
+    o[1] = cy;
+    o[2] = cz;
+    // widened by far more than the roundoff of the distances
+    o[3] = n > 0.0 ? lo * (1.0 - 1e-12) : 0.0;
+    o[4] = n > 0.0 ? hi * (1.0 + 1e-12) + 1e-300 : 0.0;
+    o[5] = n;
+    o[6] = 0.0;
+    o[7] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_fill_tiles(int64_t ntile, double *__restrict__ tb) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= ntile) return;
+  for (int w = 0; w < kTB; ++w) tb[t * kTB + w] = (w == 5) ? 1.0 : 0.0;
+}
+
+// True when every point of the tile lies beyond the quartet's cutoff radius:
+// |rp| = |sqe (r - c) + u|, u = sqe c + sqe d, is at least
+// max(|u| - sqe rho_max, sqe rho_min - |u|) on the tile.
+__device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy, double cz,
+                                         double rlo, double rhi) {
+  const double ux = fma(h.sqe, cx, h.ax), uy = fma(h.sqe, cy, h.ay), uz = fma(h.sqe, cz, h.az);
+  const double u2 = fma(ux, ux, fma(uy, uy, uz * uz));
+  const double o = fma(h.sqe, rhi, h.rcut);
+  const double i = fma(h.sqe, rlo, -h.rcut);
+  return (u2 >= o * o) || (i > 0.0 && u2 <= i * i);
+}
+
+// Skip mask: bit i of mask[w][t] is set when quartet 64 w + i is beyond its
+// cutoff radius at every point of wave tile t.  One wave per 64-quartet word:
+// lane i holds quartet 64 w + i, a ballot per tile gives the word, and the
+// words of 64 consecutive tiles are stored by the 64 lanes at once.
+__global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, int qstride,
+                                                  int64_t nq, const double *__restrict__ tb,
+                                                  int64_t ntile, int64_t nwords,
+                                                  unsigned long long *__restrict__ mask) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (w >= nwords) return;  // uniform per wave
+  const int64_t qi = w * 64 + lane;
+  const bool valid = qi < nq;
+  QHeader h{};
+  if (valid) h = load_header(q + qi * qstride);
+  for (int64_t t0 = 0; t0 < ntile; t0 += 64) {
+    unsigned long long keep = 0;
+    const int nt = (int)((ntile - t0) < 64 ? (ntile - t0) : 64);
+    for (int j = 0; j < nt; ++j) {
+      const double *__restrict__ tbw = tb + (t0 + j) * kTB;
+      const bool far = valid && tbw[5] > 0.0 && tile_far(h, tbw[0], tbw[1], tbw[2], tbw[3], tbw[4]);
+      const unsigned long long m = __ballot(far);
+      if (lane == j) keep = m;
+    }
+    if (lane < nt) mask[w * ntile + t0 + lane] = keep;
+  }
+}
+
+// Executed evaluations under the skip mask: for every word and tile, the
+// kept quartets times the tile's points (integer atomics: exact, order-free).
+__global__ __launch_bounds__(kT) void k_mask_evals(const unsigned long long *__restrict__ mask,
+                                                   const double *__restrict__ tb, int64_t ntile,
+                                                   int64_t nwords, int64_t nq,
+                                                   unsigned long long *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < nwords * ntile) {
+    const int64_t w = i / ntile, t = i - w * ntile;
+    unsigned long long live = ~mask[i];
+    const int64_t left = nq - w * 64;
+    if (left < 64) live &= (1ull << left) - 1ull;
+    v = (unsigned long long)__popcll(live) * (unsigned long long)tb[t * kTB + 5];
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+}
+
+// Calls f(qi) for the quartets of [a, b) that the skip mask keeps for
+// `tile` (all of them when mask is null).  The mask words are wave-uniform,
+// so the iteration runs on scalar bit scans.
+template <class F>
+__device__ __forceinline__ void for_live(const unsigned long long *__restrict__ mask, int64_t ntile,
+                                         int64_t tile, int64_t a, int64_t b, F &&f) {
+  if (!mask) {
+    for (int64_t qi = a; qi < b; ++qi) f(qi);
+    return;
+  }
+  for (int64_t wd = a >> 6; (wd << 6) < b; ++wd) {
+    const int64_t base = wd << 6;
+    unsigned long long live = ~mask[wd * ntile + tile];
+    if (base < a) live &= ~0ull << (a - base);
+    if (b - base < 64) live &= (1ull << (b - base)) - 1ull;
+    while (live) {
+      const int i = __builtin_ctzll(live);
+      live &= live - 1ull;
+      f(base + i);
+    }
+  }
+}
+
+}  // namespace
+
 }  // namespace inca

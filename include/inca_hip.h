@@ -281,11 +281,12 @@ int inca_get_analytic_integral(inca_ctx *ctx, double *total);
 int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
 
 /* Evaluation tiles: grid points per lane (1, 2, 4 or 8) of the FP64 and of
- * the FP32 kernel.  inca_autotune times each choice on a slice of the
- * quartets prepared by the last inca_evaluate, on the current grid, keeps
- * the fastest and returns it in best[0] (FP64) and best[1] (FP32).  It
- * overwrites the partial sums, so call inca_evaluate again before reading
- * Ivec.  No reference counterpart. */
+ * the FP32 kernel.  inca_autotune times each choice on an evenly strided
+ * sample (up to 2^20) of the quartets prepared by the last inca_evaluate or
+ * inca_prepare, on the current grid and with the current cutoff, keeps the
+ * fastest and returns it in best[0] (FP64) and best[1] (FP32).  It
+ * overwrites the partial sums, so evaluate again before reading Ivec.  No
+ * reference counterpart. */
 int inca_set_tiles(inca_ctx *ctx, int32_t points_fp64, int32_t points_fp32);
 int inca_autotune(inca_ctx *ctx, int32_t *best);
 
