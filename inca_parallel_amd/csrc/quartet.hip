@@ -269,8 +269,10 @@ __device__ double cutoff_radius(const double *c, const int Ls[3], double s0) {
     off += L + 1;
   }
   const int Lsum = Ls[0] + Ls[1] + Ls[2];
+  // three fixed-point steps from s0; the correction after them makes the
+  // bound hold whatever the iterate (F is decreasing beyond s0 > Lsum/2)
   double s = s0, F = 0.0;
-  for (int it = 0; it < 6; ++it) {
+  for (int it = 0; it < 3; ++it) {
     const double t = sqrt(s);
     double ratio = 1.0;
     off = 0;
@@ -282,7 +284,7 @@ __device__ double cutoff_radius(const double *c, const int Ls[3], double s0) {
       off += L + 1;
     }
     F = log(ratio) - s + s0;
-    if (it < 5) s = s + F;
+    if (it < 2) s = s + F;
   }
   if (F > 0.0) s += F / (1.0 - Lsum / (2.0 * s));
   const double r = sqrt(s) * (1.0 + 1e-12);
