@@ -295,7 +295,8 @@ int build_items(inca_batch *b, int P) {
         if (b->nword[m] > 0)
           hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((b->nword[m] + 3) / 4)), dim3(kT), 0,
                              b->stream, b->d_q + b->q_off[m], b->stride[m], b->nq[m],
-                             b->d_tb + toff * kTB, nt, b->nword[m], b->d_mask + b->moff[m]);
+                             b->d_tb + toff * kTB, nt, b->nword[m], b->d_mask + b->moff[m],
+                             nullptr);
       }
       toff += nt;
     }

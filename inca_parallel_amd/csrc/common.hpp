@@ -104,6 +104,8 @@ struct inca_ctx {
   size_t mask_cap = 0;
   unsigned long long *d_evals = nullptr;  // executed evaluations of the last grid loop
   size_t evals_cap = 0;
+  unsigned long long *d_wcount = nullptr; // per 64-quartet word: evaluations kept
+  size_t wcount_cap = 0;
   int64_t evals_reference = 0, evals_executed = 0;
   double *d_gcent = nullptr;  // multicentre grid: centre coordinates (3*nquad)
   double *d_partial = nullptr;

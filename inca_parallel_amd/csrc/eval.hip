@@ -34,8 +34,11 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
                                                   const double *__restrict__ gz, int64_t G,
                                                   const double *__restrict__ tb,
                                                   const unsigned long long *__restrict__ mask,
-                                                  int64_t ntile, double *__restrict__ partial) {
-  const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
+                                                  int64_t ntile, int64_t tiles, int64_t nch,
+                                                  int xcd, double *__restrict__ partial) {
+  const BlockTask task = block_task(tiles, xcd);
+  if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
+  const int64_t tile = task.tile_block * (kT / 64) +
                        __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
   const bool empty = !(tb[tile * kTB + 5] > 0.0);
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
     z[k] = ok ? gz[g] : 0.0;
     acc[k] = 0.0;
   }
-  const int64_t q0 = (int64_t)blockIdx.y * qchunk;
+  const int64_t q0 = task.chunk * qchunk;
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const int64_t g = base + (int64_t)k * 64;
-    if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
+    if (g < G) partial[task.chunk * G + g] = acc[k];
   }
 }
 
@@ -142,8 +145,11 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
                                                    const double *__restrict__ gz, int64_t G,
                                                    const double *__restrict__ tb,
                                                    const unsigned long long *__restrict__ mask,
-                                                   int64_t ntile, double *__restrict__ partial) {
-  const int64_t tile = (int64_t)blockIdx.x * (kT / 64) +
+                                                   int64_t ntile, int64_t tiles, int64_t nch,
+                                                   int xcd, double *__restrict__ partial) {
+  const BlockTask task = block_task(tiles, xcd);
+  if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
+  const int64_t tile = task.tile_block * (kT / 64) +
                        __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
   const bool empty = !(tb[tile * kTB + 5] > 0.0);
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
     z[k] = ok ? gz[g] : 0.0;
     acc[k] = 0.0;
   }
-  const int64_t q0 = (int64_t)blockIdx.y * qchunk;
+  const int64_t q0 = task.chunk * qchunk;
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
@@ -167,7 +173,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
 #pragma unroll
   for (int k = 0; k < P; ++k) {
     const int64_t g = base + (int64_t)k * 64;
-    if (g < G) partial[(int64_t)blockIdx.y * G + g] = acc[k];
+    if (g < G) partial[task.chunk * G + g] = acc[k];
   }
 }
 
@@ -247,7 +253,9 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
                 int64_t nch, double *partial) {
   const int64_t G = ctx->G;
   const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
-  dim3 grid((unsigned)tiles, (unsigned)nch);
+  const int xcd = nch >= kXcds;
+  const int64_t nblocks = tiles * (xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch);
+  dim3 grid((unsigned)nblocks);
   hipStream_t st = ctx->stream;
   // bounds of the wave tiles of this P (a few microseconds; redone per launch)
   const bool cut = ctx->cut_log2 > 0.0;
@@ -262,11 +270,12 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     const int64_t nwords = (n + 63) / 64;
     if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, (size_t)(nwords > 0 ? nwords : 1) * ntile)))
       return rc;
+    if ((rc = ensure(ctx, ctx->d_wcount, ctx->wcount_cap, (size_t)(nwords > 0 ? nwords : 1))))
+      return rc;
     if (nwords > 0) {
       hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((nwords + 3) / 4)), dim3(kT), 0, st, q,
-                         ctx->qstride, n, ctx->d_tb, ntile, nwords, ctx->d_mask);
-      hipLaunchKernelGGL(k_mask_evals, dim3((unsigned)((nwords * ntile + kT - 1) / kT)), dim3(kT),
-                         0, st, ctx->d_mask, ctx->d_tb, ntile, nwords, n, ctx->d_evals);
+                         ctx->qstride, n, ctx->d_tb, ntile, nwords, ctx->d_mask, ctx->d_wcount);
+      hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(kT), 0, st, ctx->d_wcount, nwords, ctx->d_evals);
     }
   } else {
     // only the point-count word is read: mark every tile non-empty
@@ -278,11 +287,11 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   if (mixed)                                                                                   \
     hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch, \
                        ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
-                       partial);                                                                \
+                       tiles, nch, xcd, partial);                                              \
   else                                                                                         \
     hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
                        ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
-                       partial);
+                       tiles, nch, xcd, partial);
 #define INCA_EVAL(PP)      \
   if (cut) {               \
     INCA_EVAL2(PP, true)   \

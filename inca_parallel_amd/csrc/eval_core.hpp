@@ -219,6 +219,25 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ rec, 
   }
 }
 
+// XCD-aware block order.  The device dispatches workgroups to its 8 XCDs
+// round-robin by linear index, and each XCD has its own L2.  With `xcd` set,
+// block L runs on XCD L mod 8 and the blocks of XCD x take the chunks
+// c = x (mod 8) in turn, every point tile of one chunk before the next, so a
+// chunk of quartets is streamed from HBM into one L2 and reused there by all
+// tiles.  The grid then has tiles x round_up(nch, 8) blocks, and a block
+// whose chunk is past the end has no work.  Without it (fewer than 8
+// chunks), blocks are tile-major.
+constexpr int kXcds = 8;
+struct BlockTask {
+  int64_t tile_block, chunk;
+};
+__device__ __forceinline__ BlockTask block_task(int64_t tiles, int xcd) {
+  const int64_t L = blockIdx.x;
+  if (!xcd) return {L % tiles, L / tiles};
+  const int64_t k = L / kXcds;
+  return {k % tiles, (k / tiles) * kXcds + (L % kXcds)};
+}
+
 // ---- per-point cutoff: tile bounds and skip masks (shared by the
 // single-context and batched evaluations) ----
 namespace {
@@ -327,48 +346,52 @@ __device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy,
 // cutoff radius at every point of wave tile t.  One wave per 64-quartet word:
 // lane i holds quartet 64 w + i, a ballot per tile gives the word, and the
 // words of 64 consecutive tiles are stored by the 64 lanes at once.
+//
+// wcount[w] (optional) receives the evaluations the mask keeps for the word:
+// the sum over tiles of kept quartets x the tile's points.
 __global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, int qstride,
                                                   int64_t nq, const double *__restrict__ tb,
                                                   int64_t ntile, int64_t nwords,
-                                                  unsigned long long *__restrict__ mask) {
+                                                  unsigned long long *__restrict__ mask,
+                                                  unsigned long long *__restrict__ wcount) {
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   if (w >= nwords) return;  // uniform per wave
   const int64_t qi = w * 64 + lane;
   const bool valid = qi < nq;
+  const unsigned long long vbits = __ballot(valid);
   QHeader h{};
   if (valid) h = load_header(q + qi * qstride);
+  unsigned long long kept = 0;
   for (int64_t t0 = 0; t0 < ntile; t0 += 64) {
     unsigned long long keep = 0;
     const int nt = (int)((ntile - t0) < 64 ? (ntile - t0) : 64);
     for (int j = 0; j < nt; ++j) {
       const double *__restrict__ tbw = tb + (t0 + j) * kTB;
-      const bool far = valid && tbw[5] > 0.0 && tile_far(h, tbw[0], tbw[1], tbw[2], tbw[3], tbw[4]);
+      const double npts = tbw[5];
+      const bool far = valid && npts > 0.0 && tile_far(h, tbw[0], tbw[1], tbw[2], tbw[3], tbw[4]);
       const unsigned long long m = __ballot(far);
       if (lane == j) keep = m;
+      kept += (unsigned long long)__popcll(vbits & ~m) * (unsigned long long)npts;
     }
     if (lane < nt) mask[w * ntile + t0 + lane] = keep;
   }
+  if (wcount && lane == 0) wcount[w] = kept;
 }
 
-// Executed evaluations under the skip mask: for every word and tile, the
-// kept quartets times the tile's points (integer atomics: exact, order-free).
-__global__ __launch_bounds__(kT) void k_mask_evals(const unsigned long long *__restrict__ mask,
-                                                   const double *__restrict__ tb, int64_t ntile,
-                                                   int64_t nwords, int64_t nq,
-                                                   unsigned long long *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  unsigned long long v = 0;
-  if (i < nwords * ntile) {
-    const int64_t w = i / ntile, t = i - w * ntile;
-    unsigned long long live = ~mask[i];
-    const int64_t left = nq - w * 64;
-    if (left < 64) live &= (1ull << left) - 1ull;
-    v = (unsigned long long)__popcll(live) * (unsigned long long)tb[t * kTB + 5];
+// Fixed-order sum of n counters by one block.
+__global__ __launch_bounds__(kT) void k_sum_u64(const unsigned long long *__restrict__ x,
+                                                int64_t n, unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long red[kT];
+  unsigned long long s = 0;
+  for (int64_t i = threadIdx.x; i < n; i += kT) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
   }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+  if (threadIdx.x == 0) out[0] += red[0];
 }
 
 // Calls f(qi) for the quartets of [a, b) that the skip mask keeps for
