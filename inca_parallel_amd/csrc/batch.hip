@@ -7,11 +7,14 @@
 //     (Lsum, Lx, Ly, Lz), so that every work item holds quartets of a
 //     single class and its cost is known in advance;
 //   * all grid points back to back, with per-molecule offsets.
-// Work items are (molecule, block tile of 256 P points, class chunk of
-// quartets); they are ordered by estimated cost, largest first, and a
-// persistent grid of workgroups pulls them from an atomic counter until the
-// list is exhausted, so small and large molecules share the device without
-// one launch per molecule.  Each item writes its own partial-sum slot and a
+// Work items are (molecule, block tile of 256 P points, class chunk of at most
+// 4096 quartets).  All items of one chunk go to one of 8 queues (chunk mod
+// 8), each ordered by estimated cost, largest first; a persistent grid of
+// workgroups pulls from the queues through atomic counters, block L serving
+// the queue of its own XCD (L mod 8, the round-robin dispatch) first and
+// the others once it is empty, so a chunk's records are read into one L2 and
+// small and large molecules share the device without one launch per
+// molecule.  Each item writes its own partial-sum slot and a
 // fixed-order reduction follows, so results do not depend on which
 // workgroup took which item.  The per-item arithmetic is the single-context
 // FP64 kernel's (eval_core.hpp).  Points per lane are autotuned over
@@ -64,7 +67,8 @@ struct inca_batch {
   size_t tb_cap = 0;
   unsigned long long *d_mask = nullptr;
   size_t mask_cap = 0;
-  unsigned int *d_counter = nullptr;
+  unsigned int *d_counter = nullptr;  // one work counter per XCD queue
+  int64_t *d_queues = nullptr;        // per XCD queue: first item, item count
   int64_t nitems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -78,6 +82,7 @@ namespace {
 
 constexpr int kItem = 7;  // int64 words per item: mol, tile, qoff, n, stride, partial, local q
 constexpr int kMol = 6;   // int64 words per molecule: g_off, G, part_off, nch, mask off, tiles
+constexpr int64_t kMaxChunk = 4096;  // quartets per work item at most
 
 int bfail(inca_batch *b, int code, const std::string &msg) {
   b->err = msg;
@@ -92,7 +97,7 @@ int bfail(inca_batch *b, int code, const std::string &msg) {
   } while (0)
 
 __global__ void k_batch_reset(unsigned int *counter) {
-  if (threadIdx.x == 0) counter[0] = 0u;
+  if (threadIdx.x < kXcds) counter[threadIdx.x] = 0u;
 }
 
 template <int P>
@@ -103,19 +108,39 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
                                                    const double *__restrict__ gz,
                                                    const int64_t *__restrict__ mol,
                                                    const unsigned long long *__restrict__ mask,
+                                                   const int64_t *__restrict__ queues,
                                                    double *__restrict__ partial,
                                                    unsigned int *__restrict__ counter) {
-  __shared__ unsigned int s_it;
+  __shared__ int s_next[2];  // queue, item within it (-1: every queue is empty)
   __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // block L runs on XCD L mod 8 (round-robin dispatch): it serves that XCD's
+  // queue first, then the others in turn once it is empty
+  int home = (int)(blockIdx.x % kXcds), tried = 0;
   for (;;) {
-    if (threadIdx.x == 0) s_it = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+      int found = -1;
+      unsigned int pos = 0;
+      while (tried < kXcds) {
+        pos = atomicAdd(&counter[home], 1u);
+        if ((int64_t)pos < queues[2 * home + 1]) {
+          found = home;
+          break;
+        }
+        home = (home + 1) % kXcds;
+        ++tried;
+      }
+      s_next[0] = found;
+      s_next[1] = (int)pos;
+    }
     __syncthreads();
-    const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane(s_it);
-    __syncthreads();  // s_it is read by every wave before the next fetch
-    if (it >= nitems) break;  // uniform over the block: every wave leaves together
+    const int qf = __builtin_amdgcn_readfirstlane(s_next[0]);
+    const int64_t pos = (int64_t)__builtin_amdgcn_readfirstlane(s_next[1]);
+    __syncthreads();  // read by every wave before the next fetch
+    if (qf < 0) break;  // uniform over the block: every wave leaves together
+    const int64_t it = queues[2 * qf] + pos;
     const int64_t *__restrict__ d = items + it * kItem;
     const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5], lq = d[6];
     const int stride = (int)d[4];
@@ -231,22 +256,25 @@ int build_items(inca_batch *b, int P) {
   double work = 0.0;
   for (int m = 0; m < b->nmol; ++m)
     work += (double)b->nq[m] * (double)((b->G[m] + tile_pts - 1) / tile_pts);
+  // chunks of at most kMaxChunk quartets (their records stay within an
+  // XCD's L2 while the blocks of one chunk run) and at least 16 items per
+  // persistent block
   const double target = 16.0 * std::max(1, b->nblocks);
   int64_t qchunk = (int64_t)std::ceil(work / target);
-  qchunk = std::max<int64_t>(32, std::min<int64_t>(qchunk, 1 << 20));
+  qchunk = std::max<int64_t>(32, std::min<int64_t>(qchunk, kMaxChunk));
   struct It {
     std::array<int64_t, kItem> w;
     double cost;
   };
-  std::vector<It> items;
-  int64_t part = 0;
+  std::vector<It> queue[kXcds];  // a chunk's items share one XCD queue
+  int64_t part = 0, chunk_id = 0;
   for (int m = 0; m < b->nmol; ++m) {
     const int64_t tiles = (b->G[m] + tile_pts - 1) / tile_pts;
     b->part_off[m] = part;
     int64_t c = 0;
     for (const auto &sg : b->segs[m]) {
       const int lsum = (int)(sg[2] >> 24);
-      for (int64_t s = 0; s < sg[1]; s += qchunk, ++c) {
+      for (int64_t s = 0; s < sg[1]; s += qchunk, ++c, ++chunk_id) {
         const int64_t n = std::min(qchunk, sg[1] - s);
         const int64_t qo = b->q_off[m] + (sg[0] + s) * b->stride[m];
         for (int64_t t = 0; t < tiles; ++t) {
@@ -254,21 +282,32 @@ int build_items(inca_batch *b, int P) {
           It it;
           it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m], sg[0] + s};
           it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts;
-          items.push_back(it);
+          queue[chunk_id % kXcds].push_back(it);
         }
       }
     }
     b->nch[m] = c;
     part += c * b->G[m];
   }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const It &a, const It &c) { return a.cost > c.cost; });
-  b->nitems = (int64_t)items.size();
+  b->nitems = 0;
+  int64_t qinfo[2 * kXcds];
+  for (int x = 0; x < kXcds; ++x) {
+    std::stable_sort(queue[x].begin(), queue[x].end(),
+                     [](const It &a, const It &c) { return a.cost > c.cost; });
+    qinfo[2 * x] = b->nitems;
+    qinfo[2 * x + 1] = (int64_t)queue[x].size();
+    b->nitems += (int64_t)queue[x].size();
+  }
   std::vector<int64_t> flat((size_t)std::max<int64_t>(1, b->nitems) * kItem);
-  for (size_t i = 0; i < items.size(); ++i)
-    std::copy(items[i].w.begin(), items[i].w.end(), flat.begin() + i * kItem);
+  size_t i = 0;
+  for (int x = 0; x < kXcds; ++x)
+    for (const It &it : queue[x]) {
+      std::copy(it.w.begin(), it.w.end(), flat.begin() + i * kItem);
+      ++i;
+    }
   int rc;
   if ((rc = bensure(b, b->d_items, b->items_cap, flat.size()))) return rc;
+  B_HIP(hipMemcpy(b->d_queues, qinfo, sizeof(qinfo), hipMemcpyHostToDevice));
   if ((rc = bensure(b, b->d_partial, b->partial_cap, (size_t)std::max<int64_t>(1, part)))) return rc;
   B_HIP(hipMemcpy(b->d_items, flat.data(), flat.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   // cutoff masks for this P: wave tiles of 64 P points per molecule
@@ -324,7 +363,7 @@ int enqueue(inca_batch *b, int P) {
 #define INCA_BEVAL(PP)                                                                        \
   hipLaunchKernelGGL(k_batch_eval<PP>, dim3(b->nblocks), dim3(kT), 0, st, b->d_items,         \
                      b->nitems, b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_mol,                   \
-                     b->cut ? b->d_mask : nullptr, b->d_partial, b->d_counter)
+                     b->cut ? b->d_mask : nullptr, b->d_queues, b->d_partial, b->d_counter)
     switch (P) {
       case 1: INCA_BEVAL(1); break;
       case 2: INCA_BEVAL(2); break;
@@ -353,7 +392,7 @@ int persistent_blocks(inca_batch *b, int P) {
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_eval<4>, kT, 0); break;
   }
   B_HIP(e);
-  b->nblocks = std::max(1, ncu * std::max(1, per));
+  b->nblocks = std::max(kXcds, ncu * std::max(1, per) / kXcds * kXcds);
   return 0;
 }
 
@@ -425,7 +464,8 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   if ((cap = 0, rc = bensure(b, b->d_res, cap, 2 * (size_t)nmol))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_mol, cap, kMol * (size_t)nmol))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_pmol, cap, gt))) return rc;
-  if ((cap = 0, rc = bensure(b, b->d_counter, cap, 1))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_counter, cap, kXcds))) return rc;
+  if ((cap = 0, rc = bensure(b, b->d_queues, cap, 2 * kXcds))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_pmap, cap, gt))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_cent, cap, 3 * (size_t)std::max<int64_t>(1, ncent)))) return rc;
   hipStream_t st = b->stream;
@@ -517,7 +557,7 @@ int inca_batch_destroy(inca_batch *b) {
   void *bufs[] = {b->d_q,     b->d_gx,    b->d_gy,   b->d_gz,    b->d_w,
                   b->d_wvee,  b->d_ivec,  b->d_res,  b->d_partial, b->d_items,
                   b->d_mol,   b->d_pmol,  b->d_counter, b->d_pmap, b->d_cent,
-                  b->d_tb,    b->d_mask};
+                  b->d_tb,    b->d_mask,  b->d_queues};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : b->ev)
