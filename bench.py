@@ -149,6 +149,31 @@ def profiled_traffic():
     return None
 
 
+def cube_bench(capi, eng):
+    """The cube workload on the same cluster and DM2: 80^3 voxels, step 0.15
+    bohr, centred at the origin (the cubeintra grid).  One FP64 and one mixed
+    grid loop over the prepared quartets, with the mixed error per voxel."""
+    n = eng.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
+    out = {"points": n}
+    for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
+        eng.prepare(prec, THRESH)
+        t0 = time.perf_counter()
+        iv = eng.evaluate_prepared(prec, want_ivec=True)
+        wall = time.perf_counter() - t0
+        tm = eng.timings()
+        ref, ex = eng.eval_counts()
+        out[name] = {"eval_ms": float(tm[1] + tm[2]), "wall_s": wall,
+                     "evals_per_s": ref / ((tm[1] + tm[2]) * 1e-3)}
+        out["reference_evals"], out["executed_evals"] = ref, ex
+        if name == "fp64":
+            iv64 = iv
+    big = np.abs(iv64) >= 1e-6 * np.abs(iv64).max()
+    out["mixed"]["max_rel_err_voxels_vs_fp64"] = float(
+        np.max(np.abs(iv - iv64)[big] / np.abs(iv64[big])))
+    out["mixed"]["rel_err_sum_vs_fp64"] = float(abs(iv.sum() - iv64.sum()) / abs(iv64.sum()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +182,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--natoms", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-cube", action="store_true")
     ap.add_argument("--tiles", default=None,
                     help="points per lane 'fp64,fp32' instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -284,12 +310,16 @@ def main():
         "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
                       "reduce": float(r64["kern_ms"][2])},
     }
+    grid_r = eng.get_grid()["r"]  # the radial-plot grid, for the CPU baseline
+    if not a.no_cube:
+        out["cube"] = cube_bench(capi, eng)
+        log("cube: %s" % out["cube"])
     if not a.no_batch:
         out["batch"] = batch_bench(capi, synth, torch)
         log("batch: %s" % out["batch"])
     if not a.no_cpu:
         rec_host = recs.cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(sysm, rec_host, eng.get_grid()["r"])
+        out["cpu_baseline"] = cpu_baseline(sysm, rec_host, grid_r)
     print(json.dumps(out))
     eng.close()
     return 0
