@@ -32,7 +32,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
-           "inca_mlp_set_params", "inca_mlp_forward", "inca_mlp_loss_grad", "inca_mlp_adam_step",
+           "inca_mlp_set_params", "inca_mlp_load", "inca_mlp_forward", "inca_mlp_loss_grad", "inca_mlp_adam_step",
            "inca_atom_features", "inca_learned_autogrid",
            "inca_lebedev", "inca_gauleg"]
 
@@ -105,6 +105,7 @@ def load():
     L.inca_mlp_get_params.argtypes = [vp, vp]
     L.inca_mlp_set_params.argtypes = [vp, vp]
     L.inca_mlp_forward.argtypes = [vp, C.c_int32, vp, vp]
+    L.inca_mlp_load.argtypes = [vp, C.c_char_p]
     L.inca_mlp_loss_grad.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
     L.inca_mlp_adam_step.argtypes = [vp, C.c_float]
     L.inca_atom_features.argtypes = [C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp]
@@ -214,6 +215,10 @@ class MLP:
         p = np.ascontiguousarray(p, np.float32)
         assert p.size == MLP_NPARAM
         self._ck(self._L.inca_mlp_set_params(self._h, p.ctypes.data), "inca_mlp_set_params")
+
+    def load(self, path):
+        """Parameters from a .npy file (inca_mlp_load)."""
+        self._ck(self._L.inca_mlp_load(self._h, os.fsencode(path)), "inca_mlp_load")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

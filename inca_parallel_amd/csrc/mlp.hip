@@ -27,6 +27,7 @@
 // The sizes are tiny (B x 64 x 64 with B a few thousand atoms), so plain
 // FP32 FMAs from LDS-resident weights are used rather than matrix cores.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -381,6 +382,45 @@ int inca_mlp_set_params(inca_mlp *m, const float *p) {
   M_HIP(hipMemcpy(m->d_v, z.data(), kNParam * sizeof(float), hipMemcpyHostToDevice));
   m->steps = 0;
   return 0;
+}
+
+int inca_mlp_load(inca_mlp *m, const char *path) {
+  if (!m || !path) return INCA_EINVAL;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return mfail(m, INCA_EIO, std::string("cannot open ") + path);
+  // .npy: magic, version, header length, a dict header, then the raw array
+  unsigned char pre[10];
+  std::vector<float> p(kNParam);
+  int rc = INCA_EFORMAT;
+  std::string why = "not a .npy file";
+  if (std::fread(pre, 1, 10, f) == 10 && std::memcmp(pre, "\x93NUMPY", 6) == 0) {
+    size_t hlen = 0;
+    bool ok = true;
+    if (pre[6] == 1) {
+      hlen = pre[8] | (pre[9] << 8);
+    } else {
+      unsigned char more[2];
+      ok = std::fread(more, 1, 2, f) == 2;
+      hlen = pre[8] | (pre[9] << 8) | ((size_t)more[0] << 16) | ((size_t)more[1] << 24);
+    }
+    std::string hdr(hlen, '\0');
+    ok = ok && hlen < 65536 && std::fread(&hdr[0], 1, hlen, f) == hlen;
+    auto has = [&](const char *t) { return hdr.find(t) != std::string::npos; };
+    const std::string shape = "(" + std::to_string(kNParam) + ",)";
+    if (!ok) {
+      why = "truncated .npy header";
+    } else if (!has("'descr': '<f4'") || !has("'fortran_order': False") ||
+               !has(("'shape': " + shape).c_str())) {
+      why = "expected a little-endian float32 array of shape " + shape;
+    } else if (std::fread(p.data(), sizeof(float), kNParam, f) != (size_t)kNParam) {
+      why = "truncated .npy data";
+    } else {
+      rc = 0;
+    }
+  }
+  std::fclose(f);
+  if (rc) return mfail(m, rc, std::string(path) + ": " + why);
+  return inca_mlp_set_params(m, p.data());
 }
 
 int inca_mlp_forward(inca_mlp *m, int32_t n, const float *x, float *out) {

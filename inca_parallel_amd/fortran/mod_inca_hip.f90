@@ -130,6 +130,48 @@ module inca_hip
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: t(3)
       end function
+      integer(c_int) function inca_evaluate_prepared(ctx, precision, ivec) &
+            bind(C, name="inca_evaluate_prepared")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx, ivec
+         integer(c_int32_t), value :: precision
+      end function
+      integer(c_int) function inca_get_analytic_integral(ctx, total) &
+            bind(C, name="inca_get_analytic_integral")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: total
+      end function
+      integer(c_int) function inca_mlp_create(device, seed, m) bind(C, name="inca_mlp_create")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         integer(c_int32_t), value :: device
+         integer(c_int64_t), value :: seed
+         type(c_ptr), intent(out) :: m
+      end function
+      integer(c_int) function inca_mlp_destroy(m) bind(C, name="inca_mlp_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
+      end function
+      type(c_ptr) function inca_mlp_last_error(m) bind(C, name="inca_mlp_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: m
+      end function
+      integer(c_int) function inca_mlp_load(m, path) bind(C, name="inca_mlp_load")
+         import :: c_int, c_ptr, c_char
+         type(c_ptr), value :: m
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function inca_learned_autogrid(m, natoms, an, cartes, nprim, alpha, tmn, &
+            prim_atom, nohydro, cap, nquad, cent, ps, sfalpha, nradc, nangc, radius) &
+            bind(C, name="inca_learned_autogrid")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: m
+         integer(c_int32_t), value :: natoms, nprim, nohydro, cap
+         integer(c_int32_t), intent(in) :: an(*), tmn(*), prim_atom(*)
+         real(c_double), intent(in) :: cartes(*), alpha(*)
+         integer(c_int32_t), intent(out) :: nquad
+         type(c_ptr), value :: cent, ps, sfalpha, nradc, nangc, radius
+      end function
       integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s
@@ -184,6 +226,22 @@ contains
       rc = inca_destroy(inca_ctx)
       inca_ctx = c_null_ptr
    end subroutine inca_close
+
+   !! Message of the last failing network call, as a Fortran string.
+   function inca_mlp_error_text(m) result(msg)
+      type(c_ptr), intent(in) :: m
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: chars(:)
+      integer :: n, i
+      p = inca_mlp_last_error(m)
+      n = int(c_strlen(p))
+      allocate(character(len=n) :: msg)
+      call c_f_pointer(p, chars, [n])
+      do i = 1, n
+         msg(i:i) = chars(i)
+      end do
+   end function inca_mlp_error_text
 
    !! NUL-terminated copy of a Fortran string for the C side.
    function c_string(s) result(cs)

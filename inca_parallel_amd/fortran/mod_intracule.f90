@@ -11,18 +11,20 @@ module intracule
    use intrainfo
    use quadratures
    use inca_hip
+   use autogrid, only: centercalc
    implicit none
 
    integer(c_int64_t), private :: counts(4) = 0
    double precision, private :: kern_ms(3) = 0.d0
    integer(c_int64_t), private :: evals_ref = 0, evals_exec = 0
+   character(len=200), private :: grid_note = ''
 
 contains
 
    subroutine calculate_intracule(normalize_dm2p)
       logical, intent(in) :: normalize_dm2p
       integer, parameter :: iout = 15
-      integer(c_int64_t) :: nrec, mg, rg, rrg, nr, ng
+      integer(c_int64_t) :: nrec, nr, ng
       integer(c_int32_t) :: fmt
       integer(c_int) :: rc
       integer(c_int32_t), allocatable, target :: tmn32(:)
@@ -52,10 +54,7 @@ contains
 
       ! grid of the main job (mod_intracule.f90:91-112)
       if (radial_integral) then
-         call inca_check(inca_grid_multicenter(inca_ctx, int(nquad, c_int32_t), cent, &
-              int(nradc, c_int32_t), int(nangc, c_int32_t), sfalpha, Ps, &
-              merge(1_c_int32_t, 0_c_int32_t, nosym), mg, rg, rrg), "inca_grid_multicenter")
-         maxgrid = int(mg); rgrid = int(rg); rrgrid = int(rrg)
+         call multicenter_grid()
          write(*,*) "Number of grid points to compute intracule:", rrgrid
       else if (radial_plot) then
          call inca_check(inca_grid_radial_plot(inca_ctx, int(nblock, c_int32_t), tart, stp, &
@@ -102,6 +101,7 @@ contains
          call evaluate(ivec)
          call inca_check(inca_get_eval_counts(inca_ctx, evals_ref, evals_exec), &
                          "inca_get_eval_counts")
+         if (radial_integral .and. learned_grid_used) call check_learned_grid(ivec)
          ! reductions on the main grid before any other grid replaces it
          if (radial_integral) then
             allocate(ipc(nquad), vpc(nquad))
@@ -228,6 +228,7 @@ contains
       else
          write(iout,*) "Per-point cutoff: off (every term at every point)"
       end if
+      if (len_trim(grid_note) > 0) write(iout,*) trim(grid_note)
       write(iout,*) "=========================================================="
       close(iout)
 
@@ -267,6 +268,76 @@ contains
       end if
       write(*,*) ">> INCA job finished successfully. Output written to: ", trim(outname)
    end subroutine calculate_intracule
+
+   !! The multicentre grid of the current centres (mod_intracule.f90:91-104).
+   subroutine multicenter_grid()
+      integer(c_int64_t) :: mg, rg, rrg
+      call inca_check(inca_grid_multicenter(inca_ctx, int(nquad, c_int32_t), cent, &
+           int(nradc, c_int32_t), int(nangc, c_int32_t), sfalpha, Ps, &
+           merge(1_c_int32_t, 0_c_int32_t, nosym), mg, rg, rrg), "inca_grid_multicenter")
+      maxgrid = int(mg); rgrid = int(rg); rrgrid = int(rrg)
+   end subroutine multicenter_grid
+
+   !! Sum-rule check of the learned autogrid.  Its integral is compared with
+   !! the analytic integral of the prepared terms over all space (which every
+   !! grid integral approximates).  The learned grid stands when its relative
+   !! error is at most 1e-8, or no more than 1e-8 above the plain autogrid's
+   !! (centercalc, evaluated on the same prepared quartets); otherwise the
+   !! plain autogrid's centres, grid and Ivec replace it.
+   subroutine check_learned_grid(iv)
+      double precision, allocatable, target, intent(inout) :: iv(:)
+      double precision, parameter :: tol = 1.d-8
+      double precision, allocatable :: lc(:,:), lp(:), la(:)
+      integer, allocatable :: lr(:), lg(:)
+      double precision :: exact, err_l, err_p
+
+      call inca_check(inca_get_analytic_integral(inca_ctx, exact), "inca_get_analytic_integral")
+      err_l = grid_error()
+      if (err_l <= tol) then
+         write(grid_note,'(A,ES10.3,A,ES24.16)') "Learned autogrid: accepted, sum-rule error ", &
+              err_l, "; analytic integral ", exact
+         return
+      end if
+      lc = cent; lp = Ps; la = sfalpha; lr = nradc; lg = nangc
+      write(*,*) "Learned grid sum-rule error", err_l, "; evaluating the plain autogrid"
+      call centercalc()
+      call multicenter_grid()
+      call reevaluate()
+      err_p = grid_error()
+      if (err_l > err_p + tol) then
+         write(grid_note,'(A,ES10.3,A,ES10.3,A,ES24.16)') "Learned autogrid: rejected, &
+              &sum-rule error ", err_l, "; plain autogrid used, error ", err_p, &
+              "; analytic integral ", exact
+         return
+      end if
+      nquad = size(lp)
+      deallocate(cent, Ps, sfalpha, nradc, nangc)
+      cent = lc; Ps = lp; sfalpha = la; nradc = lr; nangc = lg
+      call multicenter_grid()
+      call reevaluate()
+      write(grid_note,'(A,ES10.3,A,ES10.3,A,ES24.16)') "Learned autogrid: accepted, sum-rule &
+           &error ", err_l, "; plain autogrid ", err_p, "; analytic integral ", exact
+   contains
+      double precision function grid_error()
+         double precision, target :: tot, v
+         double precision, allocatable, target :: ip(:), vp(:)
+         allocate(ip(nquad), vp(nquad))
+         call inca_check(inca_integrate(inca_ctx, c_loc(tot), c_loc(v), c_loc(ip), c_loc(vp)), &
+              "inca_integrate")
+         grid_error = abs(tot - exact)/abs(exact)
+      end function grid_error
+      subroutine reevaluate()
+         integer :: n
+         n = max(rrgrid, rgrid, 1)
+         if (allocated(iv)) deallocate(iv)
+         allocate(iv(n))
+         iv = ZERO
+         call inca_check(inca_evaluate_prepared(inca_ctx, merge(INCA_MIXED, INCA_FP64, &
+              mixed_precision), c_loc(iv)), "inca_evaluate_prepared")
+         call inca_check(inca_get_eval_counts(inca_ctx, evals_ref, evals_exec), &
+              "inca_get_eval_counts")
+      end subroutine reevaluate
+   end subroutine check_learned_grid
 
    !! Evaluate Ivec on the engine's current grid and copy it to the host.
    subroutine evaluate(iv)

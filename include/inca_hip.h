@@ -200,6 +200,11 @@ const char *inca_mlp_last_error(const inca_mlp *m);
 int inca_mlp_get_params(inca_mlp *m, float *params);
 /* Also resets the optimiser state. */
 int inca_mlp_set_params(inca_mlp *m, const float *params);
+/* Parameters from a .npy file (little-endian float32, shape
+ * (INCA_MLP_NPARAM,), as numpy.save writes them); resets the optimiser.
+ * INCA_EIO if the file cannot be read, INCA_EFORMAT if it is not such an
+ * array. */
+int inca_mlp_load(inca_mlp *m, const char *path);
 /* out(n,9) = network(x(n,16)), row-major. */
 int inca_mlp_forward(inca_mlp *m, int32_t n, const float *x, float *out);
 /* Loss over n atoms with targets (n,3) = [log radius factor, nrad class

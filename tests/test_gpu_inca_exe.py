@@ -104,3 +104,34 @@ def test_integral_mixed_precision(tmp_path):
     val = grab(out, r"TOTAL VALUE OF INTRACULE\s*=\s*(\S+)")
     print("mixed integral rel err vs golden %.3e" % (abs(val - GD.INTEGRAL) / GD.INTEGRAL))
     assert abs(val - GD.INTEGRAL) / GD.INTEGRAL < 1e-8
+
+
+def test_integral_learned_grid(tmp_path):
+    """learned_grid_weights: the network's radii and orders on the autogrid
+    centres, with the sum-rule fallback.  The final integral may not be
+    further from the analytic integral of the prepared terms than the plain
+    autogrid's (the golden integral, same centres and orders) by more than
+    1e-8 relative; a rejected learned grid reproduces the golden integral."""
+    w = os.path.join(ROOT, "inca_parallel_amd", "data", "autogrid_mlp.npy")
+    out = run_case(tmp_path, "test_integral.inp", "learned_grid_weights = '%s'" % w)
+    txt = open(out).read()
+    note = re.search(r"Learned autogrid: (accepted|rejected).*analytic integral\s*(\S+)", txt)
+    assert note, txt
+    exact = float(note.group(2))
+    val = grab(out, r"TOTAL VALUE OF INTRACULE\s*=\s*(\S+)")
+    err_final = abs(val - exact) / abs(exact)
+    err_plain = abs(GD.INTEGRAL - exact) / abs(exact)
+    print("learned grid: %s, err final %.3e, err plain %.3e" % (note.group(1), err_final, err_plain))
+    assert err_final <= err_plain + 1e-8
+    if note.group(1) == "rejected":
+        assert abs(val - GD.INTEGRAL) / GD.INTEGRAL < 1e-10
+
+
+def test_learned_grid_bad_weights(tmp_path):
+    """A weights file that is not a (9993,) float32 .npy stops the job with
+    the loader's message."""
+    bad = tmp_path / "bad.npy"
+    bad.write_bytes(b"not a numpy file")
+    with pytest.raises(AssertionError) as e:
+        run_case(tmp_path, "test_integral.inp", "learned_grid_weights = '%s'" % bad)
+    assert "inca_mlp_load" in str(e.value)
