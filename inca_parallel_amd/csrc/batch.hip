@@ -330,7 +330,7 @@ int build_items(inca_batch *b, int P) {
         hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(nt / (kT / 64))), dim3(kT), 0, b->stream,
                            b->d_gx + b->g_off[m], b->d_gy + b->g_off[m], b->d_gz + b->g_off[m],
                            b->G[m], P, b->mode[m], b->d_pmap + b->g_off[m],
-                           b->d_cent + 3 * b->c_off[m], nt, b->d_tb + toff * kTB);
+                           b->d_cent + 3 * b->c_off[m], nt, b->d_tb + toff * kTB, nullptr);
         if (b->nword[m] > 0)
           hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((b->nword[m] + 3) / 4)), dim3(kT), 0,
                              b->stream, b->d_q + b->q_off[m], b->stride[m], b->nq[m],

@@ -66,6 +66,18 @@ struct inca_ctx {
   std::vector<int64_t> centre_first;  // first point of each centre (+ end)
   std::vector<double> radi;           // radial scan radii
   std::vector<int32_t> smn;           // kept points per radius
+  int64_t grid_serial = 0;            // bumped by every new grid
+
+  // ray tiles of a radial scan (index list per points-per-lane choice)
+  int64_t ray_serial = -1;    // grid the ray analysis belongs to
+  double ray_h = 0.0;         // radial step of the ray runs (0: no rays)
+  std::vector<int32_t> ray_runs;  // (first shell, shell count) of each ray run
+  int ray_P = 0;              // points per lane of the ray tiles (0: no list)
+  int64_t ray_ntile = 0;      // ray wave tiles (multiple of 4)
+  int64_t gen_ntile = 0;      // wave tiles of the other points, 4 per lane (multiple of 4)
+  int32_t *d_ridx = nullptr;  // [tile][k][lane] grid index (-1: none): ray tiles, then the others
+  size_t ridx_cap = 0;
+  double rho_h = 0.0;         // step whose rho the prepared records hold in word 7
 
   // DM2 records (24 B each)
   int64_t nrec = 0;

@@ -26,7 +26,10 @@
 namespace inca {
 namespace {
 
-template <int P, bool CUT>
+// IDX: the tile's points come from the index list idx[t][k][lane] (-1:
+// none) of local tile t, whose tile words and mask columns are those of
+// tile tile0 + t.
+template <int P, bool CUT, bool IDX>
 __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
@@ -35,20 +38,26 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
                                                   const double *__restrict__ tb,
                                                   const unsigned long long *__restrict__ mask,
                                                   int64_t ntile, int64_t tiles, int64_t nch,
-                                                  int xcd, double *__restrict__ partial) {
+                                                  int xcd, const int32_t *__restrict__ idx,
+                                                  int64_t tile0, double *__restrict__ partial) {
   const BlockTask task = block_task(tiles, xcd);
   if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
-  const int64_t tile = task.tile_block * (kT / 64) +
-                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const int64_t lt = task.tile_block * (kT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t tile = tile0 + lt;
+  const int64_t base = lt * (64 * P) + (threadIdx.x & 63);
   const bool empty = !(tb[tile * kTB + 5] > 0.0);
   __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
+  auto point = [&](int k) -> int64_t {
+    if (IDX) return idx[base + (int64_t)k * 64];
+    const int64_t g = base + (int64_t)k * 64;
+    return g < G ? g : -1;
+  };
   double x[P], y[P], z[P], acc[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * 64;
-    const bool ok = g < G;
+    const int64_t g = point(k);
+    const bool ok = g >= 0;
     x[k] = ok ? gx[g] : 0.0;
     y[k] = ok ? gy[g] : 0.0;
     z[k] = ok ? gz[g] : 0.0;
@@ -58,14 +67,74 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
-    const QHeader h = load_header(rec);
-    accumulate_fp64<P>(rec, h, x, y, z, acc, s_exp2);
+    accumulate_fp64<P>(rec, load_header(rec), x, y, z, acc, s_exp2);
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = base + (int64_t)k * 64;
-    if (g < G) partial[task.chunk * G + g] = acc[k];
+    const int64_t g = point(k);
+    if (g >= 0) partial[task.chunk * G + g] = acc[k];
   }
+}
+
+// Ray tiles of a radial scan (tiles 0 .. of the launch): lane `lane` of tile
+// t holds the P points idx[t][k][lane], k < P, one direction at P
+// consecutive radii a step h apart; the lane keeps its first point o and
+// h w, and the quartets are evaluated by ray_case.
+template <int P, bool CUT>
+__global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, int qstride,
+                                                  int64_t nq, int64_t qchunk,
+                                                  const double *__restrict__ gx,
+                                                  const double *__restrict__ gy,
+                                                  const double *__restrict__ gz, int64_t G,
+                                                  const double *__restrict__ tb,
+                                                  const unsigned long long *__restrict__ mask,
+                                                  int64_t ntile, int64_t tiles, int64_t nch,
+                                                  int xcd, const int32_t *__restrict__ idx,
+                                                  double h, double *__restrict__ partial) {
+  const BlockTask task = block_task(tiles, xcd);
+  if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
+  const int64_t tile = task.tile_block * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const bool empty = !(tb[tile * kTB + 5] > 0.0);
+  __shared__ double s_exp2[kExpTable];
+  stage_exp_table(s_exp2);
+  const int32_t g0 = idx[base];
+  double o[3] = {0.0, 0.0, 0.0}, hw[3] = {0.0, 0.0, 0.0};
+  if (g0 >= 0) {
+    o[0] = gx[g0];
+    o[1] = gy[g0];
+    o[2] = gz[g0];
+    const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
+    const double f = r0 > 0.0 ? h / r0 : 0.0;
+    hw[0] = f * o[0];
+    hw[1] = f * o[1];
+    hw[2] = f * o[2];
+  }
+  const double span = (P - 1) * h;
+  double acc[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) acc[k] = 0.0;
+  const int64_t q0 = task.chunk * qchunk;
+  const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
+  for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
+    const double *__restrict__ rec = q + qi * qstride;
+    ray_any<P>(rec, load_header(rec), o, hw, span, acc, s_exp2);
+  });
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int32_t g = idx[base + k * 64];
+    if (g >= 0) partial[task.chunk * G + g] = acc[k];
+  }
+}
+
+// rho = exp(-2 sqe^2 h^2) of every prepared record into word 7 (ray tiles).
+__global__ __launch_bounds__(kT) void k_ray_rho(double *__restrict__ q, int qstride, int64_t nq,
+                                                double h2) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= nq) return;
+  const double sqe = q[i * qstride];
+  q[i * qstride + 7] = exp(-2.0 * sqe * sqe * h2);
 }
 
 // Mixed precision: rp in FP64 (sqe r + sqe d cancels), everything after it
@@ -124,8 +193,7 @@ __device__ __forceinline__ void accumulate_mixed_any(const double *__restrict__ 
                                                      const QHeader &h, const double (&x)[P],
                                                      const double (&y)[P], const double (&z)[P],
                                                      double (&acc)[P]) {
-  const int mask = (h.Lx > 0) | ((h.Ly > 0) << 1) | ((h.Lz > 0) << 2);
-  switch (mask) {
+  switch (h.axes) {
     case 0: accumulate_mixed<P, false, false, false>(rec, h, x, y, z, acc); break;
     case 1: accumulate_mixed<P, true, false, false>(rec, h, x, y, z, acc); break;
     case 2: accumulate_mixed<P, false, true, false>(rec, h, x, y, z, acc); break;
@@ -167,8 +235,7 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
-    const QHeader h = load_header(rec);
-    accumulate_mixed_any<P>(rec, h, x, y, z, acc);
+    accumulate_mixed_any<P>(rec, load_header(rec), x, y, z, acc);
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -247,26 +314,202 @@ int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
 
 namespace {
 
-// Launch one evaluation kernel over quartets [q, q+n) in `nch` chunks of
-// `qch` quartets with P points per lane; partial sums go to partial[chunk][g].
+// ---- ray tiles of a radial scan ----
+//
+// A radial scan is shells of points s_j w_d (mod_build_grid.f90:306-326):
+// within one block of the scan, the shells after s = 0 keep the same
+// directions and are s_0 + j step apart.  Runs of consecutive shells with
+// equal direction lists and one common step become rays: P consecutive
+// shells of one direction are the P points of one lane, 64 such lanes a
+// wave tile.  Every other point (the s = 0 shell, shells left over at the
+// end of a run, runs of another step) goes to ordinary tiles of 64 P points
+// in grid order.  Both kinds are described by one index list, so the skip
+// mask and the kernel treat them alike.
+
+// Runs (first shell, shell count) and their common step, once per grid.
+int ray_analyse(inca_ctx *ctx) {
+  if (ctx->ray_serial == ctx->grid_serial) return 0;
+  ctx->ray_serial = ctx->grid_serial;
+  ctx->ray_h = 0.0;
+  ctx->ray_P = 0;
+  ctx->ray_runs.clear();
+  const int64_t G = ctx->G;
+  const int nsh = (int)ctx->radi.size();
+  if (ctx->grid_kind != GRID_RADIAL || G == 0 || G > INT32_MAX || nsh < 2) return 0;
+  std::vector<int64_t> off(nsh + 1, 0);
+  for (int j = 0; j < nsh; ++j) off[j + 1] = off[j] + ctx->smn[j];
+  if (off[nsh] != G) return 0;
+  std::vector<double> x(G), y(G), z(G);
+  INCA_HIP(hipMemcpyAsync(x.data(), ctx->d_gx, G * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  INCA_HIP(hipMemcpyAsync(y.data(), ctx->d_gy, G * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  INCA_HIP(hipMemcpyAsync(z.data(), ctx->d_gz, G * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  const std::vector<double> &r = ctx->radi;
+  // shells j and j+1 on the same directions: s_(j+1) x_j = s_j x_(j+1)
+  auto same_dirs = [&](int j) {
+    if (!(r[j] > 0.0) || !(r[j + 1] > r[j]) || ctx->smn[j] != ctx->smn[j + 1] ||
+        ctx->smn[j] == 0)
+      return false;
+    const double tol = 1e-13 * r[j] * r[j + 1];
+    for (int64_t d = 0; d < ctx->smn[j]; ++d) {
+      const int64_t a = off[j] + d, b = off[j + 1] + d;
+      if (std::fabs(r[j + 1] * x[a] - r[j] * x[b]) > tol ||
+          std::fabs(r[j + 1] * y[a] - r[j] * y[b]) > tol ||
+          std::fabs(r[j + 1] * z[a] - r[j] * z[b]) > tol)
+        return false;
+    }
+    return true;
+  };
+  struct Run {
+    int j0, len;
+    double h;
+  };
+  std::vector<Run> runs;
+  for (int j = 0; j + 1 < nsh;) {
+    if (!same_dirs(j)) {
+      ++j;
+      continue;
+    }
+    const double h = r[j + 1] - r[j];
+    int e = j + 1;  // last shell of the run
+    while (e + 1 < nsh && same_dirs(e) && std::fabs((r[e + 1] - r[e]) - h) <= 1e-12 * h) ++e;
+    runs.push_back(Run{j, e - j + 1, h});
+    j = e + 1;
+  }
+  if (runs.empty()) return 0;
+  // the step covering the most points; runs of other steps stay ordinary
+  double best_pts = -1.0, h_main = 0.0;
+  for (const Run &a : runs) {
+    double pts = 0.0;
+    for (const Run &b : runs)
+      if (std::fabs(b.h - a.h) <= 1e-12 * a.h) pts += (double)b.len * ctx->smn[b.j0];
+    if (pts > best_pts) {
+      best_pts = pts;
+      h_main = a.h;
+    }
+  }
+  for (const Run &a : runs)
+    if (std::fabs(a.h - h_main) <= 1e-12 * h_main) {
+      ctx->ray_runs.push_back(a.j0);
+      ctx->ray_runs.push_back(a.len);
+    }
+  ctx->ray_h = h_main;
+  return 0;
+}
+
+// Index lists for P points per ray lane: the ray tiles (P consecutive
+// shells of one direction per lane), then every other point in ordinary
+// tiles of 4 points per lane in grid order.
+constexpr int kGenP = 4;
+int ray_plan(inca_ctx *ctx, int P) {
+  int rc;
+  if ((rc = ray_analyse(ctx))) return rc;
+  if (ctx->ray_h <= 0.0 || P < 2) return 0;
+  if (ctx->ray_P == P) return 0;
+  const int64_t G = ctx->G;
+  const int nsh = (int)ctx->radi.size();
+  std::vector<int64_t> off(nsh + 1, 0);
+  for (int j = 0; j < nsh; ++j) off[j + 1] = off[j] + ctx->smn[j];
+  std::vector<uint8_t> used(G, 0);
+  std::vector<int32_t> lanes;  // ray lanes, P indices each
+  for (size_t i = 0; i < ctx->ray_runs.size(); i += 2) {
+    const int j0 = ctx->ray_runs[i], len = ctx->ray_runs[i + 1];
+    const int64_t nd = ctx->smn[j0];
+    for (int gs = 0; gs + P <= len; gs += P)
+      for (int64_t d = 0; d < nd; ++d)
+        for (int k = 0; k < P; ++k) {
+          const int64_t g = off[j0 + gs + k] + d;
+          lanes.push_back((int32_t)g);
+          used[g] = 1;
+        }
+  }
+  const int64_t nlanes = (int64_t)lanes.size() / P;
+  if (nlanes == 0) return 0;
+  const int64_t nray = (nlanes + 255) / 256 * 4;  // whole blocks of 4 waves
+  std::vector<int32_t> rest;
+  for (int64_t g = 0; g < G; ++g)
+    if (!used[g]) rest.push_back((int32_t)g);
+  const int64_t ngen = ((int64_t)rest.size() + 256 * kGenP - 1) / (256 * kGenP) * 4;
+  const size_t nray_idx = (size_t)nray * 64 * P;
+  std::vector<int32_t> idx(nray_idx + (size_t)ngen * 64 * kGenP, -1);
+  for (int64_t l = 0; l < nlanes; ++l) {
+    const int64_t t = l / 64, lane = l % 64;
+    for (int k = 0; k < P; ++k) idx[(t * P + k) * 64 + lane] = lanes[l * P + k];
+  }
+  for (size_t i = 0; i < rest.size(); ++i) idx[nray_idx + i] = rest[i];
+  if ((rc = ensure(ctx, ctx->d_ridx, ctx->ridx_cap, idx.size()))) return rc;
+  INCA_HIP(hipMemcpy(ctx->d_ridx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  ctx->ray_P = P;
+  ctx->ray_ntile = nray;
+  ctx->gen_ntile = ngen;
+  return 0;
+}
+
+// rho of the grid's ray step in word 7 of every prepared record.
+int ensure_rho(inca_ctx *ctx) {
+  int rc;
+  if ((rc = ray_analyse(ctx))) return rc;
+  if (ctx->ray_h <= 0.0 || ctx->q2 == 0 || ctx->rho_h == ctx->ray_h) return 0;
+  hipLaunchKernelGGL(k_ray_rho, dim3((unsigned)((ctx->q2 + kT - 1) / kT)), dim3(kT), 0,
+                     ctx->stream, ctx->d_q, ctx->qstride, ctx->q2, ctx->ray_h * ctx->ray_h);
+  INCA_HIP(hipGetLastError());
+  ctx->rho_h = ctx->ray_h;
+  return 0;
+}
+
+// Blocks of 4 wave tiles for one kernel with P points per lane, and whether
+// the tiles come from the ray index lists (FP64 kernel on a radial scan).
+int tile_layout(inca_ctx *ctx, bool mixed, int P, int64_t *tiles, bool *rays) {
+  *rays = false;
+  *tiles = (ctx->G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  if (mixed || P < 2 || ctx->grid_kind != GRID_RADIAL) return 0;
+  int rc;
+  if ((rc = ray_plan(ctx, P))) return rc;
+  if (ctx->ray_P != P) return 0;
+  *rays = true;
+  *tiles = (ctx->ray_ntile + ctx->gen_ntile) / 4;
+  return 0;
+}
+
+// Launch one evaluation over quartets [q, q+n) in `nch` chunks of `qch`
+// quartets with P points per lane; partial sums go to partial[chunk][g].
+// On a radial scan the FP64 evaluation runs as two kernels over the same
+// chunks: the ray tiles (k_eval_rays) and the other points (k_eval_fp64 on
+// their index list).
 int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, int64_t qch,
                 int64_t nch, double *partial) {
   const int64_t G = ctx->G;
-  const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  int64_t tiles;
+  bool rays;
+  int rc;
+  if ((rc = tile_layout(ctx, mixed, P, &tiles, &rays))) return rc;
   const int xcd = nch >= kXcds;
-  const int64_t nblocks = tiles * (xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch);
-  dim3 grid((unsigned)nblocks);
+  const int64_t cblocks = xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch;  // per tile block
   hipStream_t st = ctx->stream;
-  // bounds of the wave tiles of this P (a few microseconds; redone per launch)
   const bool cut = ctx->cut_log2 > 0.0;
   const int64_t ntile = tiles * (kT / 64);
-  int rc;
+  const int64_t nray = rays ? ctx->ray_ntile : 0, ngen = rays ? ctx->gen_ntile : 0;
+  const int32_t *ridx = ctx->d_ridx, *gidx = ctx->d_ridx + (size_t)nray * 64 * P;
   if ((rc = ensure(ctx, ctx->d_tb, ctx->tb_cap, (size_t)ntile * kTB))) return rc;
   if (cut) {
+    // bounds of the wave tiles (a few microseconds; redone per launch)
     const int mode = ctx->grid_kind == GRID_RADIAL ? 0 : (ctx->grid_kind == GRID_MULTI ? 1 : 2);
-    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
-                       ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
-                       ctx->d_tb);
+    if (rays) {
+      hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(nray / 4)), dim3(kT), 0, st, ctx->d_gx,
+                         ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, nray,
+                         ctx->d_tb, ridx);
+      if (ngen > 0)
+        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(ngen / 4)), dim3(kT), 0, st, ctx->d_gx,
+                           ctx->d_gy, ctx->d_gz, G, kGenP, mode, ctx->d_map, ctx->d_gcent, ngen,
+                           ctx->d_tb + nray * kTB, gidx);
+    } else {
+      hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
+                         ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
+                         ctx->d_tb, nullptr);
+    }
     const int64_t nwords = (n + 63) / 64;
     if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, (size_t)(nwords > 0 ? nwords : 1) * ntile)))
       return rc;
@@ -283,20 +526,33 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
                        ntile, ctx->d_tb);
   }
   INCA_HIP(hipGetLastError());
-#define INCA_EVAL2(PP, CC)                                                                     \
-  if (mixed)                                                                                   \
-    hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch, \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
-                       tiles, nch, xcd, partial);                                              \
-  else                                                                                         \
-    hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
-                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
-                       tiles, nch, xcd, partial);
+  const double h = ctx->ray_h;
+  const dim3 grid_all((unsigned)(tiles * cblocks)), grid_ray((unsigned)(nray / 4 * cblocks)),
+      grid_gen((unsigned)(ngen / 4 * cblocks));
+#define INCA_EVAL3(PP, CC)                                                                      \
+  if (mixed) {                                                                                  \
+    hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid_all, dim3(kT), 0, st, q, ctx->qstride, n,   \
+                       qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile, \
+                       tiles, nch, xcd, partial);                                               \
+  } else if (rays) {                                                                            \
+    if constexpr (PP >= 2)                                                                      \
+      hipLaunchKernelGGL((k_eval_rays<PP, CC>), grid_ray, dim3(kT), 0, st, q, ctx->qstride, n,  \
+                         qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask,       \
+                         ntile, nray / 4, nch, xcd, ridx, h, partial);                          \
+    if (ngen > 0)                                                                               \
+      hipLaunchKernelGGL((k_eval_fp64<kGenP, CC, true>), grid_gen, dim3(kT), 0, st, q,          \
+                         ctx->qstride, n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,   \
+                         ctx->d_mask, ntile, ngen / 4, nch, xcd, gidx, nray, partial);          \
+  } else {                                                                                      \
+    hipLaunchKernelGGL((k_eval_fp64<PP, CC, false>), grid_all, dim3(kT), 0, st, q,              \
+                       ctx->qstride, n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,     \
+                       ctx->d_mask, ntile, tiles, nch, xcd, nullptr, (int64_t)0, partial);      \
+  }
 #define INCA_EVAL(PP)      \
   if (cut) {               \
-    INCA_EVAL2(PP, true)   \
+    INCA_EVAL3(PP, true)   \
   } else {                 \
-    INCA_EVAL2(PP, false)  \
+    INCA_EVAL3(PP, false)  \
   }
   switch (P) {
     case 1: INCA_EVAL(1) break;
@@ -305,7 +561,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     default: INCA_EVAL(4) break;
   }
 #undef INCA_EVAL
-#undef INCA_EVAL2
+#undef INCA_EVAL3
   INCA_HIP(hipGetLastError());
   return 0;
 }
@@ -355,8 +611,10 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
   for (int r = 0; r < 2; ++r) {
     if (nr[r] == 0) continue;
-    const int P = ctx->tile_p[r];
-    const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+    int64_t tiles;
+    bool rays;
+    if ((rc = tile_layout(ctx, r == 1, ctx->tile_p[r], &tiles, &rays))) return rc;
+    if (rays && (rc = ensure_rho(ctx))) return rc;
     chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
   }
   const int64_t nchunks = nch[0] + nch[1];
@@ -398,19 +656,22 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   const int64_t n = nq < (1 << 20) ? nq : (1 << 20);
   if ((size_t)n * ctx->qstride > ctx->qtmp_cap)
     return set_err(ctx, INCA_EINVAL, "autotune needs prepared quartets");
+  int rc = ensure_rho(ctx);  // the sample carries word 7
+  if (rc) return rc;
   hipLaunchKernelGGL(k_gather_strided, dim3((unsigned)((n * ctx->qstride + kT - 1) / kT)), dim3(kT),
                      0, ctx->stream, ctx->d_q, nq, n, ctx->qstride, ctx->d_qtmp);
   INCA_HIP(hipGetLastError());
   hipEvent_t a, b;
   INCA_HIP(hipEventCreate(&a));
   INCA_HIP(hipEventCreate(&b));
-  int rc = 0;
   for (int kind = 0; kind < 2 && !rc; ++kind) {
     float best_ms = 1e30f;
     int best_p = 4;
     for (int c = 0; c < 4 && !rc; ++c) {
       const int P = cand[c];
-      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      int64_t tiles;
+      bool rays;
+      if ((rc = tile_layout(ctx, kind == 1, P, &tiles, &rays))) break;
       int64_t qch, nch;
       chunking(n, tiles, 1.0, &qch, &nch);
       if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nch * G)))) break;

@@ -113,7 +113,7 @@ __device__ __forceinline__ double exp_neg(double s, const double *__restrict__ t
 
 struct QHeader {
   double sqe, ax, ay, az, rcut;
-  int Lx, Ly, Lz;
+  int Lx, Ly, Lz, axes;  // axes: bit a set when axis a has L > 0
 };
 
 __device__ __forceinline__ QHeader load_header(const double *__restrict__ rec) {
@@ -127,6 +127,7 @@ __device__ __forceinline__ QHeader load_header(const double *__restrict__ rec) {
   h.Lx = cls & 0xff;
   h.Ly = (cls >> 8) & 0xff;
   h.Lz = (cls >> 16) & 0xff;
+  h.axes = (cls >> 24) & 7;
   return h;
 }
 
@@ -206,8 +207,7 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ rec, 
                                                 const double (&x)[P], const double (&y)[P],
                                                 const double (&z)[P], double (&acc)[P],
                                                 const double *__restrict__ tbl) {
-  const int mask = (h.Lx > 0) | ((h.Ly > 0) << 1) | ((h.Lz > 0) << 2);
-  switch (mask) {
+  switch (h.axes) {
     case 0: accumulate_case<P, false, false, false>(rec, h, x, y, z, acc, tbl); break;
     case 1: accumulate_case<P, true, false, false>(rec, h, x, y, z, acc, tbl); break;
     case 2: accumulate_case<P, false, true, false>(rec, h, x, y, z, acc, tbl); break;
@@ -216,6 +216,142 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ rec, 
     case 5: accumulate_case<P, true, false, true>(rec, h, x, y, z, acc, tbl); break;
     case 6: accumulate_case<P, false, true, true>(rec, h, x, y, z, acc, tbl); break;
     default: accumulate_case<P, true, true, true>(rec, h, x, y, z, acc, tbl); break;
+  }
+}
+
+// Ray tiles of a radial scan: the P points of a lane are r_k = o + k h w
+// (k < P) for the lane's first point o, its unit direction w and the scan's
+// step h, so along the lane
+//   rp_k = rp_0 + k d,  d = sqe h w,  s_k = |rp_k|^2,
+//   s_(k+1) - s_k = D_k,  D_0 = 2 d.rp_0 + |d|^2,  D_(k+1) - D_k = 2 |d|^2,
+// and exp(-s) follows the recurrence
+//   e_(k+1) = e_k q_k,  q_(k+1) = q_k rho,  q_0 = exp(-D_0),
+// rho = exp(-2 sqe^2 h^2) (record word 7, set for the scan's step): two
+// exponentials per lane and quartet and two products per further point, in
+// place of one exponential per point.  The points are the ideal ones,
+// o + k h w; the grid's own differ from them by the rounding of their
+// coordinates (~1e-15 bohr), which moves s by at most 2 |rp| sqe 1e-15.
+//
+// Range.  With s_0 <= 690 every factor stays a normal double while the
+// values matter: e_0 >= e^-690; q_0 <= e^690 (D_0 >= -s_0); s is convex in
+// k, so a product that underflows on the way (s_k > 708) is never followed
+// by a larger value.  A lane with s_0 > 690 has |rp_k| >= sqrt(690) -
+// (P - 1) sqe h, and the recurrence is used only when that is beyond the
+// quartet's cutoff radius (sqe (P - 1) h + rcut <= sqrt(690)): such a lane's
+// points are all below the cutoff's 2^-cut bound, and its recurrence, with
+// D_0 clamped at -700, yields values between 0 and the true ones.  Other
+// quartets (tight ones, or all of them when the cutoff is off) take
+// exp(-s_k) point by point at the same ideal points.
+constexpr double kRayMaxRp = 26.26;  // sqrt(690)
+
+// Horner in rp of one axis polynomial (degree L >= 1) at one point.
+__device__ __forceinline__ double horner1(const double *__restrict__ c, int L, double p) {
+  double v = fma(p, c[0], c[1]);
+  for (int q = 2; q <= L; ++q) v = fma(v, p, c[q]);
+  return v;
+}
+
+template <int P, bool MX, bool MY, bool MZ>
+__device__ __forceinline__ void ray_case(const double *__restrict__ rec, const QHeader &h,
+                                         const double (&o)[3], const double (&hw)[3],
+                                         double (&acc)[P], const double *__restrict__ tbl) {
+  static_assert(P >= 2, "a ray needs two points");
+  constexpr bool any = MX || MY || MZ;
+  const double *__restrict__ cx = rec + kQHead;
+  const double *__restrict__ cy = cx + h.Lx + 1;
+  const double *__restrict__ cz = cy + h.Ly + 1;
+  const double p0x = fma(h.sqe, o[0], h.ax), p0y = fma(h.sqe, o[1], h.ay),
+               p0z = fma(h.sqe, o[2], h.az);
+  const double dx = h.sqe * hw[0], dy = h.sqe * hw[1], dz = h.sqe * hw[2];
+  // constants of the degree-0 axes
+  double S = 1.0;
+  if (!MX) S = cx[0];
+  if (!MY) S = MX ? cy[0] : S * cy[0];
+  if (!MZ) S = (MX && MY) ? cz[0] : S * cz[0];
+  // product of the axis polynomials at the P points, one axis at a time
+  double w[P], p[P], v[P];
+  if (MX) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dx, p0x);
+    horner<P>(cx, h.Lx, p, w);
+  }
+  if (MY) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dy, p0y);
+    horner<P>(cy, h.Ly, p, v);
+#pragma unroll
+    for (int k = 0; k < P; ++k) w[k] = MX ? w[k] * v[k] : v[k];
+  }
+  if (MZ) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dz, p0z);
+    horner<P>(cz, h.Lz, p, v);
+#pragma unroll
+    for (int k = 0; k < P; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
+  }
+  const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
+  const double dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
+  const double dd = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double rho = rec[7];
+  double e = exp_neg(s0, tbl);
+  double qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
+  if (!(MX && MY && MZ)) e *= S;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    acc[k] = any ? fma(e, w[k], acc[k]) : acc[k] + e;
+    if (k + 1 < P) e *= qk;
+    if (k + 2 < P) qk *= rho;
+  }
+}
+
+// The same quartet point by point (exp(-s_k) at each ideal point), for the
+// quartets the recurrence does not take; one point's values live at a time.
+template <int P>
+__device__ __forceinline__ void ray_direct(const double *__restrict__ rec, const QHeader &h,
+                                        const double (&o)[3], const double (&hw)[3],
+                                        double (&acc)[P], const double *__restrict__ tbl) {
+  const double *__restrict__ c[3];
+  c[0] = rec + kQHead;
+  c[1] = c[0] + h.Lx + 1;
+  c[2] = c[1] + h.Ly + 1;
+  const int L[3] = {h.Lx, h.Ly, h.Lz};
+  const double a[3] = {h.ax, h.ay, h.az};
+  // a rolled loop over the points (the index is wave-uniform, so the
+  // accumulator is picked by a uniform branch)
+#pragma unroll 1
+  for (int k = 0; k < P; ++k) {
+    double pa[3], s = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      pa[ax] = fma(h.sqe, fma((double)k, hw[ax], o[ax]), a[ax]);
+      s = fma(pa[ax], pa[ax], s);
+    }
+    double f = exp_neg(s, tbl);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) f *= L[ax] > 0 ? horner1(c[ax], L[ax], pa[ax]) : c[ax][0];
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (j == k) acc[j] += f;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void ray_any(const double *__restrict__ rec, const QHeader &h,
+                                        const double (&o)[3], const double (&hw)[3], double span,
+                                        double (&acc)[P], const double *__restrict__ tbl) {
+  if (!(fma(h.sqe, span, h.rcut) <= kRayMaxRp)) {
+    ray_direct<P>(rec, h, o, hw, acc, tbl);
+    return;
+  }
+  switch (h.axes) {
+    case 0: ray_case<P, false, false, false>(rec, h, o, hw, acc, tbl); break;
+    case 1: ray_case<P, true, false, false>(rec, h, o, hw, acc, tbl); break;
+    case 2: ray_case<P, false, true, false>(rec, h, o, hw, acc, tbl); break;
+    case 3: ray_case<P, true, true, false>(rec, h, o, hw, acc, tbl); break;
+    case 4: ray_case<P, false, false, true>(rec, h, o, hw, acc, tbl); break;
+    case 5: ray_case<P, true, false, true>(rec, h, o, hw, acc, tbl); break;
+    case 6: ray_case<P, false, true, true>(rec, h, o, hw, acc, tbl); break;
+    default: ray_case<P, true, true, true>(rec, h, o, hw, acc, tbl); break;
   }
 }
 
@@ -243,11 +379,13 @@ __device__ __forceinline__ BlockTask block_task(int64_t tiles, int xcd) {
 namespace {
 
 // Tile bounds.  Wave w of block b owns the 64 P consecutive grid points
-// from (4 b + w) 64 P on.  For each such tile: an anchor c and the range
-// [rho_min, rho_max] of |r - c| over its points (words 0..4), and whether
-// it holds any point (word 5).  Anchors: the origin for radial scans (the
-// points lie on spheres about it), the owning centre for multicentre grids
-// when the whole tile belongs to one centre, else the tile's centroid.
+// from (4 b + w) 64 P on, or, with an index list, the points
+// idx[(t P + k) 64 + lane] (-1: none).  For each such tile: an anchor c and
+// the range [rho_min, rho_max] of |r - c| over its points (words 0..4) and
+// whether it holds any point (word 5).  Anchors: the origin for radial
+// scans (the points lie on spheres about it), the owning centre for
+// multicentre grids when the whole tile belongs to one centre, else the
+// tile's centroid.
 constexpr int kTB = 8;  // doubles per tile record
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -262,17 +400,23 @@ __global__ __launch_bounds__(kT) void k_tile_bounds(const double *__restrict__ g
                                                     int P, int mode,
                                                     const int32_t *__restrict__ map,
                                                     const double *__restrict__ gcent,
-                                                    int64_t ntile, double *__restrict__ tb) {
+                                                    int64_t ntile, double *__restrict__ tb,
+                                                    const int32_t *__restrict__ idx) {
   const int lane = threadIdx.x & 63;
   const int64_t t = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   if (t >= ntile) return;  // uniform per wave
   const int64_t g0 = t * 64 * P;
-  const int first = (mode == 1 && g0 < G) ? map[g0] : 0;
+  auto point = [&](int k) -> int64_t {
+    if (idx) return idx[(t * P + k) * 64 + lane];
+    const int64_t g = g0 + (int64_t)k * 64 + lane;
+    return g < G ? g : -1;
+  };
+  const int first = (mode == 1 && !idx && g0 < G) ? map[g0] : 0;
   double sx = 0.0, sy = 0.0, sz = 0.0, n = 0.0;
   int same = 1;
   for (int k = 0; k < P; ++k) {
-    const int64_t g = g0 + (int64_t)k * 64 + lane;
-    if (g < G) {
+    const int64_t g = point(k);
+    if (g >= 0) {
       sx += gx[g];
       sy += gy[g];
       sz += gz[g];
@@ -297,8 +441,8 @@ __global__ __launch_bounds__(kT) void k_tile_bounds(const double *__restrict__ g
   }
   double lo = HUGE_VAL, hi = 0.0;
   for (int k = 0; k < P; ++k) {
-    const int64_t g = g0 + (int64_t)k * 64 + lane;
-    if (g < G) {
+    const int64_t g = point(k);
+    if (g >= 0) {
       const double dx = gx[g] - cx, dy = gy[g] - cy, dz = gz[g] - cz;
       const double d = sqrt(dx * dx + dy * dy + dz * dz);
       lo = fmin(lo, d);

@@ -163,6 +163,7 @@ struct Scratch {
 };
 
 int alloc_grid(inca_ctx *ctx, int64_t G) {
+  ++ctx->grid_serial;
   double **arrs[5] = {&ctx->d_gx, &ctx->d_gy, &ctx->d_gz, &ctx->d_w,
                       &ctx->d_wvee};
   for (double **a : arrs) {

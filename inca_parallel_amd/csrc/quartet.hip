@@ -361,7 +361,11 @@ __global__ __launch_bounds__(kT) void k_prep(
         out[1] = sqe * (pik.P[0] - pjl.P[0]);
         out[2] = sqe * (pik.P[1] - pjl.P[1]);
         out[3] = sqe * (pik.P[2] - pjl.P[2]);
-        out[4] = __longlong_as_double((long long)(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16)));
+        // class word: L per axis in bits 0-23, the axes of nonzero degree
+        // (the evaluation's case) in bits 24-26
+        const int axes = (Ls[0] > 0) | ((Ls[1] > 0) << 1) | ((Ls[2] > 0) << 2);
+        out[4] = __longlong_as_double(
+            (long long)(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16) | (axes << 24)));
         out[5] = B;
         out[6] = cutoff_radius(out + kQHead, Ls, cut_s0);
         out[7] = 0.0;
@@ -520,6 +524,7 @@ int build_pair_table(inca_ctx *ctx) {
 
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
   hipStream_t st = ctx->stream;
+  ctx->rho_h = 0.0;  // word 7 of the new records holds no rho yet
   const int nprim = ctx->nprim;
   const int64_t nrec = ctx->nrec;
   int rc;

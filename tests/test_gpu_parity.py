@@ -308,6 +308,41 @@ def test_point_cutoff_matches_full_loop(engine, small_cluster):
         engine.set_point_cutoff(100)
 
 
+def test_ray_tiles_match_point_by_point(engine):
+    """Radial scans evaluated along rays (P >= 2 points per lane on one
+    direction, exp(-s) by the two-product recurrence) against the
+    point-by-point exponential (P = 1) on the same prepared quartets.  Two
+    scan blocks with different steps and Lebedev orders (one becomes ray
+    tiles, the other ordinary tiles), and exponents up to 2e4 so that tight
+    quartets take the point-by-point branch inside ray tiles.  The
+    recurrence follows the ideal ray points, which differ from the grid's by
+    the rounding of their coordinates."""
+    s = synth.config_cluster(seed=31, natoms=4, nheavy=3, nocc=4, exp_lo=0.05, exp_hi=2e4,
+                             shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
+    ijkl, d = s.dm2_unique()
+    p = s.prims()
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=ijkl, d=d)
+    tart = [[0.0, 3.0], [3.2, 6.0]]
+    engine.grid_radial_plot(tart, [0.1, 0.2], [110, 302])
+    out = {}
+    try:
+        for P in (1, 2, 4, 8):
+            engine.set_tiles(P, 4)
+            iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
+            out[P] = (iv.copy(), engine.radial_values().copy())
+    finally:
+        engine.set_tiles(4, 4)
+    for P in (2, 4, 8):
+        er, ea = _close(out[P][0], out[1][0])
+        rr, ra = _close(out[P][1], out[1][1])
+        print("P=%d rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e" % (
+            P, er, ea, rr, ra))
+        assert not np.array_equal(out[P][0], out[1][0])  # the ray arithmetic ran
+        assert er < 1e-11 and ea < 1e-13
+        assert rr < 1e-11 and ra < 1e-13
+
+
 def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     """Independent check of each prepared record's cutoff radius: at |rp| =
     rcut in random directions, |A e^-s Vx Vy Vz| (numpy, from the record's
