@@ -134,6 +134,140 @@ def batch_bench(capi, synth, torch, nmol=48, reps=3):
     }
 
 
+def target_bench(capi, torch, nmol=32):
+    """Radial intracule integrals of many small molecules (3-5 atoms, s/p/d,
+    the learned grid's held-out family): the plain FP64 GPU kernel (every
+    term at every point, no cutoff, one inca_evaluate per molecule) against
+    the engine with everything on (learned autogrid, mixed precision,
+    per-point cutoff, one batched hipGraph replay, sum-rule fallback to the
+    plain autogrid).  The plain kernel is timed on the plain autogrid (nrad
+    50 / nang 590 per centre, Bragg-Slater radii) and on the learned grid;
+    the engine's integrals are checked against the FP64 full loop on the
+    same grid."""
+    from inca_parallel_amd import learned_grid as LG
+    net = capi.MLP(seed=1)
+    net.set_params(np.load(LG.WEIGHTS, allow_pickle=False))
+    mols = []
+    for i in range(nmol):
+        s = LG.molecule(300000 + i)
+        p = s.prims()
+        recs = s.dm2_unique_torch(device="cuda")
+        mols.append(dict(s=s, p=p, recs=recs))
+    torch.cuda.synchronize()
+
+    def engine(m, cent, Ps, sf, nr, na):
+        p = m["p"]
+        e = capi.Engine(0)
+        e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+        e.grid_multicenter(cent, nr, na, sf, Ps, nosym=True)
+        e.set_quartets(device_ptr=m["recs"].data_ptr(), n=m["recs"].shape[0])
+        return e
+
+    def plain_grid(m):
+        p = m["p"]
+        cent, Ps, sf = capi.autogrid(p["an"], p["cartes"], True, False)
+        nq = len(Ps)
+        return cent, Ps, sf, [LG.PLAIN[0]] * nq, [LG.PLAIN[1]] * nq
+
+    def learned(m):
+        p = m["p"]
+        cent, Ps, sf, nr, na, _ = net.learned_autogrid(p["an"], p["cartes"], p["alpha"], p["tmn"],
+                                                       m["s"].ra, nohydro=True)
+        return cent, Ps, sf, list(nr), list(na)
+
+    # the plain FP64 kernel: every term at every point, one molecule at a time
+    t_plain = {"plain_grid": 0.0, "learned_grid": 0.0}
+    evals = {"plain_grid": 0.0, "learned_grid": 0.0}
+    for m in mols:
+        for key, grid in (("plain_grid", plain_grid(m)), ("learned_grid", learned(m))):
+            e = engine(m, *grid)
+            e.set_point_cutoff(0)
+            if m is mols[0]:
+                e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)  # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, cnt = e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+            tot = e.integrate()[0]
+            t_plain[key] += time.perf_counter() - t0
+            evals[key] += float(cnt[2]) * e.G
+            m[key] = dict(fp64=tot, analytic=e.analytic_integral(), points=e.G)
+            e.close()
+    # the engine: learned grid, mixed, cutoff, batched
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    engines = []
+    for m in mols:
+        e = engine(m, *learned(m))
+        e.prepare(capi.INCA_MIXED, THRESH)
+        engines.append(e)
+    t_prep = time.perf_counter() - t0
+    b = capi.Batch(engines)
+    t_create = time.perf_counter() - t0 - t_prep
+    t1 = time.perf_counter()
+    ti, _ = b.evaluate()
+    t_first = time.perf_counter() - t1
+    replay = []
+    for _ in range(3):
+        b.evaluate()
+        replay.append(b.info()["last_ms"] * 1e-3)
+    t_engine = t_prep + t_create + t_first
+    # the sum-rule fallback: the plain autogrid is evaluated (mixed, on the
+    # prepared quartets) for every molecule whose learned-grid integral is
+    # more than 1e-8 from the analytic integral
+    t1 = time.perf_counter()
+    final, used = [], []
+    for i, m in enumerate(mols):
+        ia = m["learned_grid"]["analytic"]
+        err_l = abs(ti[i] - ia) / abs(ia)
+        val, grid = ti[i], "learned_grid"
+        if err_l > LG.FALLBACK_TOL:
+            e = engines[i]
+            cent, Ps, sf, nr, na = plain_grid(m)
+            e.grid_multicenter(cent, nr, na, sf, Ps, nosym=True)
+            e.evaluate_prepared(capi.INCA_MIXED, want_ivec=False)
+            tp = e.integrate()[0]
+            if err_l > abs(tp - ia) / abs(ia) + LG.FALLBACK_TOL:
+                val, grid = tp, "plain_grid"
+        final.append(val)
+        used.append(grid)
+    t_fallback = time.perf_counter() - t1
+    for e in engines:
+        e.close()
+    b.close()
+    net.close()
+    # accuracy: the engine against the FP64 full loop on the grid it used
+    err_fast = []
+    sum_rule = {"plain_grid": [], "learned_grid": [], "engine": []}
+    for m, v, grid in zip(mols, final, used):
+        r = m[grid]["fp64"]
+        err_fast.append(abs(v - r) / abs(r))
+        for key in ("plain_grid", "learned_grid"):
+            sum_rule[key].append(abs(m[key]["fp64"] - m[key]["analytic"]) / abs(m[key]["analytic"]))
+        sum_rule["engine"].append(abs(v - m["learned_grid"]["analytic"]) /
+                                  abs(m["learned_grid"]["analytic"]))
+    return {
+        "molecules": nmol,
+        "plain_fp64_s": {k: round(v, 4) for k, v in t_plain.items()},
+        "plain_fp64_evals": evals,
+        "engine_s": t_engine,
+        "engine_parts_s": {"prepare_mixed": t_prep, "batch_create": t_create,
+                           "first_replay": t_first, "replay": float(np.mean(replay))},
+        "fallback_check_s": t_fallback,
+        "speedup_vs_plain_same_grid": t_plain["learned_grid"] / t_engine,
+        "speedup_vs_plain_same_grid_replay": t_plain["learned_grid"] / float(np.mean(replay)),
+        "speedup_vs_plain_with_fallback": t_plain["plain_grid"] / (t_engine + t_fallback),
+        "max_rel_err_engine_vs_fp64_same_grid": float(max(err_fast)),
+        "fallbacks": used.count("plain_grid"),
+        "median_sum_rule_err": {k: float(np.median(v)) for k, v in sum_rule.items()},
+        "note": "plain: inca_evaluate FP64 with the cutoff off (the reference's full loop) plus "
+                "the integral, one molecule at a time, on the learned grid (same grid) and on the "
+                "plain autogrid; engine: inca_prepare(INCA_MIXED) per molecule, batch create and "
+                "the first replay on the learned grid (replay: later replays); fallback check: "
+                "the plain autogrid evaluated for molecules whose learned-grid integral misses the "
+                "analytic integral by more than 1e-8",
+    }
+
+
 PROFILE = os.path.join(ROOT, "profiles", "eval_fp64_radial_plot_cluster151.md")
 
 
@@ -183,6 +317,8 @@ def main():
     ap.add_argument("--natoms", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cube", action="store_true")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the learned-grid + mixed + batched leg against the plain FP64 kernel")
     ap.add_argument("--tiles", default=None,
                     help="points per lane 'fp64,fp32' instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -317,6 +453,9 @@ def main():
     if not a.no_batch:
         out["batch"] = batch_bench(capi, synth, torch)
         log("batch: %s" % out["batch"])
+    if not a.no_target:
+        out["target"] = target_bench(capi, torch)
+        log("target: %s" % out["target"])
     if not a.no_cpu:
         rec_host = recs.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(sysm, rec_host, grid_r)

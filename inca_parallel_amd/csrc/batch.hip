@@ -80,7 +80,10 @@ struct inca_batch {
 namespace inca {
 namespace {
 
-constexpr int kItem = 7;  // int64 words per item: mol, tile, qoff, n, stride, partial, local q
+// int64 words per item: mol, tile, qoff, n, stride, partial, local q, and
+// the precision of its quartets (0: FP64, 1: the FP32 range of a molecule
+// prepared for INCA_MIXED)
+constexpr int kItem = 8;
 constexpr int kMol = 6;   // int64 words per molecule: g_off, G, part_off, nch, mask off, tiles
 constexpr int64_t kMaxChunk = 4096;  // quartets per work item at most
 
@@ -143,6 +146,7 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
     const int64_t it = queues[2 * qf] + pos;
     const int64_t *__restrict__ d = items + it * kItem;
     const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5], lq = d[6];
+    const bool f32 = d[7] != 0;
     const int stride = (int)d[4];
     const int64_t goff = mol[kMol * m], G = mol[kMol * m + 1];
     const int64_t moff = mol[kMol * m + 4], nt = mol[kMol * m + 5];
@@ -158,11 +162,17 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
       acc[k] = 0.0;
     }
     const double *__restrict__ q0 = q + qo - lq * stride;  // the molecule's first quartet
-    for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
-      const double *__restrict__ rec = q0 + l * stride;
-      const QHeader h = load_header(rec);
-      accumulate_fp64<P>(rec, h, x, y, z, acc, s_exp2);
-    });
+    if (f32) {
+      for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
+        const double *__restrict__ rec = q0 + l * stride;
+        accumulate_mixed_any<P>(rec, load_header(rec), x, y, z, acc);
+      });
+    } else {
+      for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
+        const double *__restrict__ rec = q0 + l * stride;
+        accumulate_fp64<P>(rec, load_header(rec), x, y, z, acc, s_exp2);
+      });
+    }
 #pragma unroll
     for (int k = 0; k < P; ++k) {
       const int64_t g = base + (int64_t)k * 64;
@@ -229,6 +239,13 @@ __global__ __launch_bounds__(kT) void k_batch_gather(const double *__restrict__ 
   dst[t] = src[perm[r] * stride + w];
 }
 
+// Class words (header word 4) of n records into a contiguous array.
+__global__ __launch_bounds__(kT) void k_class_words(const double *__restrict__ q, int64_t n,
+                                                    int stride, double *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r < n) out[r] = q[r * stride + 4];
+}
+
 __global__ __launch_bounds__(kT) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
   const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (t < n) p[t] = v;
@@ -273,15 +290,17 @@ int build_items(inca_batch *b, int P) {
     b->part_off[m] = part;
     int64_t c = 0;
     for (const auto &sg : b->segs[m]) {
-      const int lsum = (int)(sg[2] >> 24);
+      const int lsum = (int)((sg[2] >> 24) & 0xffff);
+      const int64_t f32 = sg[2] >> 40;
       for (int64_t s = 0; s < sg[1]; s += qchunk, ++c, ++chunk_id) {
         const int64_t n = std::min(qchunk, sg[1] - s);
         const int64_t qo = b->q_off[m] + (sg[0] + s) * b->stride[m];
         for (int64_t t = 0; t < tiles; ++t) {
           const int64_t pts = std::min(tile_pts, b->G[m] - t * tile_pts);
           It it;
-          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m], sg[0] + s};
-          it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts;
+          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m], sg[0] + s, f32};
+          // FP32 terms cost about half of the FP64 ones
+          it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts * (f32 ? 0.5 : 1.0);
           queue[chunk_id % kXcds].push_back(it);
         }
       }
@@ -471,6 +490,8 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   hipStream_t st = b->stream;
   int64_t *d_perm = nullptr;
   size_t perm_cap = 0;
+  double *d_cls = nullptr;  // class words of one molecule
+  size_t cls_cap = 0;
   for (int m = 0; m < nmol; ++m) {
     const inca_ctx *c = mols[m];
     const int64_t G = c->G, n = c->q2;
@@ -491,17 +512,23 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
       }
     }
     if (n == 0) continue;
-    // class of every prepared record (header word 4), sorted stably on the host
+    // class of every prepared record (header word 4), sorted stably on the
+    // host; the records of a mixed preparation keep their FP64 range ahead of
+    // the FP32 one (bit 40 of the key)
     std::vector<double> cls((size_t)n);
     B_HIP(hipStreamSynchronize(c->stream));
-    B_HIP(hipMemcpy2D(cls.data(), sizeof(double), c->d_q + 4, c->qstride * sizeof(double),
-                      sizeof(double), (size_t)n, hipMemcpyDeviceToHost));
+    if ((rc = bensure(b, d_cls, cls_cap, (size_t)n))) return rc;
+    hipLaunchKernelGGL(k_class_words, dim3(nblk(n)), dim3(kT), 0, st, c->d_q, n, c->qstride, d_cls);
+    B_HIP(hipGetLastError());
+    B_HIP(hipMemcpyAsync(cls.data(), d_cls, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    B_HIP(hipStreamSynchronize(st));
     std::vector<int64_t> key((size_t)n), perm((size_t)n);
     for (int64_t r = 0; r < n; ++r) {
       uint64_t bits;
       std::memcpy(&bits, &cls[r], sizeof(bits));
       const int64_t lx = bits & 0xff, ly = (bits >> 8) & 0xff, lz = (bits >> 16) & 0xff;
-      key[r] = ((lx + ly + lz) << 24) | (int64_t)(bits & 0xffffff);
+      const int64_t f32 = (c->prep_precision == INCA_MIXED && r >= c->q_fp64) ? 1 : 0;
+      key[r] = (f32 << 40) | ((lx + ly + lz) << 24) | (int64_t)(bits & 0xffffff);
     }
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t c2) { return key[a] < key[c2]; });
@@ -519,6 +546,7 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     B_HIP(hipStreamSynchronize(st));  // d_perm is reused by the next molecule
   }
   if (d_perm) (void)hipFree(d_perm);
+  if (d_cls) (void)hipFree(d_cls);
   B_HIP(hipStreamSynchronize(st));
   if ((rc = persistent_blocks(b, b->P))) return rc;
   if ((rc = build_items(b, b->P))) return rc;

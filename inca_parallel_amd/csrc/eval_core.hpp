@@ -355,6 +355,74 @@ __device__ __forceinline__ void ray_any(const double *__restrict__ rec, const QH
   }
 }
 
+// Mixed precision: rp in FP64 (sqe r + sqe d cancels), everything after it
+// in FP32 -- the hardware exp2 and FP32 Horner -- and each term added to an
+// FP64 accumulator.  The body is specialised on the axes of nonzero degree
+// like the FP64 kernel's.
+template <int P, bool MX, bool MY, bool MZ>
+__device__ __forceinline__ void accumulate_mixed(const double *__restrict__ rec, const QHeader &h,
+                                                 const double (&x)[P], const double (&y)[P],
+                                                 const double (&z)[P], double (&acc)[P]) {
+  // -log2(e) split so that s*log2(e) is formed to FP32 rounding without the
+  // 1.3e-8 relative bias of a single-float constant
+  const float kNL2eHi = -1.44269502162933349609375f;
+  const float kNL2eLo = -1.925963033500011e-08f;
+  const double *__restrict__ cx = rec + kQHead;
+  const double *__restrict__ cy = cx + h.Lx + 1;
+  const double *__restrict__ cz = cy + h.Ly + 1;
+  float px[P], py[P], pz[P], vx[P], vy[P], vz[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    px[k] = (float)fma(h.sqe, x[k], h.ax);
+    py[k] = (float)fma(h.sqe, y[k], h.ay);
+    pz[k] = (float)fma(h.sqe, z[k], h.az);
+  }
+  auto horner32 = [&](const double *__restrict__ c, int L, const float(&p)[P], float(&v)[P]) {
+    const float c0 = (float)c[0], c1 = (float)c[1];
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = fmaf(p[k], c0, c1);
+    for (int j = 2; j <= L; ++j) {
+      const float cj = (float)c[j];
+#pragma unroll
+      for (int k = 0; k < P; ++k) v[k] = fmaf(v[k], p[k], cj);
+    }
+  };
+  if (MX) horner32(cx, h.Lx, px, vx);
+  if (MY) horner32(cy, h.Ly, py, vy);
+  if (MZ) horner32(cz, h.Lz, pz, vz);
+  float S = 1.0f;
+  if (!MX) S = (float)cx[0];
+  if (!MY) S = MX ? (float)cy[0] : S * (float)cy[0];
+  if (!MZ) S = (MX && MY) ? (float)cz[0] : S * (float)cz[0];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const float s = fmaf(px[k], px[k], fmaf(py[k], py[k], pz[k] * pz[k]));
+    float f = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo));
+    if (!(MX && MY && MZ)) f *= S;
+    if (MX) f *= vx[k];
+    if (MY) f *= vy[k];
+    if (MZ) f *= vz[k];
+    acc[k] += (double)f;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void accumulate_mixed_any(const double *__restrict__ rec,
+                                                     const QHeader &h, const double (&x)[P],
+                                                     const double (&y)[P], const double (&z)[P],
+                                                     double (&acc)[P]) {
+  switch (h.axes) {
+    case 0: accumulate_mixed<P, false, false, false>(rec, h, x, y, z, acc); break;
+    case 1: accumulate_mixed<P, true, false, false>(rec, h, x, y, z, acc); break;
+    case 2: accumulate_mixed<P, false, true, false>(rec, h, x, y, z, acc); break;
+    case 3: accumulate_mixed<P, true, true, false>(rec, h, x, y, z, acc); break;
+    case 4: accumulate_mixed<P, false, false, true>(rec, h, x, y, z, acc); break;
+    case 5: accumulate_mixed<P, true, false, true>(rec, h, x, y, z, acc); break;
+    case 6: accumulate_mixed<P, false, true, true>(rec, h, x, y, z, acc); break;
+    default: accumulate_mixed<P, true, true, true>(rec, h, x, y, z, acc); break;
+  }
+}
+
 // XCD-aware block order.  The device dispatches workgroups to its 8 XCDs
 // round-robin by linear index, and each XCD has its own L2.  With `xcd` set,
 // block L runs on XCD L mod 8 and the blocks of XCD x take the chunks

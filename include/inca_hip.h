@@ -161,8 +161,11 @@ int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes,
  * point tile, single-class chunk) are ordered largest first and consumed by
  * a persistent kernel from an atomic work counter; the whole evaluation is
  * captured into a hipGraph at creation and replayed by inca_batch_evaluate.
- * FP64 only.  On failure *out may still be set: read inca_batch_last_error
- * and call inca_batch_destroy. */
+ * Each molecule runs in the precision its quartets were prepared for:
+ * INCA_FP64, or INCA_MIXED (its FP64 range with the FP64 body, its FP32
+ * range with the mixed one; the FP64 and FP32 ranges are sorted
+ * separately).  On failure *out may still be set: read
+ * inca_batch_last_error and call inca_batch_destroy. */
 int inca_batch_create(int32_t device, int32_t nmol, inca_ctx *const *mols,
                       inca_batch **out);
 int inca_batch_destroy(inca_batch *b);

@@ -100,6 +100,41 @@ def test_batch_autotune_recaptures(molecules):
         b.close()
 
 
+def test_batch_mixed_matches_unbatched_mixed(molecules):
+    """Molecules prepared for INCA_MIXED: the batch evaluates each one's FP64
+    range with the FP64 body and its FP32 range with the mixed body, so it
+    must match the molecule's own mixed evaluation apart from the FP64
+    summation order, and the FP64 result within the mixed tolerance."""
+    ref = []
+    try:
+        for mol in molecules:
+            e = mol["e"]
+            e.set_mixed_tail(0.05)
+            iv, _ = e.evaluate(capi.INCA_MIXED, THRESH)
+            n64, n32 = e.mixed_split()[:2]
+            assert n64 > 0 and n32 > 0, (n64, n32)
+            ref.append((iv, e.integrate()[0]))
+        b = capi.Batch([m["e"] for m in molecules])
+        try:
+            ti, _ = b.evaluate()
+            worst = worst64 = 0.0
+            for m, mol in enumerate(molecules):
+                iv = b.ivec(m)
+                err = np.abs(iv - ref[m][0]).max() / np.abs(ref[m][0]).max()
+                worst = max(worst, err)
+                assert err < 1e-13, (m, err)
+                assert abs(ti[m] - ref[m][1]) <= 1e-13 * abs(ref[m][1])
+                e64 = abs(ti[m] - mol["tot"][0]) / abs(mol["tot"][0])
+                worst64 = max(worst64, e64)
+                assert e64 < 1e-8, (m, e64)
+            print("mixed batch vs unbatched mixed %.2e; integral vs FP64 %.2e" % (worst, worst64))
+        finally:
+            b.close()
+    finally:
+        for mol in molecules:
+            mol["e"].set_mixed_tail(0.02)
+
+
 def test_batch_outlives_its_contexts():
     """The batch owns copies: destroying the contexts leaves it usable."""
     s = _molecule(1)
