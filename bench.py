@@ -272,11 +272,12 @@ PROFILE = os.path.join(ROOT, "profiles", "eval_fp64_radial_plot_cluster151.md")
 
 
 def profiled_traffic():
-    """HBM bytes per k_eval_fp64 launch on this workload, from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy)."""
+    """HBM bytes of one FP64 step's grid loop on this workload (the ray-tile
+    and other-points kernels), from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy)."""
     try:
         for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch k_eval_fp64"):
+            if line.startswith("traffic_bytes_per_fp64_step_eval"):
                 return float(line.split("=")[1])
     except OSError:
         pass
@@ -288,7 +289,8 @@ def cube_bench(capi, eng):
     bohr, centred at the origin (the cubeintra grid).  One FP64 and one mixed
     grid loop over the prepared quartets, with the mixed error per voxel."""
     n = eng.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
-    out = {"points": n}
+    eng.prepare(capi.INCA_FP64, THRESH)
+    out = {"points": n, "points_per_lane": eng.autotune()}  # tiles for this grid
     for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
         eng.prepare(prec, THRESH)
         t0 = time.perf_counter()
@@ -425,13 +427,15 @@ def main():
             "points_per_lane": {"fp64": tiles[0], "fp32": tiles[1]},
         },
         "roofline": {
-            "bound": "valu", "kernel": "k_eval_fp64",
+            "bound": "valu", "kernel": "k_eval_rays + k_eval_fp64 (ray tiles, other points)",
             "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU, "traffic": profiled_traffic(),
-            "note": "FLOP = 16 + 2*Lsum per executed eval (FMA = 2), exp not counted; eval "
-                    "kernel %.1f ms per launch (HIP events on the engine stream); traffic = HBM "
-                    "bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s); the kernel is "
-                    "bound by FP64 VALU issue" % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
+            "note": "FLOP = 16 + 2*Lsum per executed eval (FMA = 2), exp not counted; grid loop "
+                    "%.1f ms per step (HIP events on the engine stream around the skip mask, the "
+                    "ray-tile kernel and the other-points kernel); traffic = HBM bytes of those "
+                    "kernels per step from rocprofv3 FETCH_SIZE + WRITE_SIZE at 4 points per lane "
+                    "(%s); the ray kernel is bound by FP64 VALU issue (0.82 busy)"
+                    % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
         "mixed": {
             "value": evals / rm["step_s"], "unit": "evals/s",

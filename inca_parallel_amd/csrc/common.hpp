@@ -72,10 +72,13 @@ struct inca_ctx {
   int64_t ray_serial = -1;    // grid the ray analysis belongs to
   double ray_h = 0.0;         // radial step of the ray runs (0: no rays)
   std::vector<int32_t> ray_runs;  // (first shell, shell count) of each ray run
+  std::vector<int32_t> dup_of;    // per point: an earlier point at the same place, or -1
+  int32_t *d_dup = nullptr;       // (point, its copy source) pairs not evaluated
+  size_t dup_cap = 0;
+  int64_t ndup = 0;
   int ray_P = 0;              // points per lane of the ray tiles (0: no list)
   int64_t ray_ntile = 0;      // ray wave tiles (multiple of 4)
-  int64_t gen_ntile = 0;      // wave tiles of the other points, 4 per lane (multiple of 4)
-  int32_t *d_ridx = nullptr;  // [tile][k][lane] grid index (-1: none): ray tiles, then the others
+  int32_t *d_ridx = nullptr;  // [tile][k][lane] grid index of the ray tiles (-1: none)
   size_t ridx_cap = 0;
   double rho_h = 0.0;         // step whose rho the prepared records hold in word 7
 

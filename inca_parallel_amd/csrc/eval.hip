@@ -18,7 +18,9 @@
 // an exponent shift (eval_core.hpp).  Mixed path: rp in FP64 (the cancellation-prone step),
 // everything after it in FP32 -- hardware exp2, FP32 Horner, FP32 products
 // -- and every term added to an FP64 accumulator.
+#include <array>
 #include <cmath>
+#include <map>
 
 #include "common.hpp"
 #include "eval_core.hpp"
@@ -26,10 +28,7 @@
 namespace inca {
 namespace {
 
-// IDX: the tile's points come from the index list idx[t][k][lane] (-1:
-// none) of local tile t, whose tile words and mask columns are those of
-// tile tile0 + t.
-template <int P, bool CUT, bool IDX>
+template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
@@ -38,26 +37,20 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
                                                   const double *__restrict__ tb,
                                                   const unsigned long long *__restrict__ mask,
                                                   int64_t ntile, int64_t tiles, int64_t nch,
-                                                  int xcd, const int32_t *__restrict__ idx,
-                                                  int64_t tile0, double *__restrict__ partial) {
+                                                  int xcd, double *__restrict__ partial) {
   const BlockTask task = block_task(tiles, xcd);
   if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
-  const int64_t lt = task.tile_block * (kT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t tile = tile0 + lt;
-  const int64_t base = lt * (64 * P) + (threadIdx.x & 63);
+  const int64_t tile = task.tile_block * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
   const bool empty = !(tb[tile * kTB + 5] > 0.0);
   __shared__ double s_exp2[kExpTable];
   stage_exp_table(s_exp2);
-  auto point = [&](int k) -> int64_t {
-    if (IDX) return idx[base + (int64_t)k * 64];
-    const int64_t g = base + (int64_t)k * 64;
-    return g < G ? g : -1;
-  };
   double x[P], y[P], z[P], acc[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = point(k);
-    const bool ok = g >= 0;
+    const int64_t g = base + (int64_t)k * 64;
+    const bool ok = g < G;
     x[k] = ok ? gx[g] : 0.0;
     y[k] = ok ? gy[g] : 0.0;
     z[k] = ok ? gz[g] : 0.0;
@@ -71,15 +64,15 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    const int64_t g = point(k);
-    if (g >= 0) partial[task.chunk * G + g] = acc[k];
+    const int64_t g = base + (int64_t)k * 64;
+    if (g < G) partial[task.chunk * G + g] = acc[k];
   }
 }
 
-// Ray tiles of a radial scan (tiles 0 .. of the launch): lane `lane` of tile
-// t holds the P points idx[t][k][lane], k < P, one direction at P
-// consecutive radii a step h apart; the lane keeps its first point o and
-// h w, and the quartets are evaluated by ray_case.
+// Ray tiles of a radial scan: lane `lane` of tile t holds the points
+// idx[t][k][lane], k < P (-1: none), one direction at consecutive radii a
+// step h apart from its first point o (or o alone); the lane keeps o and
+// h w, and the quartets are evaluated by ray_any.
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
@@ -185,6 +178,13 @@ __global__ __launch_bounds__(kT) void k_reduce(const double *__restrict__ partia
   ivec[g] = s;
 }
 
+// Ivec of the points evaluated through another point at the same place.
+__global__ __launch_bounds__(kT) void k_copy_dups(const int32_t *__restrict__ dup, int64_t n,
+                                                  double *__restrict__ ivec) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) ivec[dup[2 * i]] = ivec[dup[2 * i + 1]];
+}
+
 __global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q, int qstride,
                                                    int64_t nq, int nbins,
                                                    unsigned long long *__restrict__ hist) {
@@ -265,6 +265,8 @@ int ray_analyse(inca_ctx *ctx) {
   ctx->ray_h = 0.0;
   ctx->ray_P = 0;
   ctx->ray_runs.clear();
+  ctx->dup_of.clear();
+  ctx->ndup = 0;
   const int64_t G = ctx->G;
   const int nsh = (int)ctx->radi.size();
   if (ctx->grid_kind != GRID_RADIAL || G == 0 || G > INT32_MAX || nsh < 2) return 0;
@@ -279,6 +281,16 @@ int ray_analyse(inca_ctx *ctx) {
   INCA_HIP(hipMemcpyAsync(z.data(), ctx->d_gz, G * sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
   INCA_HIP(hipStreamSynchronize(ctx->stream));
+  // points at the same place as an earlier one (the s = 0 shell holds one
+  // point per direction, all at the origin) are evaluated once
+  ctx->dup_of.assign(G, -1);
+  {
+    std::map<std::array<double, 3>, int32_t> seen;
+    for (int64_t g = 0; g < G; ++g) {
+      auto it = seen.emplace(std::array<double, 3>{x[g], y[g], z[g]}, (int32_t)g);
+      if (!it.second) ctx->dup_of[g] = it.first->second;
+    }
+  }
   const std::vector<double> &r = ctx->radi;
   // shells j and j+1 on the same directions: s_(j+1) x_j = s_j x_(j+1)
   auto same_dirs = [&](int j) {
@@ -332,10 +344,11 @@ int ray_analyse(inca_ctx *ctx) {
   return 0;
 }
 
-// Index lists for P points per ray lane: the ray tiles (P consecutive
-// shells of one direction per lane), then every other point in ordinary
-// tiles of 4 points per lane in grid order.
-constexpr int kGenP = 4;
+// Index list for P points per ray lane.  A run of shells gives each of its
+// directions lanes of P consecutive shells (the last lane of a run may hold
+// fewer); every other point is a lane of its own (its first slot), and a
+// point at the same place as an earlier one is not evaluated but copied
+// after the reduction.  Unused slots hold -1.
 int ray_plan(inca_ctx *ctx, int P) {
   int rc;
   if ((rc = ray_analyse(ctx))) return rc;
@@ -346,37 +359,51 @@ int ray_plan(inca_ctx *ctx, int P) {
   std::vector<int64_t> off(nsh + 1, 0);
   for (int j = 0; j < nsh; ++j) off[j + 1] = off[j] + ctx->smn[j];
   std::vector<uint8_t> used(G, 0);
-  std::vector<int32_t> lanes;  // ray lanes, P indices each
+  std::vector<int32_t> lanes;  // P slots per lane
   for (size_t i = 0; i < ctx->ray_runs.size(); i += 2) {
     const int j0 = ctx->ray_runs[i], len = ctx->ray_runs[i + 1];
     const int64_t nd = ctx->smn[j0];
-    for (int gs = 0; gs + P <= len; gs += P)
+    for (int gs = 0; gs < len; gs += P)
       for (int64_t d = 0; d < nd; ++d)
         for (int k = 0; k < P; ++k) {
-          const int64_t g = off[j0 + gs + k] + d;
-          lanes.push_back((int32_t)g);
-          used[g] = 1;
+          if (gs + k < len) {
+            const int64_t g = off[j0 + gs + k] + d;
+            lanes.push_back((int32_t)g);
+            used[g] = 1;
+          } else {
+            lanes.push_back(-1);
+          }
         }
+  }
+  std::vector<int32_t> dups;  // (copy, source) pairs
+  for (int64_t g = 0; g < G; ++g) {
+    if (used[g]) continue;
+    if (ctx->dup_of[g] >= 0) {
+      dups.push_back((int32_t)g);
+      dups.push_back(ctx->dup_of[g]);
+      continue;
+    }
+    lanes.push_back((int32_t)g);
+    for (int k = 1; k < P; ++k) lanes.push_back(-1);
   }
   const int64_t nlanes = (int64_t)lanes.size() / P;
   if (nlanes == 0) return 0;
   const int64_t nray = (nlanes + 255) / 256 * 4;  // whole blocks of 4 waves
-  std::vector<int32_t> rest;
-  for (int64_t g = 0; g < G; ++g)
-    if (!used[g]) rest.push_back((int32_t)g);
-  const int64_t ngen = ((int64_t)rest.size() + 256 * kGenP - 1) / (256 * kGenP) * 4;
-  const size_t nray_idx = (size_t)nray * 64 * P;
-  std::vector<int32_t> idx(nray_idx + (size_t)ngen * 64 * kGenP, -1);
+  std::vector<int32_t> idx((size_t)nray * 64 * P, -1);
   for (int64_t l = 0; l < nlanes; ++l) {
     const int64_t t = l / 64, lane = l % 64;
     for (int k = 0; k < P; ++k) idx[(t * P + k) * 64 + lane] = lanes[l * P + k];
   }
-  for (size_t i = 0; i < rest.size(); ++i) idx[nray_idx + i] = rest[i];
   if ((rc = ensure(ctx, ctx->d_ridx, ctx->ridx_cap, idx.size()))) return rc;
   INCA_HIP(hipMemcpy(ctx->d_ridx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  ctx->ndup = (int64_t)dups.size() / 2;
+  if (ctx->ndup > 0) {
+    if ((rc = ensure(ctx, ctx->d_dup, ctx->dup_cap, dups.size()))) return rc;
+    INCA_HIP(hipMemcpy(ctx->d_dup, dups.data(), dups.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  }
   ctx->ray_P = P;
   ctx->ray_ntile = nray;
-  ctx->gen_ntile = ngen;
   return 0;
 }
 
@@ -402,15 +429,14 @@ int tile_layout(inca_ctx *ctx, bool mixed, int P, int64_t *tiles, bool *rays) {
   if ((rc = ray_plan(ctx, P))) return rc;
   if (ctx->ray_P != P) return 0;
   *rays = true;
-  *tiles = (ctx->ray_ntile + ctx->gen_ntile) / 4;
+  *tiles = ctx->ray_ntile / 4;
   return 0;
 }
 
 // Launch one evaluation over quartets [q, q+n) in `nch` chunks of `qch`
 // quartets with P points per lane; partial sums go to partial[chunk][g].
-// On a radial scan the FP64 evaluation runs as two kernels over the same
-// chunks: the ray tiles (k_eval_rays) and the other points (k_eval_fp64 on
-// their index list).
+// The FP64 evaluation of a radial scan runs on its ray tiles
+// (k_eval_rays).
 int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, int64_t qch,
                 int64_t nch, double *partial) {
   const int64_t G = ctx->G;
@@ -418,30 +444,20 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   bool rays;
   int rc;
   if ((rc = tile_layout(ctx, mixed, P, &tiles, &rays))) return rc;
+  const int32_t *ridx = rays ? ctx->d_ridx : nullptr;
   const int xcd = nch >= kXcds;
-  const int64_t cblocks = xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch;  // per tile block
+  const int64_t nblocks = tiles * (xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch);
+  dim3 grid((unsigned)nblocks);
   hipStream_t st = ctx->stream;
+  // bounds of the wave tiles of this P (a few microseconds; redone per launch)
   const bool cut = ctx->cut_log2 > 0.0;
   const int64_t ntile = tiles * (kT / 64);
-  const int64_t nray = rays ? ctx->ray_ntile : 0, ngen = rays ? ctx->gen_ntile : 0;
-  const int32_t *ridx = ctx->d_ridx, *gidx = ctx->d_ridx + (size_t)nray * 64 * P;
   if ((rc = ensure(ctx, ctx->d_tb, ctx->tb_cap, (size_t)ntile * kTB))) return rc;
   if (cut) {
-    // bounds of the wave tiles (a few microseconds; redone per launch)
     const int mode = ctx->grid_kind == GRID_RADIAL ? 0 : (ctx->grid_kind == GRID_MULTI ? 1 : 2);
-    if (rays) {
-      hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(nray / 4)), dim3(kT), 0, st, ctx->d_gx,
-                         ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, nray,
-                         ctx->d_tb, ridx);
-      if (ngen > 0)
-        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)(ngen / 4)), dim3(kT), 0, st, ctx->d_gx,
-                           ctx->d_gy, ctx->d_gz, G, kGenP, mode, ctx->d_map, ctx->d_gcent, ngen,
-                           ctx->d_tb + nray * kTB, gidx);
-    } else {
-      hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
-                         ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
-                         ctx->d_tb, nullptr);
-    }
+    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
+                       ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
+                       ctx->d_tb, ridx);
     const int64_t nwords = (n + 63) / 64;
     if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, (size_t)(nwords > 0 ? nwords : 1) * ntile)))
       return rc;
@@ -459,32 +475,26 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   }
   INCA_HIP(hipGetLastError());
   const double h = ctx->ray_h;
-  const dim3 grid_all((unsigned)(tiles * cblocks)), grid_ray((unsigned)(nray / 4 * cblocks)),
-      grid_gen((unsigned)(ngen / 4 * cblocks));
-#define INCA_EVAL3(PP, CC)                                                                      \
-  if (mixed) {                                                                                  \
-    hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid_all, dim3(kT), 0, st, q, ctx->qstride, n,   \
-                       qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile, \
-                       tiles, nch, xcd, partial);                                               \
-  } else if (rays) {                                                                            \
-    if constexpr (PP >= 2)                                                                      \
-      hipLaunchKernelGGL((k_eval_rays<PP, CC>), grid_ray, dim3(kT), 0, st, q, ctx->qstride, n,  \
-                         qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask,       \
-                         ntile, nray / 4, nch, xcd, ridx, h, partial);                          \
-    if (ngen > 0)                                                                               \
-      hipLaunchKernelGGL((k_eval_fp64<kGenP, CC, true>), grid_gen, dim3(kT), 0, st, q,          \
-                         ctx->qstride, n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,   \
-                         ctx->d_mask, ntile, ngen / 4, nch, xcd, gidx, nray, partial);          \
-  } else {                                                                                      \
-    hipLaunchKernelGGL((k_eval_fp64<PP, CC, false>), grid_all, dim3(kT), 0, st, q,              \
-                       ctx->qstride, n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,     \
-                       ctx->d_mask, ntile, tiles, nch, xcd, nullptr, (int64_t)0, partial);      \
+#define INCA_EVAL2(PP, CC)                                                                     \
+  if (mixed) {                                                                                 \
+    hipLaunchKernelGGL((k_eval_mixed<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch, \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
+                       tiles, nch, xcd, partial);                                              \
+  } else if (rays) {                                                                           \
+    if constexpr (PP >= 2)                                                                     \
+      hipLaunchKernelGGL((k_eval_rays<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n,     \
+                         qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask,      \
+                         ntile, tiles, nch, xcd, ridx, h, partial);                            \
+  } else {                                                                                     \
+    hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
+                       tiles, nch, xcd, partial);                                              \
   }
 #define INCA_EVAL(PP)      \
   if (cut) {               \
-    INCA_EVAL3(PP, true)   \
+    INCA_EVAL2(PP, true)   \
   } else {                 \
-    INCA_EVAL3(PP, false)  \
+    INCA_EVAL2(PP, false)  \
   }
   switch (P) {
     case 1: INCA_EVAL(1) break;
@@ -493,7 +503,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     default: INCA_EVAL(4) break;
   }
 #undef INCA_EVAL
-#undef INCA_EVAL3
+#undef INCA_EVAL2
   INCA_HIP(hipGetLastError());
   return 0;
 }
@@ -541,12 +551,14 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
   const int64_t nr[2] = {n64, nq - n64};
   int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
+  bool rays64 = false;  // the FP64 range runs on the ray lists (duplicates skipped)
   for (int r = 0; r < 2; ++r) {
     if (nr[r] == 0) continue;
     int64_t tiles;
     bool rays;
     if ((rc = tile_layout(ctx, r == 1, ctx->tile_p[r], &tiles, &rays))) return rc;
     if (rays && (rc = ensure_rho(ctx))) return rc;
+    if (r == 0) rays64 = rays;
     chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
   }
   const int64_t nchunks = nch[0] + nch[1];
@@ -562,6 +574,9 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
                      ctx->d_partial, (int)nchunks, G, ctx->d_ivec);
+  if (rays64 && ctx->ndup > 0)
+    hipLaunchKernelGGL(k_copy_dups, dim3((unsigned)((ctx->ndup + kT - 1) / kT)), dim3(kT), 0, st,
+                       ctx->d_dup, ctx->ndup, ctx->d_ivec);
   INCA_HIP(hipGetLastError());
   INCA_HIP(hipEventRecord(ctx->ev[3], st));
   ctx->have_ivec = 1;
@@ -584,8 +599,12 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   const int cand[4] = {1, 2, 4, 8};
   // an evenly strided sample of the prepared quartets (file order groups
   // quartets by primitive, and with the cutoff their cost depends on where
-  // they sit), gathered into the pre-pass scratch, which is free after it
-  const int64_t n = nq < (1 << 20) ? nq : (1 << 20);
+  // they sit), gathered into the pre-pass scratch, which is free after it;
+  // about 2^36 quartet x point evaluations per timed launch (4096 to 2^22
+  // quartets), enough blocks for the launch to run at its steady occupancy
+  int64_t n = ((int64_t)1 << 36) / (G > 0 ? G : 1);
+  n = n < 4096 ? 4096 : (n > (1 << 22) ? (1 << 22) : n);
+  if (n > nq) n = nq;
   if ((size_t)n * ctx->qstride > ctx->qtmp_cap)
     return set_err(ctx, INCA_EINVAL, "autotune needs prepared quartets");
   int rc = ensure_rho(ctx);  // the sample carries word 7
@@ -604,16 +623,21 @@ int autotune(inca_ctx *ctx, int32_t *best) {
       int64_t tiles;
       bool rays;
       if ((rc = tile_layout(ctx, kind == 1, P, &tiles, &rays))) break;
+      // chunks of the size the whole evaluation would use, so that the
+      // sample sees the same L2 reuse per chunk
       int64_t qch, nch;
-      chunking(n, tiles, 1.0, &qch, &nch);
+      chunking(nq, tiles, 1.0, &qch, &nch);
+      nch = (n + qch - 1) / qch;
       if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nch * G)))) break;
-      float ms = 0.f;
-      for (int rep = 0; rep < 2 && !rc; ++rep) {  // first launch warms up
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3 && !rc; ++rep) {  // first launch warms up; best of two
+        float t = 0.f;
         INCA_HIP(hipEventRecord(a, ctx->stream));
         rc = launch_eval(ctx, kind == 1, P, ctx->d_qtmp, n, qch, nch, ctx->d_partial);
         INCA_HIP(hipEventRecord(b, ctx->stream));
         INCA_HIP(hipEventSynchronize(b));
-        INCA_HIP(hipEventElapsedTime(&ms, a, b));
+        INCA_HIP(hipEventElapsedTime(&t, a, b));
+        if (rep > 0 && t < ms) ms = t;
       }
       if (ms < best_ms) {
         best_ms = ms;

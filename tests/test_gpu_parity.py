@@ -312,8 +312,9 @@ def test_ray_tiles_match_point_by_point(engine):
     """Radial scans evaluated along rays (P >= 2 points per lane on one
     direction, exp(-s) by the two-product recurrence) against the
     point-by-point exponential (P = 1) on the same prepared quartets.  Two
-    scan blocks with different steps and Lebedev orders (one becomes ray
-    tiles, the other ordinary tiles), and exponents up to 2e4 so that tight
+    scan blocks with different steps and Lebedev orders (one becomes rays,
+    the other's points lanes of one point), the s = 0 shell (its points
+    coincide and are evaluated once), and exponents up to 2e4 so that tight
     quartets take the point-by-point branch inside ray tiles.  The
     recurrence follows the ideal ray points, which differ from the grid's by
     the rounding of their coordinates."""
