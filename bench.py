@@ -272,9 +272,9 @@ PROFILE = os.path.join(ROOT, "profiles", "eval_fp64_radial_plot_cluster151.md")
 
 
 def profiled_traffic():
-    """HBM bytes of one FP64 step's grid loop on this workload (the ray-tile
-    and other-points kernels), from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB copy)."""
+    """HBM bytes of one FP64 step's ray-tile kernel on this workload, from
+    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a
+    1 GiB copy)."""
     try:
         for line in open(PROFILE):
             if line.startswith("traffic_bytes_per_fp64_step_eval"):
@@ -427,14 +427,14 @@ def main():
             "points_per_lane": {"fp64": tiles[0], "fp32": tiles[1]},
         },
         "roofline": {
-            "bound": "valu", "kernel": "k_eval_rays + k_eval_fp64 (ray tiles, other points)",
+            "bound": "valu", "kernel": "k_eval_rays (ray tiles of the radial scan)",
             "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU, "traffic": profiled_traffic(),
             "note": "FLOP = 16 + 2*Lsum per executed eval (FMA = 2), exp not counted; grid loop "
-                    "%.1f ms per step (HIP events on the engine stream around the skip mask, the "
-                    "ray-tile kernel and the other-points kernel); traffic = HBM bytes of those "
-                    "kernels per step from rocprofv3 FETCH_SIZE + WRITE_SIZE at 4 points per lane "
-                    "(%s); the ray kernel is bound by FP64 VALU issue (0.82 busy)"
+                    "%.1f ms per step (HIP events on the engine stream around the skip mask, rho "
+                    "and the ray-tile kernel); traffic = HBM bytes of the ray kernel per step from "
+                    "rocprofv3 FETCH_SIZE + WRITE_SIZE at 4 points per lane (%s); the ray kernel "
+                    "is bound by FP64 VALU issue (0.81 busy)"
                     % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
         "mixed": {
