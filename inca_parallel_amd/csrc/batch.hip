@@ -492,6 +492,14 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   size_t perm_cap = 0;
   double *d_cls = nullptr;  // class words of one molecule
   size_t cls_cap = 0;
+  struct FreeOnReturn {  // on every return (hipFree waits for the device)
+    int64_t *&perm;
+    double *&cls;
+    ~FreeOnReturn() {
+      if (perm) (void)hipFree(perm);
+      if (cls) (void)hipFree(cls);
+    }
+  } free_on_return{d_perm, d_cls};
   for (int m = 0; m < nmol; ++m) {
     const inca_ctx *c = mols[m];
     const int64_t G = c->G, n = c->q2;
@@ -545,8 +553,6 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     B_HIP(hipGetLastError());
     B_HIP(hipStreamSynchronize(st));  // d_perm is reused by the next molecule
   }
-  if (d_perm) (void)hipFree(d_perm);
-  if (d_cls) (void)hipFree(d_cls);
   B_HIP(hipStreamSynchronize(st));
   if ((rc = persistent_blocks(b, b->P))) return rc;
   if ((rc = build_items(b, b->P))) return rc;
