@@ -499,6 +499,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   switch (P) {
     case 1: INCA_EVAL(1) break;
     case 2: INCA_EVAL(2) break;
+    case 6: INCA_EVAL(6) break;
     case 8: INCA_EVAL(8) break;
     default: INCA_EVAL(4) break;
   }
@@ -596,7 +597,7 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
 int autotune(inca_ctx *ctx, int32_t *best) {
   const int64_t G = ctx->G, nq = ctx->q2;
   if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
-  const int cand[4] = {1, 2, 4, 8};
+  const int cand[] = {1, 2, 4, 6, 8};
   // an evenly strided sample of the prepared quartets (file order groups
   // quartets by primitive, and with the cutoff their cost depends on where
   // they sit), gathered into the pre-pass scratch, which is free after it;
@@ -618,7 +619,7 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   for (int kind = 0; kind < 2 && !rc; ++kind) {
     float best_ms = 1e30f;
     int best_p = 4;
-    for (int c = 0; c < 4 && !rc; ++c) {
+    for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])) && !rc; ++c) {
       const int P = cand[c];
       int64_t tiles;
       bool rays;

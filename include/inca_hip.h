@@ -288,10 +288,12 @@ int inca_get_analytic_integral(inca_ctx *ctx, double *total);
  * counterpart for mode 1. */
 int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
 
-/* Evaluation tiles: grid points per lane (1, 2, 4 or 8) of the FP64 and of
- * the FP32 kernel.  inca_autotune times each choice on an evenly strided
- * sample (up to 2^20) of the quartets prepared by the last inca_evaluate or
- * inca_prepare, on the current grid and with the current cutoff, keeps the
+/* Evaluation tiles: grid points per lane (1, 2, 4, 6 or 8) of the FP64 and
+ * of the FP32 kernel (on a radial scan, the FP64 kernel's lanes are rays of
+ * that many radii).  inca_autotune times each choice on an evenly strided
+ * sample of the quartets prepared by the last inca_evaluate or inca_prepare
+ * (about 2^36 evaluations per launch), on the current grid and with the
+ * current cutoff, keeps the
  * fastest and returns it in best[0] (FP64) and best[1] (FP32).  It
  * overwrites the partial sums, so evaluate again before reading Ivec.  No
  * reference counterpart. */

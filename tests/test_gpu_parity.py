@@ -328,13 +328,13 @@ def test_ray_tiles_match_point_by_point(engine):
     engine.grid_radial_plot(tart, [0.1, 0.2], [110, 302])
     out = {}
     try:
-        for P in (1, 2, 4, 8):
+        for P in (1, 2, 4, 6, 8):
             engine.set_tiles(P, 4)
             iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
             out[P] = (iv.copy(), engine.radial_values().copy())
     finally:
         engine.set_tiles(4, 4)
-    for P in (2, 4, 8):
+    for P in (2, 4, 6, 8):
         er, ea = _close(out[P][0], out[1][0])
         rr, ra = _close(out[P][1], out[1][1])
         print("P=%d rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e" % (
