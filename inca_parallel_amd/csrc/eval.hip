@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, 
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
-    ray_any<P>(rec, load_header(rec), o, hw, span, acc, s_exp2);
+    ray_any<P>(rec, load_header(rec), o, hw, span, h * h, acc, s_exp2);
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {

@@ -251,9 +251,31 @@ __device__ __forceinline__ double horner1(const double *__restrict__ c, int L, d
   return v;
 }
 
+// One axis polynomial (degree L >= 1, c[0] the highest power) at the ray
+// points p_k = p0 + k d.  Degree 1 is written in k itself, V_k = V_0 + k c0 d
+// (one FMA per point instead of forming p_k and then Horner); higher
+// degrees use Horner on p_k.  The first point is p0 itself.
+template <int P>
+__device__ __forceinline__ void axis_ray(const double *__restrict__ c, int L, double p0, double d,
+                                         double (&v)[P]) {
+  if (L == 1) {
+    const double c0 = c[0];
+    const double b1 = c0 * d;
+    v[0] = fma(p0, c0, c[1]);
+#pragma unroll
+    for (int k = 1; k < P; ++k) v[k] = fma((double)k, b1, v[0]);
+    return;
+  }
+  double p[P];
+  p[0] = p0;
+#pragma unroll
+  for (int k = 1; k < P; ++k) p[k] = fma((double)k, d, p0);
+  horner<P>(c, L, p, v);
+}
+
 template <int P, bool MX, bool MY, bool MZ>
 __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const QHeader &h,
-                                         const double (&o)[3], const double (&hw)[3],
+                                         const double (&o)[3], const double (&hw)[3], double h2,
                                          double (&acc)[P], const double *__restrict__ tbl) {
   static_assert(P >= 2, "a ray needs two points");
   constexpr bool any = MX || MY || MZ;
@@ -269,29 +291,21 @@ __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const Q
   if (!MY) S = MX ? cy[0] : S * cy[0];
   if (!MZ) S = (MX && MY) ? cz[0] : S * cz[0];
   // product of the axis polynomials at the P points, one axis at a time
-  double w[P], p[P], v[P];
-  if (MX) {
-#pragma unroll
-    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dx, p0x);
-    horner<P>(cx, h.Lx, p, w);
-  }
+  double w[P], v[P];
+  if (MX) axis_ray<P>(cx, h.Lx, p0x, dx, w);
   if (MY) {
-#pragma unroll
-    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dy, p0y);
-    horner<P>(cy, h.Ly, p, v);
+    axis_ray<P>(cy, h.Ly, p0y, dy, v);
 #pragma unroll
     for (int k = 0; k < P; ++k) w[k] = MX ? w[k] * v[k] : v[k];
   }
   if (MZ) {
-#pragma unroll
-    for (int k = 0; k < P; ++k) p[k] = fma((double)k, dz, p0z);
-    horner<P>(cz, h.Lz, p, v);
+    axis_ray<P>(cz, h.Lz, p0z, dz, v);
 #pragma unroll
     for (int k = 0; k < P; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
   }
   const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
   const double dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
-  const double dd = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double dd = h.sqe * h.sqe * h2;  // |d|^2 = (sqe h)^2
   const double rho = rec[7];
   double e = exp_neg(s0, tbl);
   double qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
@@ -338,20 +352,21 @@ __device__ __forceinline__ void ray_direct(const double *__restrict__ rec, const
 template <int P>
 __device__ __forceinline__ void ray_any(const double *__restrict__ rec, const QHeader &h,
                                         const double (&o)[3], const double (&hw)[3], double span,
-                                        double (&acc)[P], const double *__restrict__ tbl) {
+                                        double h2, double (&acc)[P],
+                                        const double *__restrict__ tbl) {
   if (!(fma(h.sqe, span, h.rcut) <= kRayMaxRp)) {
     ray_direct<P>(rec, h, o, hw, acc, tbl);
     return;
   }
   switch (h.axes) {
-    case 0: ray_case<P, false, false, false>(rec, h, o, hw, acc, tbl); break;
-    case 1: ray_case<P, true, false, false>(rec, h, o, hw, acc, tbl); break;
-    case 2: ray_case<P, false, true, false>(rec, h, o, hw, acc, tbl); break;
-    case 3: ray_case<P, true, true, false>(rec, h, o, hw, acc, tbl); break;
-    case 4: ray_case<P, false, false, true>(rec, h, o, hw, acc, tbl); break;
-    case 5: ray_case<P, true, false, true>(rec, h, o, hw, acc, tbl); break;
-    case 6: ray_case<P, false, true, true>(rec, h, o, hw, acc, tbl); break;
-    default: ray_case<P, true, true, true>(rec, h, o, hw, acc, tbl); break;
+    case 0: ray_case<P, false, false, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 1: ray_case<P, true, false, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 2: ray_case<P, false, true, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 3: ray_case<P, true, true, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 4: ray_case<P, false, false, true>(rec, h, o, hw, h2, acc, tbl); break;
+    case 5: ray_case<P, true, false, true>(rec, h, o, hw, h2, acc, tbl); break;
+    case 6: ray_case<P, false, true, true>(rec, h, o, hw, h2, acc, tbl); break;
+    default: ray_case<P, true, true, true>(rec, h, o, hw, h2, acc, tbl); break;
   }
 }
 
