@@ -67,10 +67,13 @@ struct inca_ctx {
   std::vector<double> radi;           // radial scan radii
   std::vector<int32_t> smn;           // kept points per radius
   int64_t grid_serial = 0;            // bumped by every new grid
+  int32_t cube_n[3] = {0, 0, 0};      // cube grid: points per axis (z fastest)
+  double cube_step[3] = {0, 0, 0};
 
   // ray tiles of a radial scan (index list per points-per-lane choice)
   int64_t ray_serial = -1;    // grid the ray analysis belongs to
-  double ray_h = 0.0;         // radial step of the ray runs (0: no rays)
+  double ray_h = 0.0;         // step along the rays (0: no rays)
+  int ray_kind = 0;           // 1: radial-scan shells, 2: cube lines along z
   std::vector<int32_t> ray_runs;  // (first shell, shell count) of each ray run
   std::vector<int32_t> dup_of;    // per point: an earlier point at the same place, or -1
   int32_t *d_dup = nullptr;       // (point, its copy source) pairs not evaluated

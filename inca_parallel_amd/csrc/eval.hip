@@ -69,10 +69,11 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   }
 }
 
-// Ray tiles of a radial scan: lane `lane` of tile t holds the points
-// idx[t][k][lane], k < P (-1: none), one direction at consecutive radii a
-// step h apart from its first point o (or o alone); the lane keeps o and
-// h w, and the quartets are evaluated by ray_any.
+// Ray tiles: lane `lane` of tile t holds the points idx[t][k][lane], k < P
+// (-1: none), equally spaced by h w from its first point o (or o alone):
+// one direction w = o / |o| at consecutive radii of a radial scan (kind 1),
+// or consecutive points of a cube line along z (kind 2).  The lane keeps o
+// and h w, and the quartets are evaluated by ray_any.
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, 
                                                   const unsigned long long *__restrict__ mask,
                                                   int64_t ntile, int64_t tiles, int64_t nch,
                                                   int xcd, const int32_t *__restrict__ idx,
-                                                  double h, double *__restrict__ partial) {
+                                                  double h, int kind, double *__restrict__ partial) {
   const BlockTask task = block_task(tiles, xcd);
   if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
   const int64_t tile = task.tile_block * (kT / 64) +
@@ -98,11 +99,15 @@ __global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, 
     o[0] = gx[g0];
     o[1] = gy[g0];
     o[2] = gz[g0];
-    const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
-    const double f = r0 > 0.0 ? h / r0 : 0.0;
-    hw[0] = f * o[0];
-    hw[1] = f * o[1];
-    hw[2] = f * o[2];
+    if (kind == 2) {  // cube line along z
+      hw[2] = h;
+    } else {          // radial ray: w = o / |o|
+      const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
+      const double f = r0 > 0.0 ? h / r0 : 0.0;
+      hw[0] = f * o[0];
+      hw[1] = f * o[1];
+      hw[2] = f * o[2];
+    }
   }
   const double span = (P - 1) * h;
   double acc[P];
@@ -246,7 +251,10 @@ int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
 
 namespace {
 
-// ---- ray tiles of a radial scan ----
+// ---- ray tiles of a radial scan or a cube grid ----
+//
+// A cube grid is lines of points along z a step apart (mod_build_grid.f90:
+// 412-440, z fastest): each line is cut into lanes of P consecutive points.
 //
 // A radial scan is shells of points s_j w_d (mod_build_grid.f90:306-326):
 // within one block of the scan, the shells after s = 0 keep the same
@@ -267,7 +275,16 @@ int ray_analyse(inca_ctx *ctx) {
   ctx->ray_runs.clear();
   ctx->dup_of.clear();
   ctx->ndup = 0;
+  ctx->ray_kind = 0;
   const int64_t G = ctx->G;
+  if (ctx->grid_kind == GRID_CUBE) {
+    // lines along z (the fastest index), points step_z apart
+    if (G > 0 && G <= INT32_MAX && ctx->cube_n[2] >= 2 && ctx->cube_step[2] > 0.0) {
+      ctx->ray_h = ctx->cube_step[2];
+      ctx->ray_kind = 2;
+    }
+    return 0;
+  }
   const int nsh = (int)ctx->radi.size();
   if (ctx->grid_kind != GRID_RADIAL || G == 0 || G > INT32_MAX || nsh < 2) return 0;
   std::vector<int64_t> off(nsh + 1, 0);
@@ -341,6 +358,7 @@ int ray_analyse(inca_ctx *ctx) {
       ctx->ray_runs.push_back(a.len);
     }
   ctx->ray_h = h_main;
+  ctx->ray_kind = 1;
   return 0;
 }
 
@@ -355,11 +373,19 @@ int ray_plan(inca_ctx *ctx, int P) {
   if (ctx->ray_h <= 0.0 || P < 2) return 0;
   if (ctx->ray_P == P) return 0;
   const int64_t G = ctx->G;
-  const int nsh = (int)ctx->radi.size();
+  std::vector<int32_t> lanes;  // P slots per lane
+  std::vector<int32_t> dups;   // (copy, source) pairs
+  if (ctx->ray_kind == 2) {
+    // cube: each z line cut into lanes of P consecutive points
+    const int64_t nz = ctx->cube_n[2], nline = G / nz;
+    for (int64_t l = 0; l < nline; ++l)
+      for (int64_t gs = 0; gs < nz; gs += P)
+        for (int k = 0; k < P; ++k) lanes.push_back(gs + k < nz ? (int32_t)(l * nz + gs + k) : -1);
+  }
+  const int nsh = ctx->ray_kind == 1 ? (int)ctx->radi.size() : 0;
   std::vector<int64_t> off(nsh + 1, 0);
   for (int j = 0; j < nsh; ++j) off[j + 1] = off[j] + ctx->smn[j];
-  std::vector<uint8_t> used(G, 0);
-  std::vector<int32_t> lanes;  // P slots per lane
+  std::vector<uint8_t> used(ctx->ray_kind == 1 ? G : 0, 0);
   for (size_t i = 0; i < ctx->ray_runs.size(); i += 2) {
     const int j0 = ctx->ray_runs[i], len = ctx->ray_runs[i + 1];
     const int64_t nd = ctx->smn[j0];
@@ -375,8 +401,7 @@ int ray_plan(inca_ctx *ctx, int P) {
           }
         }
   }
-  std::vector<int32_t> dups;  // (copy, source) pairs
-  for (int64_t g = 0; g < G; ++g) {
+  for (int64_t g = 0; ctx->ray_kind == 1 && g < G; ++g) {
     if (used[g]) continue;
     if (ctx->dup_of[g] >= 0) {
       dups.push_back((int32_t)g);
@@ -424,7 +449,7 @@ int ensure_rho(inca_ctx *ctx) {
 int tile_layout(inca_ctx *ctx, bool mixed, int P, int64_t *tiles, bool *rays) {
   *rays = false;
   *tiles = (ctx->G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
-  if (mixed || P < 2 || ctx->grid_kind != GRID_RADIAL) return 0;
+  if (mixed || P < 2 || (ctx->grid_kind != GRID_RADIAL && ctx->grid_kind != GRID_CUBE)) return 0;
   int rc;
   if ((rc = ray_plan(ctx, P))) return rc;
   if (ctx->ray_P != P) return 0;
@@ -484,7 +509,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     if constexpr (PP >= 2)                                                                     \
       hipLaunchKernelGGL((k_eval_rays<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n,     \
                          qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask,      \
-                         ntile, tiles, nch, xcd, ridx, h, partial);                            \
+                         ntile, tiles, nch, xcd, ridx, h, ctx->ray_kind, partial);             \
   } else {                                                                                     \
     hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
                        ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \

@@ -501,6 +501,10 @@ int grid_cube(inca_ctx *ctx, const double *center, const double *step,
   INCA_HIP(hipStreamSynchronize(ctx->stream));
   *ngrid = n;
   ctx->grid_kind = GRID_CUBE;
+  for (int i = 0; i < 3; ++i) {
+    ctx->cube_n[i] = np[i];
+    ctx->cube_step[i] = step[i];
+  }
   return 0;
 }
 

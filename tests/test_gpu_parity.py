@@ -344,6 +344,39 @@ def test_ray_tiles_match_point_by_point(engine):
         assert rr < 1e-11 and ra < 1e-13
 
 
+def test_ray_tiles_on_cube_lines(engine):
+    """Cube grids run on the same ray kernel, each line along z cut into
+    lanes of P consecutive points (80 points per line, so the last lane of
+    a line is partial for P = 6), against the point-by-point kernel (P = 1),
+    with the cutoff on and off."""
+    s = synth.config_cluster(seed=33, natoms=3, nheavy=2, nocc=3, exp_lo=0.05, exp_hi=2e3,
+                             shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
+    ijkl, d = s.dm2_unique()
+    p = s.prims()
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=ijkl, d=d)
+    engine.grid_cube([0.1, -0.2, 0.3], [0.3, 0.25, 0.15], [9, 7, 80])
+    out = {}
+    try:
+        for cut in (100, 0):
+            engine.set_point_cutoff(cut)
+            for P in (1, 4, 6, 8):
+                engine.set_tiles(P, 4)
+                iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
+                out[(cut, P)] = iv.copy()
+    finally:
+        engine.set_tiles(4, 4)
+        engine.set_point_cutoff(100)
+    for cut in (100, 0):
+        for P in (4, 6, 8):
+            er, ea = _close(out[(cut, P)], out[(cut, 1)])
+            print("cube lines, cutoff %d, P=%d vs point by point: rel %.2e abs %.2e" % (
+                cut, P, er, ea))
+            if cut:  # with the cutoff off every quartet goes point by point
+                assert not np.array_equal(out[(cut, P)], out[(cut, 1)])
+            assert er < 1e-11 and ea < 1e-13
+
+
 def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     """Independent check of each prepared record's cutoff radius: at |rp| =
     rcut in random directions, |A e^-s Vx Vy Vz| (numpy, from the record's
