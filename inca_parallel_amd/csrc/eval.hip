@@ -74,8 +74,13 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
 // one direction w = o / |o| at consecutive radii of a radial scan (kind 1),
 // or consecutive points of a cube line along z (kind 2).  The lane keeps o
 // and h w, and the quartets are evaluated by ray_any.
+//
+// Occupancy 5 is asked for: at 8 points per lane the kernel needs 113 VGPRs
+// (occupancy 4) and is then limited by latency; at 96 VGPRs it spills 68 B
+// per lane and still runs faster (radial-plot workload 420 -> 408 ms;
+// occupancy 6 spills 124 B per lane and takes 828 ms).
 template <int P, bool CUT>
-__global__ __launch_bounds__(kT) void k_eval_rays(const double *__restrict__ q, int qstride,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_eval_rays(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
                                                   const double *__restrict__ gy,
