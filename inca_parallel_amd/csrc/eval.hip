@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
 // (occupancy 4) and is then limited by latency; at 96 VGPRs it spills 68 B
 // per lane and still runs faster (radial-plot workload 420 -> 408 ms;
 // occupancy 6 spills 124 B per lane and takes 828 ms).
-template <int P, bool CUT>
+template <int P, bool CUT, bool ZLINE>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_eval_rays(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                                   const unsigned long long *__restrict__ mask,
                                                   int64_t ntile, int64_t tiles, int64_t nch,
                                                   int xcd, const int32_t *__restrict__ idx,
-                                                  double h, int kind, double *__restrict__ partial) {
+                                                  double h, double *__restrict__ partial) {
   const BlockTask task = block_task(tiles, xcd);
   if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
   const int64_t tile = task.tile_block * (kT / 64) +
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_
     o[0] = gx[g0];
     o[1] = gy[g0];
     o[2] = gz[g0];
-    if (kind == 2) {  // cube line along z
+    if (ZLINE) {  // cube line along z
       hw[2] = h;
     } else {          // radial ray: w = o / |o|
       const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_
   const int64_t q1 = empty ? q0 : (q0 + qchunk < nq ? q0 + qchunk : nq);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t qi) {
     const double *__restrict__ rec = q + qi * qstride;
-    ray_any<P>(rec, load_header(rec), o, hw, span, h * h, acc, s_exp2);
+    ray_any<P, ZLINE>(rec, load_header(rec), o, hw, span, h * h, acc, s_exp2);
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {
@@ -511,10 +511,16 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
                        ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \
                        tiles, nch, xcd, partial);                                              \
   } else if (rays) {                                                                           \
-    if constexpr (PP >= 2)                                                                     \
-      hipLaunchKernelGGL((k_eval_rays<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n,     \
-                         qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask,      \
-                         ntile, tiles, nch, xcd, ridx, h, ctx->ray_kind, partial);             \
+    if constexpr (PP >= 2) {                                                                   \
+      if (ctx->ray_kind == 2)                                                                  \
+        hipLaunchKernelGGL((k_eval_rays<PP, CC, true>), grid, dim3(kT), 0, st, q, ctx->qstride,\
+                           n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, \
+                           ntile, tiles, nch, xcd, ridx, h, partial);                          \
+      else                                                                                     \
+        hipLaunchKernelGGL((k_eval_rays<PP, CC, false>), grid, dim3(kT), 0, st, q,             \
+                           ctx->qstride, n, qch, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,\
+                           ctx->d_mask, ntile, tiles, nch, xcd, ridx, h, partial);             \
+    }                                                                                          \
   } else {                                                                                     \
     hipLaunchKernelGGL((k_eval_fp64<PP, CC>), grid, dim3(kT), 0, st, q, ctx->qstride, n, qch,  \
                        ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb, ctx->d_mask, ntile,     \

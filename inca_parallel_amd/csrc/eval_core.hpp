@@ -273,46 +273,61 @@ __device__ __forceinline__ void axis_ray(const double *__restrict__ c, int L, do
   horner<P>(c, L, p, v);
 }
 
-template <int P, bool MX, bool MY, bool MZ>
+// ZLINE: the lane runs along z (cube lines), so rp_x and rp_y are the same
+// at all its points and their polynomials are evaluated once per lane.
+template <int P, bool ZLINE, bool MX, bool MY, bool MZ>
 __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const QHeader &h,
                                          const double (&o)[3], const double (&hw)[3], double h2,
                                          double (&acc)[P], const double *__restrict__ tbl) {
   static_assert(P >= 2, "a ray needs two points");
-  constexpr bool any = MX || MY || MZ;
   const double *__restrict__ cx = rec + kQHead;
   const double *__restrict__ cy = cx + h.Lx + 1;
   const double *__restrict__ cz = cy + h.Ly + 1;
   const double p0x = fma(h.sqe, o[0], h.ax), p0y = fma(h.sqe, o[1], h.ay),
                p0z = fma(h.sqe, o[2], h.az);
-  const double dx = h.sqe * hw[0], dy = h.sqe * hw[1], dz = h.sqe * hw[2];
   // constants of the degree-0 axes
   double S = 1.0;
   if (!MX) S = cx[0];
   if (!MY) S = MX ? cy[0] : S * cy[0];
   if (!MZ) S = (MX && MY) ? cz[0] : S * cz[0];
-  // product of the axis polynomials at the P points, one axis at a time
   double w[P], v[P];
-  if (MX) axis_ray<P>(cx, h.Lx, p0x, dx, w);
-  if (MY) {
-    axis_ray<P>(cy, h.Ly, p0y, dy, v);
+  double dot, dx = 0.0, dy = 0.0, dz;
+  constexpr bool perpoint = ZLINE ? MZ : (MX || MY || MZ);  // w varies along the lane
+  if (ZLINE) {
+    // x and y polynomials once per lane, folded into S
+    if (MX) S *= horner1(cx, h.Lx, p0x);
+    if (MY) S *= horner1(cy, h.Ly, p0y);
+    dz = h.sqe * hw[2];
+    if (MZ) axis_ray<P>(cz, h.Lz, p0z, dz, w);
+    dot = dz * p0z;
+  } else {
+    dx = h.sqe * hw[0];
+    dy = h.sqe * hw[1];
+    dz = h.sqe * hw[2];
+    // product of the axis polynomials at the P points, one axis at a time
+    if (MX) axis_ray<P>(cx, h.Lx, p0x, dx, w);
+    if (MY) {
+      axis_ray<P>(cy, h.Ly, p0y, dy, v);
 #pragma unroll
-    for (int k = 0; k < P; ++k) w[k] = MX ? w[k] * v[k] : v[k];
-  }
-  if (MZ) {
-    axis_ray<P>(cz, h.Lz, p0z, dz, v);
+      for (int k = 0; k < P; ++k) w[k] = MX ? w[k] * v[k] : v[k];
+    }
+    if (MZ) {
+      axis_ray<P>(cz, h.Lz, p0z, dz, v);
 #pragma unroll
-    for (int k = 0; k < P; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
+      for (int k = 0; k < P; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
+    }
+    dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
   }
   const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
-  const double dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
   const double dd = h.sqe * h.sqe * h2;  // |d|^2 = (sqe h)^2
   const double rho = rec[7];
   double e = exp_neg(s0, tbl);
   double qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
-  if (!(MX && MY && MZ)) e *= S;
+  // S multiplies e unless every factor is already in w
+  if (ZLINE || !(MX && MY && MZ)) e *= S;
 #pragma unroll
   for (int k = 0; k < P; ++k) {
-    acc[k] = any ? fma(e, w[k], acc[k]) : acc[k] + e;
+    acc[k] = perpoint ? fma(e, w[k], acc[k]) : acc[k] + e;
     if (k + 1 < P) e *= qk;
     if (k + 2 < P) qk *= rho;
   }
@@ -349,7 +364,7 @@ __device__ __forceinline__ void ray_direct(const double *__restrict__ rec, const
   }
 }
 
-template <int P>
+template <int P, bool ZLINE>
 __device__ __forceinline__ void ray_any(const double *__restrict__ rec, const QHeader &h,
                                         const double (&o)[3], const double (&hw)[3], double span,
                                         double h2, double (&acc)[P],
@@ -359,14 +374,14 @@ __device__ __forceinline__ void ray_any(const double *__restrict__ rec, const QH
     return;
   }
   switch (h.axes) {
-    case 0: ray_case<P, false, false, false>(rec, h, o, hw, h2, acc, tbl); break;
-    case 1: ray_case<P, true, false, false>(rec, h, o, hw, h2, acc, tbl); break;
-    case 2: ray_case<P, false, true, false>(rec, h, o, hw, h2, acc, tbl); break;
-    case 3: ray_case<P, true, true, false>(rec, h, o, hw, h2, acc, tbl); break;
-    case 4: ray_case<P, false, false, true>(rec, h, o, hw, h2, acc, tbl); break;
-    case 5: ray_case<P, true, false, true>(rec, h, o, hw, h2, acc, tbl); break;
-    case 6: ray_case<P, false, true, true>(rec, h, o, hw, h2, acc, tbl); break;
-    default: ray_case<P, true, true, true>(rec, h, o, hw, h2, acc, tbl); break;
+    case 0: ray_case<P, ZLINE, false, false, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 1: ray_case<P, ZLINE, true, false, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 2: ray_case<P, ZLINE, false, true, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 3: ray_case<P, ZLINE, true, true, false>(rec, h, o, hw, h2, acc, tbl); break;
+    case 4: ray_case<P, ZLINE, false, false, true>(rec, h, o, hw, h2, acc, tbl); break;
+    case 5: ray_case<P, ZLINE, true, false, true>(rec, h, o, hw, h2, acc, tbl); break;
+    case 6: ray_case<P, ZLINE, false, true, true>(rec, h, o, hw, h2, acc, tbl); break;
+    default: ray_case<P, ZLINE, true, true, true>(rec, h, o, hw, h2, acc, tbl); break;
   }
 }
 
