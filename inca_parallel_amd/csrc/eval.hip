@@ -587,31 +587,46 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
   const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
   const int64_t nr[2] = {n64, nq - n64};
+  // the kernel of each range: FP64 with tile_p[0]; the FP32 range with the
+  // mixed kernel, or (tile_p[1] == 0) on the FP64 ray tiles where the grid has
+  // them
+  bool mixedk[2] = {false, true};
+  int Pk[2] = {ctx->tile_p[0], ctx->tile_p[1]};
+  if (Pk[1] == 0) {
+    int64_t t;
+    bool ry;
+    if ((rc = tile_layout(ctx, false, Pk[0], &t, &ry))) return rc;
+    if (ry) {
+      mixedk[1] = false;
+      Pk[1] = Pk[0];
+    } else {
+      Pk[1] = 8;
+    }
+  }
   int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
-  bool rays64 = false;  // the FP64 range runs on the ray lists (duplicates skipped)
+  bool rays_used = false;  // a range ran on the ray lists (duplicate points skipped)
   for (int r = 0; r < 2; ++r) {
     if (nr[r] == 0) continue;
     int64_t tiles;
     bool rays;
-    if ((rc = tile_layout(ctx, r == 1, ctx->tile_p[r], &tiles, &rays))) return rc;
+    if ((rc = tile_layout(ctx, mixedk[r], Pk[r], &tiles, &rays))) return rc;
     if (rays && (rc = ensure_rho(ctx))) return rc;
-    if (r == 0) rays64 = rays;
+    rays_used = rays_used || rays;
     chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
   }
   const int64_t nchunks = nch[0] + nch[1];
   if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
   if (nch[0] > 0 &&
-      (rc = launch_eval(ctx, false, ctx->tile_p[0], ctx->d_q, nr[0], qch[0], nch[0],
-                        ctx->d_partial)))
+      (rc = launch_eval(ctx, mixedk[0], Pk[0], ctx->d_q, nr[0], qch[0], nch[0], ctx->d_partial)))
     return rc;
   if (nch[1] > 0 &&
-      (rc = launch_eval(ctx, true, ctx->tile_p[1], ctx->d_q + n64 * ctx->qstride, nr[1], qch[1],
+      (rc = launch_eval(ctx, mixedk[1], Pk[1], ctx->d_q + n64 * ctx->qstride, nr[1], qch[1],
                         nch[1], ctx->d_partial + nch[0] * G)))
     return rc;
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
                      ctx->d_partial, (int)nchunks, G, ctx->d_ivec);
-  if (rays64 && ctx->ndup > 0)
+  if (rays_used && ctx->ndup > 0)
     hipLaunchKernelGGL(k_copy_dups, dim3((unsigned)((ctx->ndup + kT - 1) / kT)), dim3(kT), 0, st,
                        ctx->d_dup, ctx->ndup, ctx->d_ivec);
   INCA_HIP(hipGetLastError());
@@ -679,6 +694,29 @@ int autotune(inca_ctx *ctx, int32_t *best) {
       if (ms < best_ms) {
         best_ms = ms;
         best_p = P;
+      }
+    }
+    if (kind == 1 && !rc) {
+      // the FP32 range on the FP64 ray tiles (tile_p[0], tuned above)
+      int64_t tiles;
+      bool rays;
+      if ((rc = tile_layout(ctx, false, ctx->tile_p[0], &tiles, &rays))) break;
+      if (rays) {
+        int64_t qch, nch;
+        chunking(nq, tiles, 1.0, &qch, &nch);
+        nch = (n + qch - 1) / qch;
+        if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nch * G)))) break;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3 && !rc; ++rep) {
+          float t = 0.f;
+          INCA_HIP(hipEventRecord(a, ctx->stream));
+          rc = launch_eval(ctx, false, ctx->tile_p[0], ctx->d_qtmp, n, qch, nch, ctx->d_partial);
+          INCA_HIP(hipEventRecord(b, ctx->stream));
+          INCA_HIP(hipEventSynchronize(b));
+          INCA_HIP(hipEventElapsedTime(&t, a, b));
+          if (rep > 0 && t < ms) ms = t;
+        }
+        if (ms < best_ms) best_p = 0;
       }
     }
     ctx->tile_p[kind] = best_p;

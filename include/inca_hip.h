@@ -289,8 +289,11 @@ int inca_get_analytic_integral(inca_ctx *ctx, double *total);
 int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
 
 /* Evaluation tiles: grid points per lane (1, 2, 4, 6 or 8) of the FP64 and
- * of the FP32 kernel (on a radial scan, the FP64 kernel's lanes are rays of
- * that many radii).  inca_autotune times each choice on an evenly strided
+ * of the FP32 kernel (on radial scans and cube grids, the FP64 kernel's
+ * lanes are rays of that many points).  p32 = 0 runs the FP32 range of a
+ * mixed evaluation on the FP64 ray tiles instead, where the grid has them
+ * (more accurate; faster wherever the ray kernel is); inca_autotune picks it
+ * when it times faster.  inca_autotune times each choice on an evenly strided
  * sample of the quartets prepared by the last inca_evaluate or inca_prepare
  * (about 2^36 evaluations per launch), on the current grid and with the
  * current cutoff, keeps the

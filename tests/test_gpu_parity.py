@@ -377,6 +377,43 @@ def test_ray_tiles_on_cube_lines(engine):
             assert er < 1e-11 and ea < 1e-13
 
 
+def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
+    """inca_set_tiles(p64, 0): a mixed evaluation runs its FP32 range on the
+    FP64 ray tiles, so on a radial scan and a cube it agrees with the FP64
+    evaluation up to summation order; on a multicentre grid (no rays) the
+    FP32 range keeps the mixed kernel and the mixed tolerance."""
+    s, ijkl, d = small_cluster
+    p = s.prims()
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=ijkl, d=d)
+    cent, Ps, sf = O.centercalc(p["an"], p["cartes"], True, False)
+    nq = len(Ps)
+    grids = [
+        (True, lambda: engine.grid_radial_plot([[0.0, 6.0]], [0.1], [110])),
+        (True, lambda: engine.grid_cube([0.0, 0.0, 0.0], [0.3, 0.3, 0.3], [10, 10, 24])),
+        (False, lambda: engine.grid_multicenter(cent, [30] * nq, [110] * nq, sf, Ps, nosym=True)),
+    ]
+    try:
+        engine.set_mixed_tail(0.05)
+        for rays, make in grids:
+            make()
+            engine.set_tiles(8, 4)
+            ref, _ = engine.evaluate(capi.INCA_FP64, THRESH)
+            engine.set_tiles(8, 0)
+            iv, _ = engine.evaluate(capi.INCA_MIXED, THRESH)
+            n64, n32 = engine.mixed_split()[:2]
+            assert n32 > 0
+            er, ea = _close(iv, ref)
+            print("FP32 range on ray tiles (rays=%s): rel %.2e abs %.2e" % (rays, er, ea))
+            if rays:
+                assert er < 1e-13 and ea < 1e-14
+            else:
+                assert ea < 1e-8
+    finally:
+        engine.set_tiles(4, 4)
+        engine.set_mixed_tail(0.02)
+
+
 def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     """Independent check of each prepared record's cutoff radius: at |rp| =
     rcut in random directions, |A e^-s Vx Vy Vz| (numpy, from the record's
