@@ -306,9 +306,11 @@ int inca_autotune(inca_ctx *ctx, int32_t *best);
 /* Diagnostics: copy prepared quartet records [first, first+n) of the last
  * inca_evaluate (FP64 range first, then the FP32 range in mixed mode).
  * Record layout: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
- * Lx | Ly<<8 | Lz<<16 (as integer bits), [5] B (the L1 bound of the term,
- * see inca_set_mixed_tail), [6] the cutoff radius in rp (+inf when off,
- * see inca_set_point_cutoff), [7] unused, then A*Cx(0..Lx), Cy(0..Ly),
+ * Lx | Ly<<8 | Lz<<16 | (axes of nonzero degree)<<24 (as integer bits),
+ * [5] B (the L1 bound of the term, see inca_set_mixed_tail), [6] the cutoff
+ * radius in rp (+inf when off, see inca_set_point_cutoff), [7]
+ * exp(-2 sqe^2 h^2) for the ray step h of the current radial-scan or cube
+ * grid (0 until such a grid is evaluated), then A*Cx(0..Lx), Cy(0..Ly),
  * Cz(0..Lz); *stride receives the record length in doubles.
  * No reference counterpart. */
 int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out,
