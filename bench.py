@@ -434,7 +434,7 @@ def main():
                     "%.1f ms per step (HIP events on the engine stream around the skip mask, rho "
                     "and the ray-tile kernel); traffic = HBM bytes of the ray kernel per step from "
                     "rocprofv3 FETCH_SIZE + WRITE_SIZE at 8 points per lane (%s); the ray kernel "
-                    "is bound by FP64 VALU issue (0.74 busy at occupancy 4)"
+                    "is bound by FP64 VALU issue (0.83 busy at occupancy 5)"
                     % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
         "mixed": {

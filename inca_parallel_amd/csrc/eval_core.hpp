@@ -309,7 +309,12 @@ __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const Q
     if (MY) {
       axis_ray<P>(cy, h.Ly, p0y, dy, v);
 #pragma unroll
-      for (int k = 0; k < P; ++k) w[k] = MX ? w[k] * v[k] : v[k];
+      for (int k = 0; k < P; ++k) {
+        w[k] = MX ? w[k] * v[k] : v[k];
+        // keep the product here: sunk past the z axis, it holds x, y and z
+        // values live together and spills at 8 points per lane
+        if (MX && MZ) asm volatile("" : "+v"(w[k]));
+      }
     }
     if (MZ) {
       axis_ray<P>(cz, h.Lz, p0z, dz, v);
