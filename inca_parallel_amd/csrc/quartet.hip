@@ -8,8 +8,11 @@
 //                  Screen 1 then costs two table reads per record instead of
 //                  two JA8 evaluations.
 //   k_first_bad    the reader's stop rule (first record with a bad index).
-//   k_prep<NPM>    one thread per record: screens + prefactors + polycoef;
+//   k_prep<NPM,S>  one thread per record: screens + prefactors + polycoef;
 //                  writes the prepared record into slot n and a pass flag.
+//                  S: records with every axis of degree <= 4, fits in
+//                  registers; the others are deferred to a second launch
+//                  over their compacted indices with the fit in scratch.
 //   compaction     deterministic scan of the flags, then a gather of the
 //                  surviving records in file order.
 // Records are read straight from HBM as three 8-byte words per 24-byte
@@ -68,6 +71,18 @@ const double kGhRoot[11][11] = {
 __device__ __forceinline__ double ipow(double x, int e) {
   double r = 1.0;
   for (int k = 0; k < e; ++k) r = (k == 0) ? x : r * x;
+  return r;
+}
+
+// ipow for exponents e <= E, as a loop of fixed length E with the same
+// multiplications (no run-time trip count, so fully unrolled callers keep
+// their values in registers)
+template <int E>
+__device__ __forceinline__ double ipow_upto(double x, int e) {
+  double r = 1.0;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    if (k < e) r = (k == 0) ? x : r * x;
   return r;
 }
 
@@ -246,6 +261,159 @@ __device__ void exact_poly(const double R[4], double Rijkl, const int t[4], int 
   for (int q = 0; q <= L; ++q) E[q] = acc[L - q];
 }
 
+// polycoef for an axis of degree L <= NP - 1 as an NP x NP system with
+// compile-time indices, so that the matrix stays in registers.  The
+// np = L + 1 equations sit in the leading block; the rest of the matrix is
+// the identity and the rest of the right-hand side zero.  Partial pivoting
+// never picks a padding row for a leading column (their entries there are
+// zero and the search takes strictly larger values), the eliminations
+// subtract exact zeros from the padding, and the solves skip the zero
+// unknowns, so the leading coefficients come out of the same operations as
+// in polycoef<NPM> on the np x np system (np = 1 included: the single
+// Gauss-Hermite weight divided by one).  The pivot row is a per-lane value,
+// so row exchanges are selects over the rows below.
+template <int NP>
+__device__ __forceinline__ int polycoef_pad(const double R[4], double Rik, double Rjl,
+                                            double Rijkl, const int t[4], int L, double sqe,
+                                            double invz, double alf, double (&C)[NP]) {
+  constexpr int E = NP - 1;         // largest degree, and largest t[f]
+  constexpr int NN = E / 2 + 1;     // most Gauss-Hermite nodes
+  const int np = L + 1;
+  const int n = L / 2 + 1;
+  double M[NP][NP];  // M[i][j] = y_i^(L - j) in the leading block
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const double xx = -0.5 + (double)(i + 1);
+    const double y = sqe * (xx + Rik - Rjl);
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      M[i][j] = (i < np && j < np) ? ipow_upto<E>(y, L - j) : (i == j ? 1.0 : 0.0);
+    const double am = (alf - 0.5) * xx, ap = (alf + 0.5) * xx;
+    double c = 0.0;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      if (k < n) {
+        const double a = invz * c_gh_x[n - 1][k];
+        const double w = ipow_upto<E>(a + am + (Rijkl - R[0]), t[0]) *
+                         ipow_upto<E>(a + ap + (Rijkl - R[1]), t[1]) *
+                         ipow_upto<E>(a + am + (Rijkl - R[2]), t[2]) *
+                         ipow_upto<E>(a + ap + (Rijkl - R[3]), t[3]);
+        c += c_gh_w[n - 1][k] * w;
+      }
+    }
+    C[i] = (i < np) ? c : 0.0;
+  }
+  int piv[NP];
+  int info = 0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    int p = k;
+    double amax = fabs(M[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < NP; ++i) {
+      const double v = fabs(M[i][k]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    piv[k] = p;
+    // Exchange rows k and p.  A zero pivot column leaves p = k (the search
+    // takes strictly larger values), so the exchange is unconditional.
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      double r = M[k][j];
+#pragma unroll
+      for (int i = k + 1; i < NP; ++i) r = (p == i) ? M[i][j] : r;
+      const double a = M[k][j];
+#pragma unroll
+      for (int i = k + 1; i < NP; ++i) M[i][j] = (p == i) ? a : M[i][j];
+      M[k][j] = r;
+    }
+    const double pv = M[k][k];
+    if (pv != 0.0) {
+      if (fabs(pv) >= 2.2250738585072014e-308) {
+        const double rp = 1.0 / pv;
+#pragma unroll
+        for (int i = k + 1; i < NP; ++i) M[i][k] *= rp;
+      } else {
+#pragma unroll
+        for (int i = k + 1; i < NP; ++i) M[i][k] /= pv;
+      }
+    } else if (info == 0) {
+      info = k + 1;
+    }
+#pragma unroll
+    for (int j = k + 1; j < NP; ++j) {
+      const double u = M[k][j];
+#pragma unroll
+      for (int i = k + 1; i < NP; ++i) M[i][j] -= M[i][k] * u;
+    }
+  }
+  int bad = (info != 0);
+  if (!bad) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+#pragma unroll
+      for (int i = k + 1; i < NP; ++i) {
+        const bool sw = (piv[k] == i);
+        const double a = C[k], b = C[i];
+        C[k] = sw ? b : a;
+        C[i] = sw ? a : b;
+      }
+#pragma unroll
+    for (int k = 0; k < NP; ++k)
+      if (C[k] != 0.0)
+#pragma unroll
+        for (int i = k + 1; i < NP; ++i) C[i] -= C[k] * M[i][k];
+#pragma unroll
+    for (int k = NP - 1; k >= 0; --k)
+      if (C[k] != 0.0) {
+        C[k] /= M[k][k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) C[i] -= C[k] * M[i][k];
+      }
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+      if (!(fabs(C[i]) <= 1.0e3)) bad = 1;
+  }
+  if (bad)
+#pragma unroll
+    for (int i = 0; i < NP; ++i) C[i] = 0.0;
+  return bad;
+}
+
+// One axis of degree L <= NP - 1 through polycoef_pad: second-screen factor,
+// L1 bound and integral factors, and the coefficients written to out[0..L]
+// (fitted polynomials only; the exact-polynomial mode takes polycoef<NPM>).
+template <int NP>
+__device__ __forceinline__ int axis_pad(const double R[4], double Rik, double Rjl, double Pijkl,
+                                        const int t[4], int L, double sqe, double invz,
+                                        double alf, double scale, double &U, double &B,
+                                        double &M, double *__restrict__ out) {
+  double C[NP];
+  const int bad = polycoef_pad<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  double u = 0.0;
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+    if (q <= L) u += C[q] * c_wm[q];
+  U *= u;
+  double bax = 0.0, max_ = 0.0;
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+    if (q <= L) {
+      const double g = c_gmom[L - q];
+      bax += fabs(C[q]) * g;
+      if (((L - q) & 1) == 0) max_ += C[q] * g;
+    }
+  B *= bax * (1.0 / sqe);
+  M *= max_ * (1.0 / sqe);
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+    if (q <= L) out[q] = scale * C[q];
+  return bad;
+}
+
 // Radius in rp beyond which the term A exp(-|rp|^2) Vx Vy Vz stays below
 // 2^-cut_log2 of its peak bound |A| prod_axis sum_k |c_k| ymax(k): with
 // |rp_axis| <= |rp| = sqrt(s), |term| <= |A| e^-s prod_axis H_axis(sqrt(s)),
@@ -292,19 +460,44 @@ __device__ double cutoff_radius(const double *c, const int Ls[3], double s0) {
 }
 
 // One thread per record; the output record layout is described at kQHead.
-template <int NPM>
+//
+// SMALL: the records whose three axes all have degree below NPM, each axis
+// by the padded compile-time-size fit (axis_pad), so nothing is indexed at
+// run time and the matrices stay in registers (fitted polynomials only); every other record is marked in
+// `defer` and left alone.  Otherwise: the records idx[0..nrun) (all records
+// when idx is null), each axis by the run-time-size fit, whose matrix lives
+// in scratch.
+template <int NPM, bool SMALL>
 __global__ __launch_bounds__(kT) void k_prep(
-    const int32_t *__restrict__ rec, int64_t nvalid, int nprim, const double *__restrict__ prim,
-    const int32_t *__restrict__ tmn, const PairEntry *__restrict__ pair, double lim,
-    int normalize, int qstride, int exact, double cut_s0, double *__restrict__ qout,
-    uint8_t *__restrict__ flag, unsigned long long *__restrict__ cnt,
+    const int32_t *__restrict__ rec, int64_t nrun, const int64_t *__restrict__ idx, int nprim,
+    const double *__restrict__ prim, const int32_t *__restrict__ tmn,
+    const PairEntry *__restrict__ pair, double lim, int normalize, int qstride, int exact,
+    double cut_s0, double *__restrict__ qout, uint8_t *__restrict__ flag,
+    uint8_t *__restrict__ defer, unsigned long long *__restrict__ cnt,
     double *__restrict__ msum) {
-  const int64_t n = (int64_t)blockIdx.x * kT + threadIdx.x;
+  static_assert(!SMALL || NPM <= 5, "the compile-time fits cover degrees 0-4");
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
   int pass1 = 0, pass2 = 0, nfail = 0;
   double mom = 0.0;  // signed whole-space integral of a surviving term
-  if (n < nvalid) {
-    const int32_t *p = rec + 6 * n;
-    const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+  bool run = g < nrun;
+  int64_t n = g;
+  if (run && !SMALL && idx) n = idx[g];
+  const int32_t *p = rec + 6 * n;
+  int i = 0, j = 0, k = 0, l = 0;
+  if (run) {
+    i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+    if (SMALL) {
+      int lmax = 0;
+      for (int ax = 0; ax < 3; ++ax) {
+        const int *tt = tmn + ax * nprim;
+        lmax = max(lmax, tt[i] + tt[j] + tt[k] + tt[l]);
+      }
+      const bool big = lmax >= NPM;
+      if (defer) defer[n] = (uint8_t)big;
+      run = !big;
+    }
+  }
+  if (run) {
     double D;
     __builtin_memcpy(&D, p + 4, sizeof(double));
     const double *nrm = prim + 4 * nprim;
@@ -330,6 +523,7 @@ __global__ __launch_bounds__(kT) void k_prep(
       // (per axis, int q(y) exp(-y^2) dr with dr = dy / sqe)
       double B = fabs(A), M = A;
       double C[NPM];
+#pragma unroll 1
       for (int ax = 0; ax < 3; ++ax) {
         const double *Rx = prim + (1 + ax) * nprim;
         const double R[4] = {Rx[i], Rx[j], Rx[k], Rx[l]};
@@ -338,6 +532,14 @@ __global__ __launch_bounds__(kT) void k_prep(
         const int L = t[0] + t[1] + t[2] + t[3];
         Ls[ax] = L;
         const double Pijkl = (aik * pik.P[ax] + ajl * pjl.P[ax]) * (1.0 / zeta);
+        if (SMALL) {
+          const double scale = (ax == 0) ? A : 1.0;
+          const double Rik = pik.P[ax], Rjl = pjl.P[ax];
+          nfail += axis_pad<NPM>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, scale, U, B, M,
+                                 out + off);
+          off += L + 1;
+          continue;
+        }
         nfail += polycoef<NPM>(R, pik.P[ax], pjl.P[ax], Pijkl, t, L + 1, sqe, invz, alf, C);
         double u = 0.0;
         for (int q = 0; q <= L; ++q) u += C[q] * c_wm[q];
@@ -550,36 +752,70 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // mod_intracule.f90:163
   const double lim = thresh * (1.0 / ((double)nprim * ((double)nprim + 1.0) * 0.5));
   const bool want_mom = nvalid > 0;
-  double *d_mom = nullptr;  // block partials of sum M, then the total after them
+  // block partials of sum M (the small-degree pass, then the deferred
+  // records' pass), then the total after them
+  const int64_t nb1 = nblk(nvalid);
+  double *d_mom = nullptr;
+  uint8_t *d_defer = nullptr;
   ctx->term_integral = 0.0;
-  if (want_mom) INCA_HIP(hipMalloc((void **)&d_mom, ((size_t)nblk(nvalid) + 1) * sizeof(double)));
+  if (want_mom) INCA_HIP(hipMalloc((void **)&d_mom, ((size_t)2 * nb1 + 1) * sizeof(double)));
   if (nvalid > 0) {
+    // Fitted polynomials: records whose axes are all of degree <= 4 (most
+    // of them) go through the register-resident fits; the rest are
+    // compacted in file order and handled by the scratch kernel sized for
+    // the basis.  Exact polynomials: the scratch kernel takes every record.
+    const bool split = !ctx->poly_exact;
+    hipError_t e = hipSuccess;
+    if (split && la > 4) e = hipMalloc((void **)&d_defer, (size_t)nvalid);
+    if (split && e == hipSuccess) {
+      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(nvalid)), dim3(kT), 0, st,
+                         (const int32_t *)ctx->d_rec, nvalid, (const int64_t *)nullptr, nprim,
+                         ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim, ctx->normalize, stride, 0,
+                         ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,
+                         d_defer, ctx->d_cnt, d_mom);
+      e = hipGetLastError();
+    }
+    int64_t nrun = split ? 0 : nvalid;  // records for the scratch kernel
+    if (e == hipSuccess && d_defer) {
+      rc = compact_flags(ctx, d_defer, nvalid, ctx->d_idx, &nrun);
+      if (rc) {
+        (void)hipFree(d_defer);
+        if (d_mom) (void)hipFree(d_mom);
+        return rc;
+      }
+    }
+    const int64_t *run_idx = split ? ctx->d_idx : nullptr;
+    double *run_mom = split ? d_mom + nb1 : d_mom;
+    if (e == hipSuccess && nrun > 0) {
 #define INCA_PREP(NPM)                                                                        \
-  hipLaunchKernelGGL(k_prep<NPM>, dim3(nblk(nvalid)), dim3(kT), 0, st,                       \
-                     (const int32_t *)ctx->d_rec, nvalid, nprim, ctx->d_prim, ctx->d_tmn,     \
-                     ctx->d_pair, lim, ctx->normalize, stride, ctx->poly_exact,               \
-                     ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp,                        \
-                     ctx->d_flag, ctx->d_cnt, d_mom)
-    if (la <= 4) INCA_PREP(5);
-    else if (la <= 8) INCA_PREP(9);
-    else if (la <= 12) INCA_PREP(13);
-    else if (la <= 16) INCA_PREP(17);
-    else INCA_PREP(22);
+  hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st,                \
+                     (const int32_t *)ctx->d_rec, nrun, run_idx, nprim, ctx->d_prim,          \
+                     ctx->d_tmn, ctx->d_pair, lim, ctx->normalize, stride, ctx->poly_exact,   \
+                     ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
+                     (uint8_t *)nullptr, ctx->d_cnt, run_mom)
+      if (la <= 4) INCA_PREP(5);
+      else if (la <= 8) INCA_PREP(9);
+      else if (la <= 12) INCA_PREP(13);
+      else if (la <= 16) INCA_PREP(17);
+      else INCA_PREP(22);
 #undef INCA_PREP
-    hipError_t e = hipGetLastError();
+      e = hipGetLastError();
+    }
+    const int64_t nbt =
+        split ? nb1 + (nrun > 0 ? (int64_t)nblk(nrun) : 0) : (int64_t)nblk(nrun);
     if (e == hipSuccess && want_mom) {
-      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, d_mom, (int64_t)nblk(nvalid),
-                         d_mom + nblk(nvalid));
+      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, d_mom, nbt, d_mom + nbt);
       e = hipGetLastError();
     }
     if (e == hipSuccess && want_mom)
-      e = hipMemcpyAsync(&ctx->term_integral, d_mom + nblk(nvalid), sizeof(double),
+      e = hipMemcpyAsync(&ctx->term_integral, d_mom + nbt, sizeof(double),
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && want_mom) e = hipStreamSynchronize(st);
     // in the grids' convention: their weights carry 2 pi where the volume
     // element has 4 pi (mod_build_grid.f90:132)
     ctx->term_integral *= 0.5;
     if (d_mom) (void)hipFree(d_mom);
+    if (d_defer) (void)hipFree(d_defer);
     if (e != hipSuccess) return hip_fail(ctx, e, "quartet preparation", __LINE__);
   }
   int64_t q2 = 0;

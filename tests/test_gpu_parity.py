@@ -455,6 +455,56 @@ def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):
     assert worst <= 2.0 ** -100
 
 
+def test_prepared_records_match_oracle_fits(engine, small_cluster):
+    """Record by record against the oracle's restatement of the pre-pass
+    (screens, prefactors, polycoef fits): 3000 records whose axes all have
+    degree <= 4, fitted in registers as padded 5 x 5 systems, and 2000 with
+    an axis of degree 5-8, deferred to the scratch kernel after compaction,
+    interleaved in file order.  Survivors, sqe, the class word and every
+    coefficient must agree (1e-13 of the axis's largest coefficient)."""
+    s, ijkl, d = small_cluster
+    p = s.prims()
+    tm = np.asarray(p["tmn"]).reshape(-1, 3)
+    lmax = tm[ijkl - 1].sum(axis=1).max(axis=1)
+    rng = np.random.default_rng(11)
+    small = np.flatnonzero(lmax <= 4)
+    big = np.flatnonzero(lmax >= 5)
+    assert len(big) >= 2000 and lmax.max() == 8
+    sel = np.sort(np.concatenate([rng.choice(small, 3000, replace=False),
+                                  rng.choice(big, 2000, replace=False)]))
+    sub, dsub = np.ascontiguousarray(ijkl[sel]), np.ascontiguousarray(d[sel])
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=sub, d=dsub)
+    engine.grid_points(np.zeros((1, 3)))
+    _, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
+    n = int(cnt[2])
+    recs = engine.prepared(0, n)
+    ref = [O.prepare_quartet(p, sub[r], dsub[r], THRESH) for r in range(len(sel))]
+    keep = [q for q in ref if q.pass1 and q.pass2]
+    assert n == len(keep)
+    assert int(cnt[1]) == sum(q.pass1 for q in ref)
+    assert int(cnt[3]) == sum(q.polyfail for q in ref)
+    n_big = 0
+    worst = 0.0
+    for r, q in zip(recs, keep):
+        cls = int(np.frombuffer(np.float64(r[4]).tobytes(), np.int64)[0])
+        L = [cls & 0xff, (cls >> 8) & 0xff, (cls >> 16) & 0xff]
+        assert L == list(q.L)
+        n_big += max(L) >= 5
+        assert abs(r[0] - q.sqe) <= 1e-15 * q.sqe
+        off = 8
+        for a in range(3):
+            c = np.array(q.C[a][:L[a] + 1]) * (q.A if a == 0 else 1.0)
+            g = r[off:off + L[a] + 1]
+            scale = max(np.abs(c).max(), 1e-300)
+            worst = max(worst, float(np.abs(g - c).max() / scale))
+            off += L[a] + 1
+    print("survivors %d (%d with an axis of degree >= 5), max coefficient error %.2e" % (
+        n, n_big, worst))
+    assert n_big > 0 and n - n_big > 0
+    assert worst <= 1e-13
+
+
 def test_fp64_exp_accuracy_through_one_s_primitive(engine):
     """One s primitive (alpha = 1) and the record (1,1,1,1): sqe = 1 and
     d = 0, so I(r) / I(0) = exp(-|r|^2) exactly.  Points on the x axis with
