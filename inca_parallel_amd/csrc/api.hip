@@ -439,10 +439,10 @@ int inca_set_polynomials(inca_ctx *ctx, int32_t mode) {
 
 int inca_set_tiles(inca_ctx *ctx, int32_t p64, int32_t p32) {
   CHECK_CTX(ctx);
-  auto ok = [](int p) { return p == 1 || p == 2 || p == 4 || p == 6 || p == 8; };
+  auto ok = [](int p) { return p == 1 || p == 2 || p == 4 || p == 6 || p == 8 || p == 10; };
   if (!ok(p64) || !(ok(p32) || p32 == 0))
     return set_err(ctx, INCA_EINVAL,
-                   "points per lane must be 1, 2, 4, 6 or 8 (or 0 for the FP32 range on ray tiles)");
+                   "points per lane must be 1, 2, 4, 6, 8 or 10 (or 0 for the FP32 range on ray tiles)");
   ctx->tile_p[0] = p64;
   ctx->tile_p[1] = p32;
   return 0;

@@ -75,12 +75,12 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
 // or consecutive points of a cube line along z (kind 2).  The lane keeps o
 // and h w, and the quartets are evaluated by ray_any.
 //
-// Occupancy 5 is asked for: at 8 points per lane the kernel needs 113 VGPRs
-// (occupancy 4) and is then limited by latency; at 96 VGPRs it spills 68 B
-// per lane and still runs faster (radial-plot workload 420 -> 408 ms;
-// occupancy 6 spills 124 B per lane and takes 828 ms).
+// Occupancy 5 is asked for up to 8 points per lane, 4 beyond.  At 8 points
+// the kernel fits 96 VGPRs with two values saved once per lane outside the
+// quartet loop (radial-plot workload 368 ms, against 417 ms at occupancy 4;
+// occupancy 6 spills inside the loop and takes 828 ms).
 template <int P, bool CUT, bool ZLINE>
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) void k_eval_rays(const double *__restrict__ q, int qstride,
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(P >= 10 ? 4 : 5))) void k_eval_rays(const double *__restrict__ q, int qstride,
                                                   int64_t nq, int64_t qchunk,
                                                   const double *__restrict__ gx,
                                                   const double *__restrict__ gy,
@@ -537,6 +537,7 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
     case 2: INCA_EVAL(2) break;
     case 6: INCA_EVAL(6) break;
     case 8: INCA_EVAL(8) break;
+    case 10: INCA_EVAL(10) break;
     default: INCA_EVAL(4) break;
   }
 #undef INCA_EVAL
@@ -648,7 +649,7 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
 int autotune(inca_ctx *ctx, int32_t *best) {
   const int64_t G = ctx->G, nq = ctx->q2;
   if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
-  const int cand[] = {1, 2, 4, 6, 8};
+  const int cand[] = {1, 2, 4, 6, 8, 10};
   // an evenly strided sample of the prepared quartets (file order groups
   // quartets by primitive, and with the cutoff their cost depends on where
   // they sit), gathered into the pre-pass scratch, which is free after it;

@@ -288,7 +288,7 @@ int inca_get_analytic_integral(inca_ctx *ctx, double *total);
  * counterpart for mode 1. */
 int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
 
-/* Evaluation tiles: grid points per lane (1, 2, 4, 6 or 8) of the FP64 and
+/* Evaluation tiles: grid points per lane (1, 2, 4, 6, 8 or 10) of the FP64 and
  * of the FP32 kernel (on radial scans and cube grids, the FP64 kernel's
  * lanes are rays of that many points).  p32 = 0 runs the FP32 range of a
  * mixed evaluation on the FP64 ray tiles instead, where the grid has them

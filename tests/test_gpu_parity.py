@@ -328,13 +328,13 @@ def test_ray_tiles_match_point_by_point(engine):
     engine.grid_radial_plot(tart, [0.1, 0.2], [110, 302])
     out = {}
     try:
-        for P in (1, 2, 4, 6, 8):
+        for P in (1, 2, 4, 6, 8, 10):
             engine.set_tiles(P, 4)
             iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
             out[P] = (iv.copy(), engine.radial_values().copy())
     finally:
         engine.set_tiles(4, 4)
-    for P in (2, 4, 6, 8):
+    for P in (2, 4, 6, 8, 10):
         er, ea = _close(out[P][0], out[1][0])
         rr, ra = _close(out[P][1], out[1][1])
         print("P=%d rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e" % (
@@ -360,7 +360,7 @@ def test_ray_tiles_on_cube_lines(engine):
     try:
         for cut in (100, 0):
             engine.set_point_cutoff(cut)
-            for P in (1, 4, 6, 8):
+            for P in (1, 4, 6, 8, 10):
                 engine.set_tiles(P, 4)
                 iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
                 out[(cut, P)] = iv.copy()
@@ -368,7 +368,7 @@ def test_ray_tiles_on_cube_lines(engine):
         engine.set_tiles(4, 4)
         engine.set_point_cutoff(100)
     for cut in (100, 0):
-        for P in (4, 6, 8):
+        for P in (4, 6, 8, 10):
             er, ea = _close(out[(cut, P)], out[(cut, 1)])
             print("cube lines, cutoff %d, P=%d vs point by point: rel %.2e abs %.2e" % (
                 cut, P, er, ea))
