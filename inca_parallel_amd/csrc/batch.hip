@@ -4,12 +4,16 @@
 // calculate_intracule).  A batch takes contexts whose grids and quartets are
 // already prepared (inca_prepare) and lays them out in one HBM arena:
 //   * each molecule's quartets, stably sorted by angular-momentum class
-//     (Lsum, Lx, Ly, Lz), so that every work item holds quartets of a
-//     single class and its cost is known in advance;
+//     (Lsum, Lz, Ly, Lx; the FP32 range of a mixed preparation after the
+//     FP64 one), so that every work item holds quartets of a single class
+//     and its cost is known in advance.  One device radix sort over
+//     (molecule, precision, class) keys orders the whole batch, and a
+//     run-length encoding of the sorted keys gives the class segments;
 //   * all grid points back to back, with per-molecule offsets.
 // Work items are (molecule, block tile of 256 P points, class chunk of at most
 // 4096 quartets).  All items of one chunk go to one of 8 queues (chunk mod
-// 8), each ordered by estimated cost, largest first; a persistent grid of
+// 8), each ordered by the chunks' estimated cost, largest first, and
+// expanded on the device from the host's chunk table; a persistent grid of
 // workgroups pulls from the queues through atomic counters, block L serving
 // the queue of its own XCD (L mod 8, the round-robin dispatch) first and
 // the others once it is empty, so a chunk's records are read into one L2 and
@@ -31,6 +35,9 @@
 #include <numeric>
 #include <string>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
 
 #include "common.hpp"
 #include "eval_core.hpp"
@@ -54,6 +61,8 @@ struct inca_batch {
   size_t partial_cap = 0;
   int64_t *d_items = nullptr;
   size_t items_cap = 0;
+  int64_t *d_chunks = nullptr;  // chunk table the items are expanded from
+  size_t chunks_cap = 0;
   int64_t *d_mol = nullptr;   // per molecule: kMol words (see build_items)
   int32_t *d_pmol = nullptr;  // molecule of every point
   // per-point cutoff (when every context had it on): tile anchors come from
@@ -239,11 +248,44 @@ __global__ __launch_bounds__(kT) void k_batch_gather(const double *__restrict__ 
   dst[t] = src[perm[r] * stride + w];
 }
 
-// Class words (header word 4) of n records into a contiguous array.
-__global__ __launch_bounds__(kT) void k_class_words(const double *__restrict__ q, int64_t n,
-                                                    int stride, double *__restrict__ out) {
+// Sort key of each of the n prepared records of molecule `mol` and its
+// index: molecule, then the FP32 range of a mixed preparation (records from
+// nfp64 on) after the FP64 one, then Lsum, Lz, Ly, Lx (header word 4).
+constexpr int kKeyMolShift = 41;
+__global__ __launch_bounds__(kT) void k_class_keys(const double *__restrict__ q, int64_t n,
+                                                   int stride, int64_t nfp64, uint64_t mol,
+                                                   uint64_t *__restrict__ key,
+                                                   int64_t *__restrict__ idx) {
   const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (r < n) out[r] = q[r * stride + 4];
+  if (r >= n) return;
+  const uint64_t bits = (uint64_t)__double_as_longlong(q[r * stride + 4]);
+  const uint64_t lsum = (bits & 0xff) + ((bits >> 8) & 0xff) + ((bits >> 16) & 0xff);
+  const uint64_t f32 = r >= nfp64 ? 1u : 0u;
+  key[r] = (mol << kKeyMolShift) | (f32 << 40) | (lsum << 24) | (bits & 0xffffff);
+  idx[r] = r;
+}
+
+// Work items from the chunk table: row c (kChunkWords words: mol, qoff, n,
+// stride, partial, local q, precision, first item, tiles) gives one item per
+// block tile, items first .. first + tiles - 1.
+constexpr int kChunkWords = 9;
+__global__ __launch_bounds__(kT) void k_expand_items(const int64_t *__restrict__ tab,
+                                                     int64_t nchunk, int64_t *__restrict__ items) {
+  const int64_t c = blockIdx.x;
+  if (c >= nchunk) return;
+  const int64_t *__restrict__ r = tab + c * kChunkWords;
+  const int64_t first = r[7], tiles = r[8];
+  for (int64_t t = threadIdx.x; t < tiles; t += kT) {
+    int64_t *__restrict__ it = items + (first + t) * kItem;
+    it[0] = r[0];
+    it[1] = t;
+    it[2] = r[1];
+    it[3] = r[2];
+    it[4] = r[3];
+    it[5] = r[4];
+    it[6] = r[5];
+    it[7] = r[6];
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -266,8 +308,11 @@ int bensure(inca_batch *b, T *&ptr, size_t &cap, size_t n) {
 }
 
 // Items for points-per-lane P: chunks of at most qchunk quartets that never
-// straddle a class boundary, one item per (chunk, block tile), ordered by
-// estimated cost (quartets x (16 + 2 Lsum) x points), largest first.
+// straddle a class boundary, one item per (chunk, block tile).  Each XCD
+// queue takes every 8th chunk; its chunks are ordered by estimated cost
+// (quartets x (16 + 2 Lsum) x points of a full tile, FP32 at half), largest
+// first, and a chunk's items stay together in tile order.  Only the chunk
+// table is built on the host; the items are expanded on the device.
 int build_items(inca_batch *b, int P) {
   const int64_t tile_pts = (int64_t)kT * P;
   double work = 0.0;
@@ -279,11 +324,12 @@ int build_items(inca_batch *b, int P) {
   const double target = 16.0 * std::max(1, b->nblocks);
   int64_t qchunk = (int64_t)std::ceil(work / target);
   qchunk = std::max<int64_t>(32, std::min<int64_t>(qchunk, kMaxChunk));
-  struct It {
-    std::array<int64_t, kItem> w;
+  struct Chunk {
+    int64_t m, qo, n, part, lq, f32, tiles;
     double cost;
   };
-  std::vector<It> queue[kXcds];  // a chunk's items share one XCD queue
+  std::vector<Chunk> ch;
+  std::vector<int64_t> order[kXcds];  // a chunk's items share one XCD queue
   int64_t part = 0, chunk_id = 0;
   for (int m = 0; m < b->nmol; ++m) {
     const int64_t tiles = (b->G[m] + tile_pts - 1) / tile_pts;
@@ -291,18 +337,15 @@ int build_items(inca_batch *b, int P) {
     int64_t c = 0;
     for (const auto &sg : b->segs[m]) {
       const int lsum = (int)((sg[2] >> 24) & 0xffff);
-      const int64_t f32 = sg[2] >> 40;
+      const int64_t f32 = (sg[2] >> 40) & 1;
       for (int64_t s = 0; s < sg[1]; s += qchunk, ++c, ++chunk_id) {
         const int64_t n = std::min(qchunk, sg[1] - s);
         const int64_t qo = b->q_off[m] + (sg[0] + s) * b->stride[m];
-        for (int64_t t = 0; t < tiles; ++t) {
-          const int64_t pts = std::min(tile_pts, b->G[m] - t * tile_pts);
-          It it;
-          it.w = {m, t, qo, n, b->stride[m], part + c * b->G[m], sg[0] + s, f32};
-          // FP32 terms cost about half of the FP64 ones
-          it.cost = (double)n * (16.0 + 2.0 * lsum) * (double)pts * (f32 ? 0.5 : 1.0);
-          queue[chunk_id % kXcds].push_back(it);
-        }
+        // FP32 terms cost about half of the FP64 ones
+        const double cost = (double)n * (16.0 + 2.0 * lsum) *
+                            (double)std::min(tile_pts, b->G[m]) * (f32 ? 0.5 : 1.0);
+        order[chunk_id % kXcds].push_back((int64_t)ch.size());
+        ch.push_back({m, qo, n, part + c * b->G[m], sg[0] + s, f32, tiles, cost});
       }
     }
     b->nch[m] = c;
@@ -310,25 +353,35 @@ int build_items(inca_batch *b, int P) {
   }
   b->nitems = 0;
   int64_t qinfo[2 * kXcds];
+  std::vector<int64_t> tab(std::max<size_t>(1, ch.size()) * kChunkWords);
+  size_t row = 0;
   for (int x = 0; x < kXcds; ++x) {
-    std::stable_sort(queue[x].begin(), queue[x].end(),
-                     [](const It &a, const It &c) { return a.cost > c.cost; });
+    std::stable_sort(order[x].begin(), order[x].end(),
+                     [&](int64_t a, int64_t c) { return ch[a].cost > ch[c].cost; });
     qinfo[2 * x] = b->nitems;
-    qinfo[2 * x + 1] = (int64_t)queue[x].size();
-    b->nitems += (int64_t)queue[x].size();
-  }
-  std::vector<int64_t> flat((size_t)std::max<int64_t>(1, b->nitems) * kItem);
-  size_t i = 0;
-  for (int x = 0; x < kXcds; ++x)
-    for (const It &it : queue[x]) {
-      std::copy(it.w.begin(), it.w.end(), flat.begin() + i * kItem);
-      ++i;
+    for (int64_t ci : order[x]) {
+      const Chunk &k = ch[ci];
+      const int64_t w[kChunkWords] = {k.m,   k.qo,  k.n,         b->stride[k.m], k.part,
+                                      k.lq,  k.f32, b->nitems,   k.tiles};
+      std::copy(w, w + kChunkWords, tab.begin() + row * kChunkWords);
+      ++row;
+      b->nitems += k.tiles;
     }
+    qinfo[2 * x + 1] = b->nitems - qinfo[2 * x];
+  }
   int rc;
-  if ((rc = bensure(b, b->d_items, b->items_cap, flat.size()))) return rc;
+  if ((rc = bensure(b, b->d_items, b->items_cap,
+                    (size_t)std::max<int64_t>(1, b->nitems) * kItem)) ||
+      (rc = bensure(b, b->d_chunks, b->chunks_cap, tab.size())))
+    return rc;
   B_HIP(hipMemcpy(b->d_queues, qinfo, sizeof(qinfo), hipMemcpyHostToDevice));
   if ((rc = bensure(b, b->d_partial, b->partial_cap, (size_t)std::max<int64_t>(1, part)))) return rc;
-  B_HIP(hipMemcpy(b->d_items, flat.data(), flat.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  B_HIP(hipMemcpy(b->d_chunks, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  if (!ch.empty()) {
+    hipLaunchKernelGGL(k_expand_items, dim3((unsigned)ch.size()), dim3(kT), 0, b->stream,
+                       b->d_chunks, (int64_t)ch.size(), b->d_items);
+    B_HIP(hipGetLastError());
+  }
   // cutoff masks for this P: wave tiles of 64 P points per molecule
   int64_t tb_total = 0, mask_total = 0;
   for (int m = 0; m < b->nmol; ++m) {
@@ -434,6 +487,22 @@ int capture(inca_batch *b) {
   return 0;
 }
 
+// Device buffers freed on every return (hipFree waits for the device).
+struct DeviceTemps {
+  std::vector<void *> p;
+  ~DeviceTemps() {
+    for (void *q : p)
+      if (q) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t get(T *&ptr, size_t n) {
+    ptr = nullptr;
+    hipError_t e = hipMalloc((void **)&ptr, std::max<size_t>(1, n) * sizeof(T));
+    if (e == hipSuccess) p.push_back(ptr);
+    return e;
+  }
+};
+
 int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   b->nmol = nmol;
   b->G.assign(nmol, 0);
@@ -488,21 +557,19 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   if ((cap = 0, rc = bensure(b, b->d_pmap, cap, gt))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_cent, cap, 3 * (size_t)std::max<int64_t>(1, ncent)))) return rc;
   hipStream_t st = b->stream;
-  int64_t *d_perm = nullptr;
-  size_t perm_cap = 0;
-  double *d_cls = nullptr;  // class words of one molecule
-  size_t cls_cap = 0;
-  struct FreeOnReturn {  // on every return (hipFree waits for the device)
-    int64_t *&perm;
-    double *&cls;
-    ~FreeOnReturn() {
-      if (perm) (void)hipFree(perm);
-      if (cls) (void)hipFree(cls);
-    }
-  } free_on_return{d_perm, d_cls};
+  DeviceTemps tmp;  // temporaries of the class sort
+  int64_t Qtot = 0;
+  std::vector<int64_t> qstart(nmol, 0);
+  for (int m = 0; m < nmol; ++m) {
+    qstart[m] = Qtot;
+    Qtot += b->nq[m];
+    B_HIP(hipStreamSynchronize(mols[m]->stream));  // the context's own work is done
+  }
+  if (Qtot > (int64_t)UINT32_MAX)
+    return bfail(b, INCA_EINVAL, "a batch holds at most 2^32 - 1 prepared quartets");
   for (int m = 0; m < nmol; ++m) {
     const inca_ctx *c = mols[m];
-    const int64_t G = c->G, n = c->q2;
+    const int64_t G = c->G;
     if (G > 0) {
       const double *src[5] = {c->d_gx, c->d_gy, c->d_gz, c->d_w, c->d_wvee};
       double *dst[5] = {b->d_gx, b->d_gy, b->d_gz, b->d_w, b->d_wvee};
@@ -519,39 +586,72 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
                              hipMemcpyDeviceToDevice, st));
       }
     }
-    if (n == 0) continue;
-    // class of every prepared record (header word 4), sorted stably on the
-    // host; the records of a mixed preparation keep their FP64 range ahead of
-    // the FP32 one (bit 40 of the key)
-    std::vector<double> cls((size_t)n);
-    B_HIP(hipStreamSynchronize(c->stream));
-    if ((rc = bensure(b, d_cls, cls_cap, (size_t)n))) return rc;
-    hipLaunchKernelGGL(k_class_words, dim3(nblk(n)), dim3(kT), 0, st, c->d_q, n, c->qstride, d_cls);
+  }
+  if (Qtot > 0) {
+    // Every prepared record's (molecule, precision, class) key, one stable
+    // radix sort over the batch, and the runs of equal keys: each molecule's
+    // records in class order, the FP32 range of a mixed preparation after
+    // the FP64 one, with the file order kept within a class.
+    uint64_t *d_kin, *d_kout, *d_uniq;
+    int64_t *d_vin, *d_vout;
+    unsigned int *d_cnt, *d_nruns;
+    B_HIP(tmp.get(d_kin, Qtot));
+    B_HIP(tmp.get(d_kout, Qtot));
+    B_HIP(tmp.get(d_vin, Qtot));
+    B_HIP(tmp.get(d_vout, Qtot));
+    B_HIP(tmp.get(d_uniq, Qtot));
+    B_HIP(tmp.get(d_cnt, Qtot));
+    B_HIP(tmp.get(d_nruns, 1));
+    for (int m = 0; m < nmol; ++m) {
+      const inca_ctx *c = mols[m];
+      const int64_t n = c->q2;
+      if (n == 0) continue;
+      const int64_t nfp64 = c->prep_precision == INCA_MIXED ? c->q_fp64 : n;
+      hipLaunchKernelGGL(k_class_keys, dim3(nblk(n)), dim3(kT), 0, st, c->d_q, n, c->qstride,
+                         nfp64, (uint64_t)m, d_kin + qstart[m], d_vin + qstart[m]);
+    }
     B_HIP(hipGetLastError());
-    B_HIP(hipMemcpyAsync(cls.data(), d_cls, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    unsigned int end_bit = kKeyMolShift + 1;
+    while (end_bit < 64 && ((uint64_t)(nmol - 1) >> (end_bit - kKeyMolShift)) != 0) ++end_bit;
+    size_t sort_bytes = 0, rle_bytes = 0;
+    B_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_kin, d_kout, d_vin, d_vout,
+                                    (size_t)Qtot, 0u, end_bit, st));
+    B_HIP(rocprim::run_length_encode(nullptr, rle_bytes, d_kout, (unsigned int)Qtot, d_uniq,
+                                     d_cnt, d_nruns, st));
+    char *d_temp;
+    B_HIP(tmp.get(d_temp, std::max(sort_bytes, rle_bytes)));
+    B_HIP(rocprim::radix_sort_pairs(d_temp, sort_bytes, d_kin, d_kout, d_vin, d_vout,
+                                    (size_t)Qtot, 0u, end_bit, st));
+    B_HIP(rocprim::run_length_encode(d_temp, rle_bytes, d_kout, (unsigned int)Qtot, d_uniq,
+                                     d_cnt, d_nruns, st));
+    for (int m = 0; m < nmol; ++m) {
+      const inca_ctx *c = mols[m];
+      const int64_t n = c->q2;
+      if (n == 0) continue;
+      hipLaunchKernelGGL(k_batch_gather, dim3(nblk(n * c->qstride)), dim3(kT), 0, st, c->d_q,
+                         d_vout + qstart[m], n, c->qstride, b->d_q + b->q_off[m]);
+    }
+    B_HIP(hipGetLastError());
+    unsigned int nruns = 0;
+    B_HIP(hipMemcpyAsync(&nruns, d_nruns, sizeof(nruns), hipMemcpyDeviceToHost, st));
     B_HIP(hipStreamSynchronize(st));
-    std::vector<int64_t> key((size_t)n), perm((size_t)n);
-    for (int64_t r = 0; r < n; ++r) {
-      uint64_t bits;
-      std::memcpy(&bits, &cls[r], sizeof(bits));
-      const int64_t lx = bits & 0xff, ly = (bits >> 8) & 0xff, lz = (bits >> 16) & 0xff;
-      const int64_t f32 = (c->prep_precision == INCA_MIXED && r >= c->q_fp64) ? 1 : 0;
-      key[r] = (f32 << 40) | ((lx + ly + lz) << 24) | (int64_t)(bits & 0xffffff);
+    std::vector<uint64_t> uniq(nruns);
+    std::vector<unsigned int> cnt(nruns);
+    B_HIP(hipMemcpyAsync(uniq.data(), d_uniq, nruns * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    B_HIP(hipMemcpyAsync(cnt.data(), d_cnt, nruns * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                         st));
+    B_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> at(nmol, 0);  // next local record of each molecule
+    for (unsigned int r = 0; r < nruns; ++r) {
+      const int m = (int)(uniq[r] >> kKeyMolShift);
+      const int64_t key = (int64_t)(uniq[r] & ((1ull << kKeyMolShift) - 1));
+      if (m < 0 || m >= nmol || at[m] + (int64_t)cnt[r] > b->nq[m])
+        return bfail(b, INCA_EHIP, "class sort: inconsistent runs");
+      b->segs[m].push_back({at[m], (int64_t)cnt[r], key});
+      at[m] += cnt[r];
     }
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t c2) { return key[a] < key[c2]; });
-    for (int64_t r = 0; r < n;) {
-      int64_t e = r;
-      while (e < n && key[perm[e]] == key[perm[r]]) ++e;
-      b->segs[m].push_back({r, e - r, key[perm[r]]});
-      r = e;
-    }
-    if ((rc = bensure(b, d_perm, perm_cap, (size_t)n))) return rc;
-    B_HIP(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_batch_gather, dim3(nblk(n * c->qstride)), dim3(kT), 0, st, c->d_q, d_perm,
-                       n, c->qstride, b->d_q + b->q_off[m]);
-    B_HIP(hipGetLastError());
-    B_HIP(hipStreamSynchronize(st));  // d_perm is reused by the next molecule
+    for (int m = 0; m < nmol; ++m)
+      if (at[m] != b->nq[m]) return bfail(b, INCA_EHIP, "class sort: inconsistent runs");
   }
   B_HIP(hipStreamSynchronize(st));
   if ((rc = persistent_blocks(b, b->P))) return rc;
@@ -591,7 +691,7 @@ int inca_batch_destroy(inca_batch *b) {
   void *bufs[] = {b->d_q,     b->d_gx,    b->d_gy,   b->d_gz,    b->d_w,
                   b->d_wvee,  b->d_ivec,  b->d_res,  b->d_partial, b->d_items,
                   b->d_mol,   b->d_pmol,  b->d_counter, b->d_pmap, b->d_cent,
-                  b->d_tb,    b->d_mask,  b->d_queues};
+                  b->d_tb,    b->d_mask,  b->d_queues, b->d_chunks};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : b->ev)
