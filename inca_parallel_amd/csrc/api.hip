@@ -77,7 +77,8 @@ int inca_destroy(inca_ctx *c) {
   void *bufs[] = {c->d_prim, c->d_tmn, c->d_pair, c->d_gx, c->d_gy, c->d_gz, c->d_w,
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
-                  c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup};
+                  c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup,
+                  c->d_red};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)

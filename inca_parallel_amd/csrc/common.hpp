@@ -128,6 +128,8 @@ struct inca_ctx {
   double *d_gcent = nullptr;  // multicentre grid: centre coordinates (3*nquad)
   double *d_partial = nullptr;
   size_t partial_cap = 0;
+  double *d_red = nullptr;  // slice sums of the chunk partials
+  size_t red_cap = 0;
   double *d_ivec = nullptr;
   size_t ivec_cap = 0;
   int have_ivec = 0;

@@ -179,6 +179,22 @@ __global__ __launch_bounds__(kT) void k_eval_mixed(const double *__restrict__ q,
   }
 }
 
+// First stage of the chunk reduction when there are many chunks: slice s
+// (blockIdx.y) adds chunks [nchunks s / nslice, nchunks (s + 1) / nslice)
+// of every point in order; k_reduce then adds the slices in order.
+__global__ __launch_bounds__(kT) void k_reduce_slices(const double *__restrict__ partial,
+                                                      int nchunks, int nslice, int64_t G,
+                                                      double *__restrict__ red) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= G) return;
+  const int s = blockIdx.y;
+  const int c0 = (int)((int64_t)nchunks * s / nslice);
+  const int c1 = (int)((int64_t)nchunks * (s + 1) / nslice);
+  double acc = 0.0;
+  for (int c = c0; c < c1; ++c) acc += partial[(int64_t)c * G + g];
+  red[(int64_t)s * G + g] = acc;
+}
+
 __global__ __launch_bounds__(kT) void k_reduce(const double *__restrict__ partial, int nchunks,
                                                int64_t G, double *__restrict__ ivec) {
   const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
@@ -493,6 +509,10 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
       return rc;
     if ((rc = ensure(ctx, ctx->d_wcount, ctx->wcount_cap, (size_t)(nwords > 0 ? nwords : 1))))
       return rc;
+    // the executed-evaluation counter is added to here; the grid loop clears
+    // it first, and autotune launches (which may come before any grid loop)
+    // only need it to exist
+    if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
     if (nwords > 0) {
       hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((nwords + 3) / 4)), dim3(kT), 0, st, q,
                          ctx->qstride, n, ctx->d_tb, ntile, nwords, ctx->d_mask, ctx->d_wcount);
@@ -625,8 +645,25 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
                         nch[1], ctx->d_partial + nch[0] * G)))
     return rc;
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
-  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st,
-                     ctx->d_partial, (int)nchunks, G, ctx->d_ivec);
+  {
+    // Chunk partials to Ivec in a fixed order.  With few points per chunk
+    // row, one thread per point would leave most CUs idle, so the chunks are
+    // first summed in slices (about 2048 blocks in all).
+    const int64_t pblk = (G + kT - 1) / kT;
+    int64_t nslice = std::min<int64_t>(nchunks, std::max<int64_t>(1, 2048 / pblk));
+    if (nslice > 65535) nslice = 65535;
+    const double *src = ctx->d_partial;
+    int nsum = (int)nchunks;
+    if (nslice > 1 && nchunks >= 4 * nslice) {
+      if ((rc = ensure(ctx, ctx->d_red, ctx->red_cap, (size_t)(nslice * G)))) return rc;
+      hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)pblk, (unsigned)nslice), dim3(kT), 0, st,
+                         ctx->d_partial, (int)nchunks, (int)nslice, G, ctx->d_red);
+      src = ctx->d_red;
+      nsum = (int)nslice;
+    }
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)pblk), dim3(kT), 0, st, src, nsum, G,
+                       ctx->d_ivec);
+  }
   if (rays_used && ctx->ndup > 0)
     hipLaunchKernelGGL(k_copy_dups, dim3((unsigned)((ctx->ndup + kT - 1) / kT)), dim3(kT), 0, st,
                        ctx->d_dup, ctx->ndup, ctx->d_ivec);

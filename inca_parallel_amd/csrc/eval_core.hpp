@@ -252,29 +252,64 @@ __device__ __forceinline__ double horner1(const double *__restrict__ c, int L, d
 }
 
 // One axis polynomial (degree L >= 1, c[0] the highest power) at the ray
-// points p_k = p0 + k d.  Degree 1 is written in k itself, V_k = V_0 + k c0 d
-// (one FMA per point instead of forming p_k and then Horner); higher
-// degrees use Horner on p_k.  The first point is p0 itself.
-template <int P>
+// points p_k = p0 + k d, k = k0 .. k0 + N - 1.  Degree 1 is written in k
+// itself, V_k = V_0 + k c0 d (one FMA per point instead of forming p_k and
+// then Horner); higher degrees use Horner on p_k.  Point 0 is p0 itself.
+template <int N>
 __device__ __forceinline__ void axis_ray(const double *__restrict__ c, int L, double p0, double d,
-                                         double (&v)[P]) {
+                                         int k0, double (&v)[N]) {
   if (L == 1) {
     const double c0 = c[0];
     const double b1 = c0 * d;
-    v[0] = fma(p0, c0, c[1]);
+    const double v0 = fma(p0, c0, c[1]);
 #pragma unroll
-    for (int k = 1; k < P; ++k) v[k] = fma((double)k, b1, v[0]);
+    for (int k = 0; k < N; ++k) v[k] = (k0 + k == 0) ? v0 : fma((double)(k0 + k), b1, v0);
     return;
   }
-  double p[P];
-  p[0] = p0;
+  double p[N];
 #pragma unroll
-  for (int k = 1; k < P; ++k) p[k] = fma((double)k, d, p0);
-  horner<P>(c, L, p, v);
+  for (int k = 0; k < N; ++k) p[k] = (k0 + k == 0) ? p0 : fma((double)(k0 + k), d, p0);
+  horner<N>(c, L, p, v);
 }
 
-// ZLINE: the lane runs along z (cube lines), so rp_x and rp_y are the same
-// at all its points and their polynomials are evaluated once per lane.
+// Product of the axis polynomials with L > 0 (MX, MY, MZ) at the N lane
+// points k0 .. k0 + N - 1, one axis at a time.  ZLINE: only z varies along
+// the lane.
+template <int N, bool ZLINE, bool MX, bool MY, bool MZ>
+__device__ __forceinline__ void ray_poly(const double *__restrict__ cx,
+                                         const double *__restrict__ cy,
+                                         const double *__restrict__ cz, const QHeader &h,
+                                         double p0x, double p0y, double p0z, double dx, double dy,
+                                         double dz, int k0, double (&w)[N]) {
+  double v[N];
+  if (ZLINE) {
+    if (MZ) axis_ray<N>(cz, h.Lz, p0z, dz, k0, w);
+    return;
+  }
+  if (MX) axis_ray<N>(cx, h.Lx, p0x, dx, k0, w);
+  if (MY) {
+    axis_ray<N>(cy, h.Ly, p0y, dy, k0, v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      w[k] = MX ? w[k] * v[k] : v[k];
+      // keep the product here: sunk past the z axis, it holds x, y and z
+      // values live together and spills at 8 points per lane
+      if (MX && MZ) asm volatile("" : "+v"(w[k]));
+    }
+  }
+  if (MZ) {
+    axis_ray<N>(cz, h.Lz, p0z, dz, k0, v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
+  }
+}
+
+// One quartet on a lane of P points o + k h w (k = 0 .. P-1) whose axes
+// with L > 0 are exactly those flagged MX, MY, MZ.  exp(-s_k) follows the
+// two-product recurrence e_{k+1} = e_k q_k, q_{k+1} = q_k rho from two
+// exponentials at the lane's start (rho = exp(-2 sqe^2 h^2), record word 7).
+// ZLINE: the lane runs along z (a cube line), so the x and y polynomials
+// are constant and fold into the uniform factor S.
 template <int P, bool ZLINE, bool MX, bool MY, bool MZ>
 __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const QHeader &h,
                                          const double (&o)[3], const double (&hw)[3], double h2,
@@ -290,7 +325,6 @@ __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const Q
   if (!MX) S = cx[0];
   if (!MY) S = MX ? cy[0] : S * cy[0];
   if (!MZ) S = (MX && MY) ? cz[0] : S * cz[0];
-  double w[P], v[P];
   double dot, dx = 0.0, dy = 0.0, dz;
   constexpr bool perpoint = ZLINE ? MZ : (MX || MY || MZ);  // w varies along the lane
   if (ZLINE) {
@@ -298,31 +332,15 @@ __device__ __forceinline__ void ray_case(const double *__restrict__ rec, const Q
     if (MX) S *= horner1(cx, h.Lx, p0x);
     if (MY) S *= horner1(cy, h.Ly, p0y);
     dz = h.sqe * hw[2];
-    if (MZ) axis_ray<P>(cz, h.Lz, p0z, dz, w);
     dot = dz * p0z;
   } else {
     dx = h.sqe * hw[0];
     dy = h.sqe * hw[1];
     dz = h.sqe * hw[2];
-    // product of the axis polynomials at the P points, one axis at a time
-    if (MX) axis_ray<P>(cx, h.Lx, p0x, dx, w);
-    if (MY) {
-      axis_ray<P>(cy, h.Ly, p0y, dy, v);
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        w[k] = MX ? w[k] * v[k] : v[k];
-        // keep the product here: sunk past the z axis, it holds x, y and z
-        // values live together and spills at 8 points per lane
-        if (MX && MZ) asm volatile("" : "+v"(w[k]));
-      }
-    }
-    if (MZ) {
-      axis_ray<P>(cz, h.Lz, p0z, dz, v);
-#pragma unroll
-      for (int k = 0; k < P; ++k) w[k] = (MX || MY) ? w[k] * v[k] : v[k];
-    }
     dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
   }
+  double w[P];
+  ray_poly<P, ZLINE, MX, MY, MZ>(cx, cy, cz, h, p0x, p0y, p0z, dx, dy, dz, 0, w);
   const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
   const double dd = h.sqe * h.sqe * h2;  // |d|^2 = (sqe h)^2
   const double rho = rec[7];

@@ -344,6 +344,31 @@ def test_ray_tiles_match_point_by_point(engine):
         assert rr < 1e-11 and ra < 1e-13
 
 
+def test_autotune_before_any_grid_loop():
+    """inca_autotune straight after inca_prepare on a new context (no grid
+    loop has run yet, so no buffer of one exists), on a cube grid with the
+    cutoff on; the tuned choice then evaluates like a fixed one."""
+    s = synth.config_cluster(seed=34, natoms=3, nheavy=2, nocc=3,
+                             shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
+    ijkl, d = s.dm2_unique()
+    p = s.prims()
+    e = capi.Engine(0)
+    try:
+        e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+        e.set_quartets(ijkl=ijkl, d=d)
+        e.grid_cube([0.0, 0.0, 0.0], [0.2, 0.2, 0.2], [8, 8, 32])
+        e.prepare(capi.INCA_FP64, THRESH)
+        best = e.autotune()
+        assert best[0] in (1, 2, 4, 6, 8, 10) and best[1] in (0, 1, 2, 4, 6, 8, 10)
+        tuned = e.evaluate_prepared(capi.INCA_FP64, want_ivec=True).copy()
+        e.set_tiles(1, 4)
+        ref = e.evaluate_prepared(capi.INCA_FP64, want_ivec=True)
+        er, ea = _close(tuned, ref)
+        assert er < 1e-11 and ea < 1e-13
+    finally:
+        e.close()
+
+
 def test_ray_tiles_on_cube_lines(engine):
     """Cube grids run on the same ray kernel, each line along z cut into
     lanes of P consecutive points (80 points per line, so the last lane of
