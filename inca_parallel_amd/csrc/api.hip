@@ -78,7 +78,7 @@ int inca_destroy(inca_ctx *c) {
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
                   c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup,
-                  c->d_red};
+                  c->d_red, c->d_tmp};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)

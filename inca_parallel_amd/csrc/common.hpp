@@ -130,6 +130,11 @@ struct inca_ctx {
   size_t partial_cap = 0;
   double *d_red = nullptr;  // slice sums of the chunk partials
   size_t red_cap = 0;
+  // device scratch of one API call (every call that uses it synchronises
+  // before returning), and the capacities of the grid arrays
+  char *d_tmp = nullptr;
+  size_t tmp_cap = 0;
+  size_t grid_cap = 0, gcent_cap = 0;
   double *d_ivec = nullptr;
   size_t ivec_cap = 0;
   int have_ivec = 0;
@@ -166,6 +171,22 @@ int ensure(inca_ctx *ctx, T *&ptr, size_t &cap, size_t n) {
   cap = n;
   return 0;
 }
+
+// Offsets of sub-buffers carved from one scratch block, 256-byte aligned:
+// first add() every buffer, then ensure the block and take at().
+struct Carve {
+  size_t size = 0;
+  template <class T>
+  size_t add(int64_t n) {
+    const size_t o = size;
+    size += ((size_t)(n > 0 ? n : 1) * sizeof(T) + 255) & ~(size_t)255;
+    return o;
+  }
+  template <class T>
+  static T *at(char *base, size_t off) {
+    return reinterpret_cast<T *>(base + off);
+  }
+};
 
 // Deterministic stream compaction of a 0/1 byte flag array:
 // idx[0..total) = positions of set flags, in increasing order.

@@ -162,25 +162,29 @@ struct Scratch {
   }
 };
 
+// Grid arrays for G points; arrays large enough from an earlier grid are
+// kept.
 int alloc_grid(inca_ctx *ctx, int64_t G) {
   ++ctx->grid_serial;
+  const size_t n = (size_t)(G > 0 ? G : 1);
   double **arrs[5] = {&ctx->d_gx, &ctx->d_gy, &ctx->d_gz, &ctx->d_w,
                       &ctx->d_wvee};
-  for (double **a : arrs) {
-    if (*a) (void)hipFree(*a);
-    *a = nullptr;
+  if (n > ctx->grid_cap || !ctx->d_gx || !ctx->d_map) {
+    for (double **a : arrs) {
+      if (*a) (void)hipFree(*a);
+      *a = nullptr;
+    }
+    if (ctx->d_map) (void)hipFree(ctx->d_map);
+    ctx->d_map = nullptr;
+    ctx->grid_cap = 0;
+    for (double **a : arrs)
+      if (hipMalloc((void **)a, n * sizeof(double)) != hipSuccess)
+        return set_err(ctx, INCA_ENOMEM, "grid allocation of %lld points failed",
+                       (long long)G);
+    if (hipMalloc((void **)&ctx->d_map, n * sizeof(int32_t)) != hipSuccess)
+      return set_err(ctx, INCA_ENOMEM, "grid allocation failed");
+    ctx->grid_cap = n;
   }
-  if (ctx->d_map) (void)hipFree(ctx->d_map);
-  ctx->d_map = nullptr;
-  if (ctx->d_gcent) (void)hipFree(ctx->d_gcent);
-  ctx->d_gcent = nullptr;
-  size_t n = (size_t)(G > 0 ? G : 1);
-  for (double **a : arrs)
-    if (hipMalloc((void **)a, n * sizeof(double)) != hipSuccess)
-      return set_err(ctx, INCA_ENOMEM, "grid allocation of %lld points failed",
-                     (long long)G);
-  if (hipMalloc((void **)&ctx->d_map, n * sizeof(int32_t)) != hipSuccess)
-    return set_err(ctx, INCA_ENOMEM, "grid allocation failed");
   ctx->G = G;
   ctx->have_ivec = 0;
   return 0;
@@ -225,23 +229,31 @@ int grid_multicenter(inca_ctx *ctx, int nquad, const double *cent,
     np += (int64_t)nrad * nang;
   }
   *maxgrid = np;
-  Scratch S;
-  CentreInfo *d_ci;
-  double *d_rad, *d_wl, *rx, *ry, *rz, *rw, *d_cent, *d_R, *d_X;
-  double4 *d_ang;
-  int32_t *rown;
-  uint8_t *fl;
-  int64_t *idx;
+  // every temporary from the context's scratch block (the symmetry-kept
+  // arrays at their upper bound np); the call synchronises before returning
+  Carve cv;
+  const size_t o_ci = cv.add<CentreInfo>(nquad), o_rad = cv.add<double>(rad.size()),
+               o_wl = cv.add<double>(wl.size()), o_ang = cv.add<double4>(ang.size()),
+               o_rx = cv.add<double>(np), o_ry = cv.add<double>(np), o_rz = cv.add<double>(np),
+               o_rw = cv.add<double>(np), o_rown = cv.add<int32_t>(np),
+               o_fl = cv.add<uint8_t>(np), o_idx = cv.add<int64_t>(np),
+               o_cent = cv.add<double>(3 * nquad), o_R = cv.add<double>((int64_t)nquad * nquad),
+               o_X = cv.add<double>((int64_t)nquad * nquad), o_kx = cv.add<double>(np),
+               o_ky = cv.add<double>(np), o_kz = cv.add<double>(np), o_kw = cv.add<double>(np),
+               o_kown = cv.add<int32_t>(np);
   int rc = 0;
-  if ((rc = S.alloc(ctx, d_ci, nquad)) || (rc = S.alloc(ctx, d_rad, rad.size())) ||
-      (rc = S.alloc(ctx, d_wl, wl.size())) || (rc = S.alloc(ctx, d_ang, ang.size())) ||
-      (rc = S.alloc(ctx, rx, np)) || (rc = S.alloc(ctx, ry, np)) ||
-      (rc = S.alloc(ctx, rz, np)) || (rc = S.alloc(ctx, rw, np)) ||
-      (rc = S.alloc(ctx, rown, np)) || (rc = S.alloc(ctx, fl, np)) ||
-      (rc = S.alloc(ctx, idx, np)) || (rc = S.alloc(ctx, d_cent, 3 * nquad)) ||
-      (rc = S.alloc(ctx, d_R, (int64_t)nquad * nquad)) ||
-      (rc = S.alloc(ctx, d_X, (int64_t)nquad * nquad)))
-    return rc;
+  if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
+  char *tb = ctx->d_tmp;
+  CentreInfo *d_ci = Carve::at<CentreInfo>(tb, o_ci);
+  double *d_rad = Carve::at<double>(tb, o_rad), *d_wl = Carve::at<double>(tb, o_wl);
+  double4 *d_ang = Carve::at<double4>(tb, o_ang);
+  double *rx = Carve::at<double>(tb, o_rx), *ry = Carve::at<double>(tb, o_ry),
+         *rz = Carve::at<double>(tb, o_rz), *rw = Carve::at<double>(tb, o_rw);
+  int32_t *rown = Carve::at<int32_t>(tb, o_rown);
+  uint8_t *fl = Carve::at<uint8_t>(tb, o_fl);
+  int64_t *idx = Carve::at<int64_t>(tb, o_idx);
+  double *d_cent = Carve::at<double>(tb, o_cent), *d_R = Carve::at<double>(tb, o_R),
+         *d_X = Carve::at<double>(tb, o_X);
   hipStream_t st = ctx->stream;
   INCA_HIP(hipMemcpyAsync(d_ci, ci.data(), nquad * sizeof(CentreInfo), hipMemcpyHostToDevice, st));
   INCA_HIP(hipMemcpyAsync(d_rad, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -254,12 +266,9 @@ int grid_multicenter(inca_ctx *ctx, int nquad, const double *cent,
   if ((rc = compact_flags(ctx, fl, np, idx, &nk))) return rc;
   *rgrid = nk;
   // gather the symmetry-kept points
-  double *kx, *ky, *kz, *kw;
-  int32_t *kown;
-  if ((rc = S.alloc(ctx, kx, nk)) || (rc = S.alloc(ctx, ky, nk)) ||
-      (rc = S.alloc(ctx, kz, nk)) || (rc = S.alloc(ctx, kw, nk)) ||
-      (rc = S.alloc(ctx, kown, nk)))
-    return rc;
+  double *kx = Carve::at<double>(tb, o_kx), *ky = Carve::at<double>(tb, o_ky),
+         *kz = Carve::at<double>(tb, o_kz), *kw = Carve::at<double>(tb, o_kw);
+  int32_t *kown = Carve::at<int32_t>(tb, o_kown);
   if (nk > 0) {
     hipLaunchKernelGGL(k_gather<double>, dim3(nblk(nk)), dim3(kT), 0, st, rx, idx, nk, kx);
     hipLaunchKernelGGL(k_gather<double>, dim3(nblk(nk)), dim3(kT), 0, st, ry, idx, nk, ky);
@@ -306,8 +315,10 @@ int grid_multicenter(inca_ctx *ctx, int nquad, const double *cent,
   for (int c = nquad - 1; c >= 0; --c)
     if (ctx->centre_first[c] > ctx->centre_first[c + 1]) ctx->centre_first[c] = ctx->centre_first[c + 1];
   // centre coordinates: the anchors of the evaluation's tile bounds
-  INCA_HIP(hipMalloc((void **)&ctx->d_gcent, 3 * (size_t)nquad * sizeof(double)));
-  INCA_HIP(hipMemcpy(ctx->d_gcent, cent, 3 * (size_t)nquad * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = ensure(ctx, ctx->d_gcent, ctx->gcent_cap, 3 * (size_t)nquad))) return rc;
+  INCA_HIP(hipMemcpyAsync(ctx->d_gcent, cent, 3 * (size_t)nquad * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+  INCA_HIP(hipStreamSynchronize(st));
   *rrgrid = nf;
   ctx->nquad = nquad;
   ctx->grid_kind = GRID_MULTI;

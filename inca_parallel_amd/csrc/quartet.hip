@@ -755,10 +755,15 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // block partials of sum M (the small-degree pass, then the deferred
   // records' pass), then the total after them
   const int64_t nb1 = nblk(nvalid);
-  double *d_mom = nullptr;
+  // temporaries from the context's scratch block: the term-integral
+  // partials, the deferral flags, and the mixed split's histogram and flags
+  Carve cv;
+  const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(nvalid),
+               o_hist = cv.add<unsigned long long>(kMagBins), o_f2 = cv.add<uint8_t>(nvalid);
+  if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
+  double *d_mom = want_mom ? Carve::at<double>(ctx->d_tmp, o_mom) : nullptr;
   uint8_t *d_defer = nullptr;
   ctx->term_integral = 0.0;
-  if (want_mom) INCA_HIP(hipMalloc((void **)&d_mom, ((size_t)2 * nb1 + 1) * sizeof(double)));
   if (nvalid > 0) {
     // Fitted polynomials: records whose axes are all of degree <= 4 (most
     // of them) go through the register-resident fits; the rest are
@@ -766,7 +771,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     // the basis.  Exact polynomials: the scratch kernel takes every record.
     const bool split = !ctx->poly_exact;
     hipError_t e = hipSuccess;
-    if (split && la > 4) e = hipMalloc((void **)&d_defer, (size_t)nvalid);
+    if (split && la > 4) d_defer = Carve::at<uint8_t>(ctx->d_tmp, o_def);
     if (split && e == hipSuccess) {
       hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(nvalid)), dim3(kT), 0, st,
                          (const int32_t *)ctx->d_rec, nvalid, (const int64_t *)nullptr, nprim,
@@ -778,11 +783,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     int64_t nrun = split ? 0 : nvalid;  // records for the scratch kernel
     if (e == hipSuccess && d_defer) {
       rc = compact_flags(ctx, d_defer, nvalid, ctx->d_idx, &nrun);
-      if (rc) {
-        (void)hipFree(d_defer);
-        if (d_mom) (void)hipFree(d_mom);
-        return rc;
-      }
+      if (rc) return rc;
     }
     const int64_t *run_idx = split ? ctx->d_idx : nullptr;
     double *run_mom = split ? d_mom + nb1 : d_mom;
@@ -814,8 +815,6 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     // in the grids' convention: their weights carry 2 pi where the volume
     // element has 4 pi (mod_build_grid.f90:132)
     ctx->term_integral *= 0.5;
-    if (d_mom) (void)hipFree(d_mom);
-    if (d_defer) (void)hipFree(d_defer);
     if (e != hipSuccess) return hip_fail(ctx, e, "quartet preparation", __LINE__);
   }
   int64_t q2 = 0;
@@ -829,8 +828,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     // ulps of its L1 norm, bounded by B; the lightest terms go to the FP32
     // kernel while their summed B stays within mixed_tail * |sum M| (sum M
     // being the integral of I over all space), the rest stays FP64.
-    unsigned long long *d_h = nullptr;
-    INCA_HIP(hipMalloc((void **)&d_h, kMagBins * sizeof(unsigned long long)));
+    unsigned long long *d_h = Carve::at<unsigned long long>(ctx->d_tmp, o_hist);
     std::vector<unsigned long long> h(kMagBins, 0);
     INCA_HIP(hipMemcpyAsync(d_h, h.data(), kMagBins * sizeof(unsigned long long),
                             hipMemcpyHostToDevice, st));
@@ -841,7 +839,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       e = hipMemcpyAsync(h.data(), d_h, kMagBins * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(d_h);
+
     if (e != hipSuccess) return hip_fail(ctx, e, "magnitude histogram", __LINE__);
     // mass of a bin ~ count * 2^(exponent + 1/2)
     std::vector<double> mass(kMagBins);
@@ -860,8 +858,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     while (cut < kMagBins && cum + mass[cut] <= budget) cum += mass[cut++];
     const double tau = std::ldexp(1.0, cut - kMagOff);
     ctx->mixed_tau = tau;
-    uint8_t *d_f2 = nullptr;
-    INCA_HIP(hipMalloc((void **)&d_f2, (size_t)nvalid));
+    uint8_t *d_f2 = Carve::at<uint8_t>(ctx->d_tmp, o_f2);
     int64_t nh = 0, nl = 0;
     hipLaunchKernelGGL(k_split_flags, dim3(nblk(nvalid)), dim3(kT), 0, st, ctx->d_qtmp, stride,
                        ctx->d_flag, nvalid, tau, 1, d_f2);
@@ -878,7 +875,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       hipLaunchKernelGGL(k_gather_records, dim3(nblk(nl * stride)), dim3(kT), 0, st,
                          ctx->d_qtmp, ctx->d_idx, nl, stride, ctx->d_q + nh * stride);
     e = hipStreamSynchronize(st);
-    (void)hipFree(d_f2);
+
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(ctx, e, "mixed split", __LINE__);
     ctx->q_fp64 = nh;
