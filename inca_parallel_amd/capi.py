@@ -27,6 +27,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
+           "inca_set_cube_contraction",
            "inca_get_eval_counts",
            "inca_prepare", "inca_evaluate_prepared", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
@@ -85,6 +86,7 @@ def load():
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_get_analytic_integral.argtypes = [vp, vp]
     L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
+    L.inca_set_cube_contraction.argtypes = [vp, C.c_int32]
     L.inca_get_eval_counts.argtypes = [vp, vp, vp]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
     L.inca_evaluate_prepared.argtypes = [vp, C.c_int32, vp]
@@ -521,6 +523,12 @@ class Engine:
         the 2^-log2_rel radius of each quartet."""
         self._ck(self._L.inca_set_point_cutoff(self._h, float(log2_rel)),
                  "inca_set_point_cutoff")
+
+    def set_cube_contraction(self, on):
+        """1 (default): cube grids by the per-axis contraction (every quartet
+        at every voxel, FP64); 0: term by term like other grids."""
+        self._ck(self._L.inca_set_cube_contraction(self._h, int(on)),
+                 "inca_set_cube_contraction")
 
     def eval_counts(self):
         """(reference evals, executed evals) of the last grid loop."""

@@ -78,7 +78,7 @@ int inca_destroy(inca_ctx *c) {
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
                   c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup,
-                  c->d_red, c->d_tmp};
+                  c->d_red, c->d_tmp, c->d_cf};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
@@ -414,6 +414,13 @@ int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel) {
   if (!(log2_rel == 0.0 || (log2_rel >= 64.0 && log2_rel <= 1000.0)))
     return set_err(ctx, INCA_EINVAL, "point cutoff must be 0 (off) or in [64, 1000]");
   ctx->cut_log2 = log2_rel;
+  return 0;
+}
+
+int inca_set_cube_contraction(inca_ctx *ctx, int32_t on) {
+  CHECK_CTX(ctx);
+  if (on != 0 && on != 1) return set_err(ctx, INCA_EINVAL, "cube contraction must be 0 or 1");
+  ctx->cube_contract = on;
   return 0;
 }
 

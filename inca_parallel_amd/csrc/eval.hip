@@ -131,6 +131,98 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(P >= 10 ? 4 
   }
 }
 
+// Cube contraction, per-axis factors: F[r][a] for the n records of a batch
+// and the cube coordinates a: x (nx of them, zero up to nxp), then y, then
+// z (the cube's point g = (i ny + j) nz + k sits at (gx[i ny nz], gy[j nz],
+// gz[k])).  F = exp(-rp^2) P(rp), rp = sqe c + (the record's offset on
+// that axis), P the axis polynomial (the x one carries A).
+__global__ __launch_bounds__(kT) void k_cube_factors(const double *__restrict__ q, int qstride,
+                                                     int64_t n, int nx, int nxp, int ny, int nz,
+                                                     const double *__restrict__ gx,
+                                                     const double *__restrict__ gy,
+                                                     const double *__restrict__ gz,
+                                                     double *__restrict__ F) {
+  __shared__ double s_exp2[kExpTable];
+  stage_exp_table(s_exp2);
+  const int64_t nf = (int64_t)nxp + ny + nz;
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= n * nf) return;
+  const int64_t r = t / nf;
+  const int a = (int)(t - r * nf);
+  if (a >= nx && a < nxp) {  // padding of the x factors
+    F[t] = 0.0;
+    return;
+  }
+  const double *__restrict__ rec = q + r * qstride;
+  const QHeader h = load_header(rec);
+  const double *__restrict__ c = rec + kQHead;
+  int L;
+  double coord, off;
+  if (a < nx) {
+    L = h.Lx;
+    coord = gx[(int64_t)a * ny * nz];
+    off = h.ax;
+  } else if (a < nxp + ny) {
+    c += h.Lx + 1;
+    L = h.Ly;
+    coord = gy[(int64_t)(a - nxp) * nz];
+    off = h.ay;
+  } else {
+    c += h.Lx + h.Ly + 2;
+    L = h.Lz;
+    coord = gz[a - nxp - ny];
+    off = h.az;
+  }
+  const double rp = fma(h.sqe, coord, off);
+  const double poly = (L == 0) ? c[0] : horner1(c, L, rp);
+  F[t] = exp_neg(rp * rp, s_exp2) * poly;
+}
+
+// Cube contraction: partial[c][(i ny + j) nz + k] (=, or += when
+// `accumulate`) sum over the records of chunk c (qch records each) of
+// Fx(i) Fy(j) Fz(k).  Block x: i block (IB consecutive i) and block of 256
+// consecutive (j, k); block y: the chunk.  Each lane holds one (j, k) and
+// IB sums; per record it forms Fy Fz once and adds Fx(i) times it for its
+// IB values of i, which are the same for the whole wave (scalar loads; the
+// x factors are zero-padded to nxp = ni IB, so no i needs a test).
+template <int IB>
+__global__ __launch_bounds__(kT) void k_cube_contract(const double *__restrict__ F, int64_t n,
+                                                      int64_t qch, int nx, int ny, int nz,
+                                                      int64_t ni, int accumulate,
+                                                      double *__restrict__ partial, int64_t G) {
+  const int nxp = (int)(ni * IB);
+  const int64_t nf = (int64_t)nxp + ny + nz;
+  const int64_t njk = (int64_t)ny * nz;
+  const int64_t bi = blockIdx.x % ni, bjk = blockIdx.x / ni;
+  const int64_t chunk = blockIdx.y;
+  const int64_t q0 = chunk * qch;
+  const int64_t q1 = (q0 + qch < n) ? q0 + qch : n;
+  const int i0 = (int)(bi * IB);
+  const int64_t jk = bjk * kT + threadIdx.x;
+  const bool valid = jk < njk;
+  const int64_t jkc = valid ? jk : 0;
+  const int j = (int)(jkc / nz), k = (int)(jkc - (int64_t)(jkc / nz) * nz);
+  double acc[IB];
+#pragma unroll
+  for (int ii = 0; ii < IB; ++ii) acc[ii] = 0.0;
+  for (int64_t qi = q0; qi < q1; ++qi) {
+    const double *__restrict__ Fq = F + qi * nf;
+    const double t = Fq[nxp + j] * Fq[nxp + ny + k];
+#pragma unroll
+    for (int ii = 0; ii < IB; ++ii) acc[ii] = fma(Fq[i0 + ii], t, acc[ii]);
+  }
+  if (!valid) return;
+  double *__restrict__ row = partial + chunk * G;
+#pragma unroll
+  for (int ii = 0; ii < IB; ++ii) {
+    const int i = i0 + ii;
+    if (i < nx) {
+      const int64_t g = (int64_t)i * njk + jk;
+      row[g] = accumulate ? row[g] + acc[ii] : acc[ii];
+    }
+  }
+}
+
 // rho = exp(-2 sqe^2 h^2) of every prepared record into word 7 (ray tiles).
 __global__ __launch_bounds__(kT) void k_ray_rho(double *__restrict__ q, int qstride, int64_t nq,
                                                 double h2) {
@@ -586,6 +678,69 @@ void chunking(int64_t n, int64_t tiles, double share, int64_t *qch, int64_t *nch
 
 }  // namespace
 
+// The cube contraction over every prepared quartet: factors for batches of
+// quartets (at most 2^28 doubles at a time), each batch contracted into the
+// same `*nrows` partial rows, one per chunk, added in batch order.
+int cube_contract(inca_ctx *ctx, int64_t *nrows) {
+  hipStream_t st = ctx->stream;
+  const int nx = ctx->cube_n[0], ny = ctx->cube_n[1], nz = ctx->cube_n[2];
+  const int64_t G = ctx->G, nq = ctx->q2;
+  const int64_t njk = (int64_t)ny * nz;
+  if ((int64_t)nx * njk != G) return set_err(ctx, INCA_EINVAL, "cube grid of unexpected size");
+  // x values per lane: the choice that wastes the fewest, the larger on ties
+  const int ibs[5] = {8, 16, 24, 32, 40};
+  int IB = 8;
+  int64_t best = INT64_MAX;
+  for (int c : ibs) {
+    const int64_t span = (nx + c - 1) / c * (int64_t)c;
+    if (span <= best) {
+      best = span;
+      IB = c;
+    }
+  }
+  const int64_t ni = (nx + IB - 1) / IB, njkb = (njk + kT - 1) / kT;
+  const int nxp = (int)(ni * IB);  // x factors zero-padded to whole i blocks
+  const int64_t nfp = (int64_t)nxp + ny + nz;
+  // chunks: about 2048 blocks per batch
+  const int64_t per_chunk = ni * njkb;
+  // quartets per factor batch: within 2^28 doubles, and at most half of
+  // them beyond 8192 (no batch need hold every quartet)
+  const int64_t nb = std::max<int64_t>(
+      1, std::min<int64_t>({nq, ((int64_t)1 << 28) / nfp,
+                            std::max<int64_t>(4096, (nq + 1) / 2)}));
+  int64_t cpb = std::max<int64_t>(1, (2048 + per_chunk - 1) / per_chunk);
+  cpb = std::min<int64_t>(cpb, std::max<int64_t>(1, (nb + 63) / 64));
+  if (cpb > 65535) cpb = 65535;
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)(nb * nfp)))) return rc;
+  if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(cpb * G)))) return rc;
+  for (int64_t qb = 0; qb < nq; qb += nb) {
+    const int64_t n = std::min(nb, nq - qb);
+    const int64_t nt = n * nfp;
+    hipLaunchKernelGGL(k_cube_factors, dim3((unsigned)((nt + kT - 1) / kT)), dim3(kT), 0, st,
+                       ctx->d_q + qb * ctx->qstride, ctx->qstride, n, nx, nxp, ny, nz, ctx->d_gx,
+                       ctx->d_gy, ctx->d_gz, ctx->d_cf);
+    // every batch fills all cpb rows (a chunk may be empty: it writes zeros)
+    const int64_t qch = (n + cpb - 1) / cpb;
+    const dim3 grid((unsigned)per_chunk, (unsigned)cpb);
+    const int accumulate = qb > 0;
+#define INCA_CUBE(IBV)                                                                         \
+  hipLaunchKernelGGL(k_cube_contract<IBV>, grid, dim3(kT), 0, st, ctx->d_cf, n, qch, nx, ny, nz, \
+                     ni, accumulate, ctx->d_partial, G)
+    switch (IB) {
+      case 8: INCA_CUBE(8); break;
+      case 16: INCA_CUBE(16); break;
+      case 24: INCA_CUBE(24); break;
+      case 32: INCA_CUBE(32); break;
+      default: INCA_CUBE(40); break;
+    }
+#undef INCA_CUBE
+    INCA_HIP(hipGetLastError());
+  }
+  *nrows = cpb;
+  return 0;
+}
+
 int evaluate_grid(inca_ctx *ctx, int precision) {
   hipStream_t st = ctx->stream;
   const int64_t G = ctx->G, nq = ctx->q2;
@@ -605,45 +760,51 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   }
   if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ctx->d_evals);
-  // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
-  const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
-  const int64_t nr[2] = {n64, nq - n64};
-  // the kernel of each range: FP64 with tile_p[0]; the FP32 range with the
-  // mixed kernel, or (tile_p[1] == 0) on the FP64 ray tiles where the grid has
-  // them
-  bool mixedk[2] = {false, true};
-  int Pk[2] = {ctx->tile_p[0], ctx->tile_p[1]};
-  if (Pk[1] == 0) {
-    int64_t t;
-    bool ry;
-    if ((rc = tile_layout(ctx, false, Pk[0], &t, &ry))) return rc;
-    if (ry) {
-      mixedk[1] = false;
-      Pk[1] = Pk[0];
-    } else {
-      Pk[1] = 8;
-    }
-  }
-  int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
+  const bool contracted = ctx->grid_kind == GRID_CUBE && ctx->cube_contract;
+  int64_t nchunks = 0;
   bool rays_used = false;  // a range ran on the ray lists (duplicate points skipped)
-  for (int r = 0; r < 2; ++r) {
-    if (nr[r] == 0) continue;
-    int64_t tiles;
-    bool rays;
-    if ((rc = tile_layout(ctx, mixedk[r], Pk[r], &tiles, &rays))) return rc;
-    if (rays && (rc = ensure_rho(ctx))) return rc;
-    rays_used = rays_used || rays;
-    chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
+  if (contracted) {
+    if ((rc = cube_contract(ctx, &nchunks))) return rc;
+  } else {
+    // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
+    const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
+    const int64_t nr[2] = {n64, nq - n64};
+    // the kernel of each range: FP64 with tile_p[0]; the FP32 range with the
+    // mixed kernel, or (tile_p[1] == 0) on the FP64 ray tiles where the grid has
+    // them
+    bool mixedk[2] = {false, true};
+    int Pk[2] = {ctx->tile_p[0], ctx->tile_p[1]};
+    if (Pk[1] == 0) {
+      int64_t t;
+      bool ry;
+      if ((rc = tile_layout(ctx, false, Pk[0], &t, &ry))) return rc;
+      if (ry) {
+        mixedk[1] = false;
+        Pk[1] = Pk[0];
+      } else {
+        Pk[1] = 8;
+      }
+    }
+    int64_t nch[2] = {0, 0}, qch[2] = {1, 1};
+    for (int r = 0; r < 2; ++r) {
+      if (nr[r] == 0) continue;
+      int64_t tiles;
+      bool rays;
+      if ((rc = tile_layout(ctx, mixedk[r], Pk[r], &tiles, &rays))) return rc;
+      if (rays && (rc = ensure_rho(ctx))) return rc;
+      rays_used = rays_used || rays;
+      chunking(nr[r], tiles, (double)nr[r] / (double)nq, &qch[r], &nch[r]);
+    }
+    nchunks = nch[0] + nch[1];
+    if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
+    if (nch[0] > 0 &&
+        (rc = launch_eval(ctx, mixedk[0], Pk[0], ctx->d_q, nr[0], qch[0], nch[0], ctx->d_partial)))
+      return rc;
+    if (nch[1] > 0 &&
+        (rc = launch_eval(ctx, mixedk[1], Pk[1], ctx->d_q + n64 * ctx->qstride, nr[1], qch[1],
+                          nch[1], ctx->d_partial + nch[0] * G)))
+      return rc;
   }
-  const int64_t nchunks = nch[0] + nch[1];
-  if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
-  if (nch[0] > 0 &&
-      (rc = launch_eval(ctx, mixedk[0], Pk[0], ctx->d_q, nr[0], qch[0], nch[0], ctx->d_partial)))
-    return rc;
-  if (nch[1] > 0 &&
-      (rc = launch_eval(ctx, mixedk[1], Pk[1], ctx->d_q + n64 * ctx->qstride, nr[1], qch[1],
-                        nch[1], ctx->d_partial + nch[0] * G)))
-    return rc;
   INCA_HIP(hipEventRecord(ctx->ev[2], st));
   {
     // Chunk partials to Ivec in a fixed order.  With few points per chunk
@@ -672,7 +833,7 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   ctx->have_ivec = 1;
   ctx->evals_reference = (int64_t)nq * G;
   ctx->evals_executed = ctx->evals_reference;
-  if (ctx->cut_log2 > 0.0) {
+  if (ctx->cut_log2 > 0.0 && !contracted) {
     unsigned long long ex = 0;
     INCA_HIP(hipMemcpyAsync(&ex, ctx->d_evals, sizeof(ex), hipMemcpyDeviceToHost, st));
     INCA_HIP(hipStreamSynchronize(st));

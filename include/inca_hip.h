@@ -265,6 +265,19 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
  * effect at the next inca_evaluate / inca_prepare.  No reference
  * counterpart. */
 int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
+
+/* Cube grids by the per-axis contraction (on = 1, the default).  A cube
+ * (mod_build_grid.f90:412-440) is a tensor product of nx x-, ny y- and nz
+ * z-coordinates, and exp(-|rp|^2) is the product of its per-axis factors,
+ * so every term is A Fx(x) Fy(y) Fz(z) with F(c) = exp(-rp_c^2) P_c(rp_c).
+ * The factors are formed once per quartet and coordinate (nx + ny + nz of
+ * them) and Ivec(i,j,k) = sum_q Fx_q(i) Fy_q(j) Fz_q(k) is contracted with
+ * one FMA per quartet and voxel: every quartet at every voxel, in FP64 (a
+ * mixed evaluation too), with no cutoff.  0 evaluates cubes term by term
+ * like other grids.  Takes effect at the next evaluation.  No reference
+ * counterpart: the reference evaluates each voxel's terms directly
+ * (mod_intracule.f90:345-358). */
+int inca_set_cube_contraction(inca_ctx *ctx, int32_t on);
 /* Evaluations of the last grid loop: reference = quartets x points (the
  * reference's count), executed = those the cutoff kept (equal when off). */
 int inca_get_eval_counts(inca_ctx *ctx, int64_t *reference, int64_t *executed);
