@@ -180,11 +180,13 @@ __global__ __launch_bounds__(kT) void k_cube_factors(const double *__restrict__ 
 
 // Cube contraction: partial[c][(i ny + j) nz + k] (=, or += when
 // `accumulate`) sum over the records of chunk c (qch records each) of
-// Fx(i) Fy(j) Fz(k).  Block x: i block (IB consecutive i) and block of 256
-// consecutive (j, k); block y: the chunk.  Each lane holds one (j, k) and
-// IB sums; per record it forms Fy Fz once and adds Fx(i) times it for its
-// IB values of i, which are the same for the whole wave (scalar loads; the
-// x factors are zero-padded to nxp = ni IB, so no i needs a test).
+// Fx(i) Fy(j) Fz(k).  Block x: an i block (IB consecutive i) and a 16 x 16
+// tile of (j, k), each wave 8 x 8 of it, so that a wave reads 8 y and 8 z
+// factors per record (its vector-cache traffic, not its FMAs, bounds the
+// loop otherwise); block y: the chunk.  Each lane holds one (j, k) and IB
+// sums; per record it forms Fy Fz once and adds Fx(i) times it for its IB
+// values of i, which are the same for the whole wave (scalar loads; the x
+// factors are zero-padded to nxp = ni IB, so no i needs a test).
 template <int IB>
 __global__ __launch_bounds__(kT) void k_cube_contract(const double *__restrict__ F, int64_t n,
                                                       int64_t qch, int nx, int ny, int nz,
@@ -193,15 +195,19 @@ __global__ __launch_bounds__(kT) void k_cube_contract(const double *__restrict__
   const int nxp = (int)(ni * IB);
   const int64_t nf = (int64_t)nxp + ny + nz;
   const int64_t njk = (int64_t)ny * nz;
-  const int64_t bi = blockIdx.x % ni, bjk = blockIdx.x / ni;
+  const int njt = (ny + 15) / 16;
+  const int64_t bi = blockIdx.x % ni, bt = blockIdx.x / ni;
+  const int bj = (int)(bt % njt), bk = (int)(bt / njt);
   const int64_t chunk = blockIdx.y;
   const int64_t q0 = chunk * qch;
   const int64_t q1 = (q0 + qch < n) ? q0 + qch : n;
   const int i0 = (int)(bi * IB);
-  const int64_t jk = bjk * kT + threadIdx.x;
-  const bool valid = jk < njk;
-  const int64_t jkc = valid ? jk : 0;
-  const int j = (int)(jkc / nz), k = (int)(jkc - (int64_t)(jkc / nz) * nz);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int jj = bj * 16 + (w >> 1) * 8 + (lane >> 3);
+  const int kk = bk * 16 + (w & 1) * 8 + (lane & 7);
+  const bool valid = jj < ny && kk < nz;
+  const int j = valid ? jj : 0, k = valid ? kk : 0;
+  const int64_t jk = (int64_t)j * nz + k;
   double acc[IB];
 #pragma unroll
   for (int ii = 0; ii < IB; ++ii) acc[ii] = 0.0;
@@ -698,7 +704,8 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
       IB = c;
     }
   }
-  const int64_t ni = (nx + IB - 1) / IB, njkb = (njk + kT - 1) / kT;
+  const int64_t ni = (nx + IB - 1) / IB;
+  const int64_t njkb = (int64_t)((ny + 15) / 16) * ((nz + 15) / 16);  // 16 x 16 (j, k) tiles
   const int nxp = (int)(ni * IB);  // x factors zero-padded to whole i blocks
   const int64_t nfp = (int64_t)nxp + ny + nz;
   // chunks: about 2048 blocks per batch
