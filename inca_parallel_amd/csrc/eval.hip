@@ -708,14 +708,28 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
   const int64_t njkb = (int64_t)((ny + 15) / 16) * ((nz + 15) / 16);  // 16 x 16 (j, k) tiles
   const int nxp = (int)(ni * IB);  // x factors zero-padded to whole i blocks
   const int64_t nfp = (int64_t)nxp + ny + nz;
-  // chunks: about 2048 blocks per batch
+  // chunks: the blocks of a batch fill two rounds of the device (every block
+  // does the same work, so a partial last round would idle most CUs)
   const int64_t per_chunk = ni * njkb;
-  // quartets per factor batch: within 2^28 doubles, and at most half of
-  // them beyond 8192 (no batch need hold every quartet)
+  int ncu = 0, per_cu = 0;
+  INCA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  hipError_t oe = hipSuccess;
+  switch (IB) {
+    case 8: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<8>, kT, 0); break;
+    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<16>, kT, 0); break;
+    case 24: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<24>, kT, 0); break;
+    case 32: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<32>, kT, 0); break;
+    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<40>, kT, 0); break;
+  }
+  if (oe != hipSuccess) return hip_fail(ctx, oe, "occupancy query", __LINE__);
+  const int64_t resident = (int64_t)std::max(1, ncu) * std::max(1, per_cu);
+  // quartets per factor batch: within 2^28 doubles (2 GB; 2^26 ran as
+  // fast, 2^30 slower as each chunk's rows left the caches), and at most half
+  // of them beyond 8192 (no batch need hold every quartet)
   const int64_t nb = std::max<int64_t>(
       1, std::min<int64_t>({nq, ((int64_t)1 << 28) / nfp,
                             std::max<int64_t>(4096, (nq + 1) / 2)}));
-  int64_t cpb = std::max<int64_t>(1, (2048 + per_chunk - 1) / per_chunk);
+  int64_t cpb = std::max<int64_t>(1, 2 * resident / per_chunk);
   cpb = std::min<int64_t>(cpb, std::max<int64_t>(1, (nb + 63) / 64));
   if (cpb > 65535) cpb = 65535;
   int rc;
