@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kT) void k_class_hist(const double *__restrict__ q,
     atomicAdd(&h[L], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < 64 && threadIdx.x < nbins && h[threadIdx.x])
+  if (threadIdx.x < 64 && (int)threadIdx.x < nbins && h[threadIdx.x])
     atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
 

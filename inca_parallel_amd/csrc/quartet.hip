@@ -578,11 +578,9 @@ __global__ __launch_bounds__(kT) void k_prep(
   }
   // wave-aggregated counters
   unsigned long long b1 = __ballot(pass1);
-  unsigned long long bf = 0;
   int wsum = nfail;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) wsum += __shfl_xor(wsum, d, 64);
-  (void)bf;
   if (__lane_id() == 0) {
     if (b1) atomicAdd(&cnt[1], (unsigned long long)__popcll(b1));
     if (wsum) atomicAdd(&cnt[3], (unsigned long long)wsum);
