@@ -300,6 +300,10 @@ int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records, int32_t on_
   INCA_HIP(hipSetDevice(ctx->device));
   if (int rc = ensure(ctx, ctx->d_rec, ctx->rec_cap, (size_t)(n > 0 ? n : 1) * 24))
     return rc;
+  // Device records may still be in flight on the producer's stream (the
+  // context's own stream does not order itself after the default stream):
+  // wait for all work queued on the device before copying them.
+  if (n > 0 && on_device) INCA_HIP(hipDeviceSynchronize());
   if (n > 0)
     INCA_HIP(hipMemcpyAsync(ctx->d_rec, records, 24 * (size_t)n,
                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,

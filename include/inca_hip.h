@@ -103,7 +103,9 @@ int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec,
                    int32_t *format);
 /* Same records from memory: n packed 24-byte records (int32 i,j,k,l;
  * double D).  on_device != 0: `records` is a device pointer that is copied
- * device-to-device (the caller keeps ownership). */
+ * device-to-device (the caller keeps ownership); the call first waits for
+ * all work queued on the device, so records still being written by a
+ * kernel on any stream are copied complete. */
 int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
                       int32_t on_device);
 

@@ -197,6 +197,34 @@ def test_empty_inputs(engine, ch4):
     assert np.array_equal(engine.radial_values(), np.zeros(3))
 
 
+def test_device_records_wait_for_their_producer(engine, ch4):
+    """Records handed over as a device pointer while torch is still writing
+    them: the copy is queued on torch's stream behind 40 FP64 products of
+    4096 x 4096 matrices, and the pointer is passed at once.  The context's stream
+    does not order itself after torch's, so the entry point must wait for the
+    device; results equal those of the same records from host memory."""
+    import torch
+    n = len(ch4["d"])
+    host = np.empty(n, dtype=[("ijkl", "<i4", 4), ("d", "<f8")])
+    host["ijkl"], host["d"] = ch4["ijkl"], ch4["d"]
+    src = torch.from_numpy(host.view(np.int32).reshape(n, 6).copy()).cuda()
+    dst = torch.zeros_like(src)
+    a = torch.rand(4096, 4096, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    engine.set_primitives(ch4["alpha"], ch4["tmn"], ch4["x"], ch4["y"], ch4["z"])
+    engine.grid_points(np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.5], [1.0, 1.0, 1.0]]))
+    for _ in range(40):  # 5.5 TFLOP in FP64 ahead of the copy
+        b = a @ a
+    dst.copy_(src)  # device to device, asynchronous on torch's stream
+    engine.set_quartets(device_ptr=dst.data_ptr(), n=n)
+    iv_dev, cnt_dev = engine.evaluate(capi.INCA_FP64, THRESH)
+    engine.set_quartets(records=host)
+    iv_host, cnt_host = engine.evaluate(capi.INCA_FP64, THRESH)
+    assert list(cnt_dev) == list(cnt_host)
+    assert np.array_equal(iv_dev, iv_host)
+    torch.cuda.synchronize()
+
+
 def test_cube_grid_layout(engine):
     for npts, step, centre in (((4, 3, 5), (0.2, 0.3, 0.1), (0.5, -1.0, 2.0)),
                                ((2, 2, 2), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))):
