@@ -1,0 +1,127 @@
+"""The radial-plot workload at its full size, through properties that do not
+depend on size.
+
+The workload: 10-atom C/N/O/H cluster, 151 s/p/d primitives, every
+symmetry-unique DM2 record (65,855,026), and s = 0..10 in steps of 0.05 on
+LD0302 (32,502 points).  The oracle cannot finish this size in seconds, so
+the checks are internal:
+  * the record count is the closed form npair (npair + 1) / 2 with
+    npair = nprim (nprim + 1) / 2, and the screens only remove records;
+  * the ray tiles (8 points per lane, exp(-s) by recurrence) against the
+    point-by-point kernel on the same prepared quartets;
+  * the per-point cutoff against the full quartet x point loop;
+  * mixed precision against FP64 (1e-8 on the integral, the precision
+    target of the mixed path);
+  * two evaluations bitwise identical (fixed-order reductions).
+The FP64 bar is the one of the oracle parity tests: 1e-10 relative wherever
+|I| >= 1e-6 max|I|, and 1e-13 of max|I| everywhere.  The oracle-based parity
+of each kernel is in test_gpu_parity.py and test_gpu_ch4.py at sizes the
+oracle finishes."""
+import numpy as np
+import pytest
+
+from inca_parallel_amd import capi, synth
+
+pytestmark = pytest.mark.gpu
+THRESH = 1e-12
+
+
+def _close(a, b):
+    """(max relative error where |b| >= 1e-6 max|b|, max |a-b| / max|b|)"""
+    scale = np.abs(b).max()
+    big = np.abs(b) >= 1e-6 * scale
+    err_rel = (np.abs(a - b)[big] / np.abs(b[big])).max() if big.any() else 0.0
+    return err_rel, np.abs(a - b).max() / scale
+
+
+def _integral(rad, radi):
+    """The integral of I over all space from the radial values on equally
+    spaced radii, up to the constant step: sum_s rintra(s) s^2."""
+    return float(np.sum(rad * radi * radi))
+
+
+@pytest.fixture(scope="module")
+def full():
+    import torch
+    s = synth.config_cluster(seed=2, natoms=10)
+    recs = s.dm2_unique_torch(device="cuda")
+    e = capi.Engine(0)
+    p = s.prims()
+    e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    nradi, G = e.grid_radial_plot([[0.0, 10.0]], [0.05], [302])
+    e.set_quartets(device_ptr=recs.data_ptr(), n=recs.shape[0])
+    n = recs.shape[0]
+    del recs
+    torch.cuda.empty_cache()
+    e.set_tiles(8, 8)
+    iv, cnt = e.evaluate(capi.INCA_FP64, THRESH)
+    print("full size: %d records, counts %s, %d radii, %d points" % (n, list(cnt), nradi, G),
+          flush=True)
+    radi, _ = e.radial_layout()
+    rad = e.radial_values().copy()
+    yield dict(s=s, e=e, n=n, nradi=nradi, G=G, iv=iv.copy(), cnt=cnt.copy(), rad=rad,
+               radi=radi, tot=_integral(rad, radi))
+    e.close()
+
+
+def test_record_count_and_screens(full):
+    nprim = full["s"].nprim
+    npair = nprim * (nprim + 1) // 2
+    assert nprim == 151
+    assert full["n"] == npair * (npair + 1) // 2 == 65855026
+    cnt = full["cnt"]
+    assert cnt[0] == full["n"]
+    assert 0 < cnt[2] <= cnt[1] <= cnt[0]
+    assert (full["nradi"], full["G"]) == (201, 32502)
+
+
+def test_rays_match_point_by_point(full):
+    e = full["e"]
+    try:
+        e.set_tiles(1, 8)
+        iv1, cnt1 = e.evaluate(capi.INCA_FP64, THRESH)
+        rad1 = e.radial_values()
+    finally:
+        e.set_tiles(8, 8)
+    assert list(cnt1) == list(full["cnt"])
+    assert not np.array_equal(iv1, full["iv"])  # two different kernels ran
+    er, ea = _close(full["iv"], iv1)
+    rr, ra = _close(full["rad"], rad1)
+    print("rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e"
+          % (er, ea, rr, ra))
+    assert er < 1e-10 and ea < 1e-13
+    assert rr < 1e-10 and ra < 1e-13
+
+
+def test_cutoff_matches_full_loop(full):
+    e = full["e"]
+    try:
+        e.set_point_cutoff(0)
+        iv0, _ = e.evaluate(capi.INCA_FP64, THRESH)
+        ref0, ex0 = e.eval_counts()
+    finally:
+        e.set_point_cutoff(100)
+    assert ex0 == ref0 == int(full["cnt"][2]) * full["G"]
+    er, ea = _close(full["iv"], iv0)
+    print("cutoff vs full loop: rel %.2e abs %.2e" % (er, ea))
+    assert er < 1e-10 and ea < 1e-13
+
+
+def test_mixed_within_target_of_fp64(full):
+    e = full["e"]
+    e.evaluate(capi.INCA_MIXED, THRESH, want_ivec=False)
+    rad = e.radial_values()
+    tot = _integral(rad, full["radi"])
+    n64, n32, _ = e.mixed_split()
+    print("mixed: %d FP64 + %d FP32 quartets, integral rel err %.2e" % (
+        n64, n32, abs(tot - full["tot"]) / abs(full["tot"])))
+    assert n32 > 0 and n64 + n32 == full["cnt"][2]
+    assert abs(tot - full["tot"]) <= 1e-8 * abs(full["tot"])
+    rr, _ = _close(rad, full["rad"])
+    assert rr < 1e-8
+
+
+def test_repeat_is_bitwise_identical(full):
+    iv, cnt = full["e"].evaluate(capi.INCA_FP64, THRESH)
+    assert list(cnt) == list(full["cnt"])
+    assert np.array_equal(iv, full["iv"])
