@@ -1,5 +1,5 @@
-"""The radial-plot workload at its full size, through properties that do not
-depend on size.
+"""The radial-plot and cube workloads at their full size, through properties
+that do not depend on size.
 
 The workload: 10-atom C/N/O/H cluster, 151 s/p/d primitives, every
 symmetry-unique DM2 record (65,855,026), and s = 0..10 in steps of 0.05 on
@@ -12,7 +12,9 @@ the checks are internal:
   * the per-point cutoff against the full quartet x point loop;
   * mixed precision against FP64 (1e-8 on the integral, the precision
     target of the mixed path);
-  * two evaluations bitwise identical (fixed-order reductions).
+  * two evaluations bitwise identical (fixed-order reductions);
+  * the 80^3 cube of the same records by the per-axis contraction against
+    the term-by-term ray tiles.
 The FP64 bar is the one of the oracle parity tests: 1e-10 relative wherever
 |I| >= 1e-6 max|I|, and 1e-13 of max|I| everywhere.  The oracle-based parity
 of each kernel is in test_gpu_parity.py and test_gpu_ch4.py at sizes the
@@ -125,3 +127,27 @@ def test_repeat_is_bitwise_identical(full):
     iv, cnt = full["e"].evaluate(capi.INCA_FP64, THRESH)
     assert list(cnt) == list(full["cnt"])
     assert np.array_equal(iv, full["iv"])
+
+
+def test_cube_contraction_matches_ray_tiles(full):
+    """The 80^3 cube (0.15 bohr steps, centred at the origin) on the same
+    records: the per-axis contraction (every quartet at every voxel) against
+    the term-by-term ray tiles along z (cutoff on).  Runs last: it replaces
+    the grid."""
+    e = full["e"]
+    n = e.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
+    assert n == 512000
+    try:
+        e.set_cube_contraction(0)
+        ray, cnt = e.evaluate(capi.INCA_FP64, THRESH)
+        ray = ray.copy()
+        e.set_cube_contraction(1)
+        got, cnt2 = e.evaluate(capi.INCA_FP64, THRESH)
+        ref_n, ex = e.eval_counts()
+    finally:
+        e.set_cube_contraction(1)
+    assert list(cnt) == list(cnt2) == list(full["cnt"])
+    assert ex == ref_n == int(cnt[2]) * n
+    er, ea = _close(got, ray)
+    print("80^3 cube, contraction vs ray tiles: rel %.2e abs %.2e" % (er, ea))
+    assert er < 1e-10 and ea < 1e-13
