@@ -16,11 +16,13 @@ namespace inca {
 constexpr int kLmaxAxis = 21;
 constexpr int kNpMax = kLmaxAxis + 1;
 // Prepared-quartet record: [0] sqe, [1..3] sqe*(P_ik - P_jl), [4] class bits
-// Lx | Ly<<8 | Lz<<16, [5] B = |A| prod_axis sum_k |C_k| Gamma((k+1)/2) / sqe,
-// a bound on the L1 norm of the term over all space, [6] rcut: beyond
-// |rp| = rcut the term stays below 2^-cut_log2 of its peak bound (+inf when
-// the cutoff is off), [7] unused, then the coefficients A*Cx(0..Lx),
-// Cy(0..Ly), Cz(0..Lz) (C(0) = highest power).
+// Lx | Ly<<8 | Lz<<16 | (axes of nonzero degree)<<24, [5] B = |A| prod_axis
+// sum_k |C_k| Gamma((k+1)/2) / sqe, a bound on the L1 norm of the term over
+// all space, [6] rcut: beyond |rp| = rcut the term stays below 2^-cut_log2
+// of its peak bound (+inf when the cutoff is off), [7] rho = exp(-2 sqe^2
+// h^2) for the ray step h of the current radial-scan or cube grid (set by
+// k_ray_rho, 0 before), then the coefficients A*Cx(0..Lx), Cy(0..Ly),
+// Cz(0..Lz) (C(0) = highest power).
 constexpr int kQHead = 8;
 
 enum GridKind { GRID_NONE = 0, GRID_MULTI = 1, GRID_RADIAL = 2, GRID_CUBE = 3, GRID_POINTS = 4 };
