@@ -526,7 +526,8 @@ class Engine:
 
     def set_cube_contraction(self, on):
         """1 (default): cube grids by the per-axis contraction (every quartet
-        at every voxel, FP64); 0: term by term like other grids."""
+        at every voxel, FP64) on the vector units; 2: the same contraction on
+        the FP64 matrix cores; 0: term by term like other grids."""
         self._ck(self._L.inca_set_cube_contraction(self._h, int(on)),
                  "inca_set_cube_contraction")
 

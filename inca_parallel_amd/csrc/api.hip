@@ -423,7 +423,7 @@ int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel) {
 
 int inca_set_cube_contraction(inca_ctx *ctx, int32_t on) {
   CHECK_CTX(ctx);
-  if (on != 0 && on != 1) return set_err(ctx, INCA_EINVAL, "cube contraction must be 0 or 1");
+  if (on < 0 || on > 2) return set_err(ctx, INCA_EINVAL, "cube contraction must be 0, 1 or 2");
   ctx->cube_contract = on;
   return 0;
 }

@@ -11,8 +11,9 @@
 // writes its sums to partial[chunk][g]; k_reduce adds the chunks in a fixed
 // order, so results are bitwise reproducible.
 //
-// Arithmetic.  This is exp-bound VALU work (no MFMA: nothing here is a
-// matrix product).  FP64 path: rp by one FMA per axis, exp(-s) by
+// Arithmetic.  The quartet x point loop is exp-bound VALU work; only the
+// per-axis contraction of cube grids is a matrix product (vector FMAs by
+// default, FP64 MFMA as an option).  FP64 path: rp by one FMA per axis, exp(-s) by
 // reduction by ln2/256 to |r| <= ln2/512, a degree-4 Taylor polynomial of
 // e^r (truncation error 3.8e-17), a 256-entry table of 2^(j/256) in LDS and
 // an exponent shift (eval_core.hpp).  Mixed path: rp in FP64 (the cancellation-prone step),
@@ -227,6 +228,90 @@ __global__ __launch_bounds__(kT) void k_cube_contract(const double *__restrict__
       row[g] = accumulate ? row[g] + acc[ii] : acc[ii];
     }
   }
+}
+
+// The same contraction on the FP64 matrix cores (mode 2; the vector kernel
+// above is the default: on the 80^3 benchmark cube this one takes 1.35-1.42 s
+// against 0.86 s, about 0.31 of the FP64 matrix peak whatever the number of
+// x tiles per wave and the occupancy).  Over the records of a
+// chunk it is the matrix product C[i][jk] = sum_q Fx[q][i] (Fy[q][j] Fz[q][k])
+// (jk = j nz + k, so C is the chunk's partial row in grid order).  A wave
+// computes MI x 2 tiles of 16 x values by 16 (j, k) pairs with
+// v_mfma_f64_16x16x4f64, four records per step; lane l supplies
+//   A: Fx[q0 + l/16][i0 + l%16]  (16 x 4),
+//   B: Fy[q][j] Fz[q][k] for q = q0 + l/16 and pair jk0 + l%16  (4 x 16),
+// and receives C[i0 + l/16 + 4r][jk0 + l%16] in register r (the lane layout
+// of the instruction on gfx950, checked against a host product).  Block x:
+// the group of 16 MI x values (ig = blockIdx.x mod nig) and 128 pairs (4
+// waves of 32); block y: the chunk.  The x factors are zero-padded to
+// nxp = 16 MI nig and records past the chunk supply A = 0, so no operand
+// needs a test.
+template <int MI>
+__global__ __launch_bounds__(kT) void k_cube_contract_mfma(const double *__restrict__ F,
+                                                           int64_t n, int64_t qch, int nx,
+                                                           int nxp, int ny, int nz, int nig,
+                                                           int accumulate,
+                                                           double *__restrict__ partial,
+                                                           int64_t G) {
+  const int64_t nf = (int64_t)nxp + ny + nz;
+  const int64_t njk = (int64_t)ny * nz;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ig = (int)(blockIdx.x % (unsigned)nig);
+  const int64_t jb = blockIdx.x / (unsigned)nig;
+  const int64_t chunk = blockIdx.y;
+  const int64_t q0 = chunk * qch;
+  const int64_t q1 = (q0 + qch < n) ? q0 + qch : n;
+  const int i0 = ig * 16 * MI;
+  const int64_t jk0 = jb * 128 + (int64_t)w * 32;
+  const int kq = lane >> 4;             // record of the step
+  const int ia = i0 + (lane & 15);      // x value of A in tile 0
+  int64_t jy[2], kz[2];
+  bool jok[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int64_t jk = jk0 + t * 16 + (lane & 15);
+    jok[t] = jk < njk;
+    const int64_t jc = jok[t] ? jk : 0;
+    jy[t] = nxp + jc / nz;
+    kz[t] = (int64_t)nxp + ny + jc % nz;
+  }
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  d4 acc[MI][2];
+#pragma unroll
+  for (int m = 0; m < MI; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) acc[m][t] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t q = q0; q < q1; q += 4) {
+    const int64_t qq = q + kq;
+    const bool ok = qq < q1;
+    const double *__restrict__ row = F + (ok ? qq : q) * nf;
+    double a[MI], b[2];
+#pragma unroll
+    for (int m = 0; m < MI; ++m) a[m] = ok ? row[ia + 16 * m] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) b[t] = jok[t] ? row[jy[t]] * row[kz[t]] : 0.0;
+#pragma unroll
+    for (int m = 0; m < MI; ++m)
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+        acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[t], acc[m][t], 0, 0, 0);
+  }
+  double *__restrict__ out = partial + chunk * G;
+#pragma unroll
+  for (int m = 0; m < MI; ++m)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t jk = jk0 + t * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + 16 * m + kq + 4 * r;
+        if (i < nx && jk < njk) {
+          const int64_t g = (int64_t)i * njk + jk;
+          out[g] = accumulate ? out[g] + acc[m][t][r] : acc[m][t][r];
+        }
+      }
+    }
 }
 
 // rho = exp(-2 sqe^2 h^2) of every prepared record into word 7 (ray tiles).
@@ -693,35 +778,55 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
   const int64_t G = ctx->G, nq = ctx->q2;
   const int64_t njk = (int64_t)ny * nz;
   if ((int64_t)nx * njk != G) return set_err(ctx, INCA_EINVAL, "cube grid of unexpected size");
-  // x values per lane: the choice that wastes the fewest, the larger on ties
-  const int ibs[5] = {8, 16, 24, 32, 40};
-  int IB = 8;
-  int64_t best = INT64_MAX;
-  for (int c : ibs) {
-    const int64_t span = (nx + c - 1) / c * (int64_t)c;
-    if (span <= best) {
-      best = span;
-      IB = c;
-    }
-  }
-  const int64_t ni = (nx + IB - 1) / IB;
-  const int64_t njkb = (int64_t)((ny + 15) / 16) * ((nz + 15) / 16);  // 16 x 16 (j, k) tiles
-  const int nxp = (int)(ni * IB);  // x factors zero-padded to whole i blocks
-  const int64_t nfp = (int64_t)nxp + ny + nz;
-  // chunks: the blocks of a batch fill two rounds of the device (every block
-  // does the same work, so a partial last round would idle most CUs)
-  const int64_t per_chunk = ni * njkb;
+  const bool mfma = ctx->cube_contract == 2;
   int ncu = 0, per_cu = 0;
   INCA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
   hipError_t oe = hipSuccess;
-  switch (IB) {
-    case 8: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<8>, kT, 0); break;
-    case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<16>, kT, 0); break;
-    case 24: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<24>, kT, 0); break;
-    case 32: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<32>, kT, 0); break;
-    default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<40>, kT, 0); break;
+  int IB = 8, MI = 1, nig = 1, nxp = 0;
+  int64_t ni = 1, per_chunk = 1;
+  if (mfma) {
+    // matrix cores (mode 2): x tiles of 16 in groups of at most 5 per wave, 128
+    // (j, k) pairs per block
+    const int ntl = (nx + 15) / 16;
+    nig = (ntl + 4) / 5;
+    MI = (ntl + nig - 1) / nig;
+    nxp = 16 * MI * nig;
+    per_chunk = (int64_t)nig * ((njk + 127) / 128);
+    switch (MI) {
+      case 1: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<1>, kT, 0); break;
+      case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<2>, kT, 0); break;
+      case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<3>, kT, 0); break;
+      case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<4>, kT, 0); break;
+      default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<5>, kT, 0); break;
+    }
+  } else {
+    // vector units: x values per lane, the choice that wastes the fewest,
+    // the larger on ties
+    const int ibs[5] = {8, 16, 24, 32, 40};
+    int64_t best = INT64_MAX;
+    for (int c : ibs) {
+      const int64_t span = (nx + c - 1) / c * (int64_t)c;
+      if (span <= best) {
+        best = span;
+        IB = c;
+      }
+    }
+    ni = (nx + IB - 1) / IB;
+    const int64_t njkb = (int64_t)((ny + 15) / 16) * ((nz + 15) / 16);  // 16 x 16 (j, k) tiles
+    nxp = (int)(ni * IB);  // x factors zero-padded to whole i blocks
+    per_chunk = ni * njkb;
+    switch (IB) {
+      case 8: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<8>, kT, 0); break;
+      case 16: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<16>, kT, 0); break;
+      case 24: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<24>, kT, 0); break;
+      case 32: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<32>, kT, 0); break;
+      default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<40>, kT, 0); break;
+    }
   }
   if (oe != hipSuccess) return hip_fail(ctx, oe, "occupancy query", __LINE__);
+  const int64_t nfp = (int64_t)nxp + ny + nz;
+  // chunks: the blocks of a batch fill two rounds of the device (every block
+  // does the same work, so a partial last round would idle most CUs)
   const int64_t resident = (int64_t)std::max(1, ncu) * std::max(1, per_cu);
   // quartets per factor batch: within 2^28 doubles (2 GB; 2^26 ran as
   // fast, 2^30 slower as each chunk's rows left the caches), and at most half
@@ -745,17 +850,31 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
     const int64_t qch = (n + cpb - 1) / cpb;
     const dim3 grid((unsigned)per_chunk, (unsigned)cpb);
     const int accumulate = qb > 0;
+    if (mfma) {
+#define INCA_CUBEM(MIV)                                                                      \
+  hipLaunchKernelGGL(k_cube_contract_mfma<MIV>, grid, dim3(kT), 0, st, ctx->d_cf, n, qch, nx, \
+                     nxp, ny, nz, nig, accumulate, ctx->d_partial, G)
+      switch (MI) {
+        case 1: INCA_CUBEM(1); break;
+        case 2: INCA_CUBEM(2); break;
+        case 3: INCA_CUBEM(3); break;
+        case 4: INCA_CUBEM(4); break;
+        default: INCA_CUBEM(5); break;
+      }
+#undef INCA_CUBEM
+    } else {
 #define INCA_CUBE(IBV)                                                                         \
   hipLaunchKernelGGL(k_cube_contract<IBV>, grid, dim3(kT), 0, st, ctx->d_cf, n, qch, nx, ny, nz, \
                      ni, accumulate, ctx->d_partial, G)
-    switch (IB) {
-      case 8: INCA_CUBE(8); break;
-      case 16: INCA_CUBE(16); break;
-      case 24: INCA_CUBE(24); break;
-      case 32: INCA_CUBE(32); break;
-      default: INCA_CUBE(40); break;
-    }
+      switch (IB) {
+        case 8: INCA_CUBE(8); break;
+        case 16: INCA_CUBE(16); break;
+        case 24: INCA_CUBE(24); break;
+        case 32: INCA_CUBE(32); break;
+        default: INCA_CUBE(40); break;
+      }
 #undef INCA_CUBE
+    }
     INCA_HIP(hipGetLastError());
   }
   *nrows = cpb;

@@ -275,7 +275,12 @@ int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
  * The factors are formed once per quartet and coordinate (nx + ny + nz of
  * them) and Ivec(i,j,k) = sum_q Fx_q(i) Fy_q(j) Fz_q(k) is contracted with
  * one FMA per quartet and voxel: every quartet at every voxel, in FP64 (a
- * mixed evaluation too), with no cutoff.  0 evaluates cubes term by term
+ * mixed evaluation too), with no cutoff.  For each chunk of quartets the
+ * contraction is the matrix product of the x factors with the row-wise
+ * Kronecker product of the y and z factors; on = 1 runs it on the vector
+ * units, on = 2 on the FP64 matrix cores (v_mfma_f64_16x16x4f64: the same
+ * chain of fused multiply-adds, bitwise equal when the chunking matches;
+ * 1.6x slower on the 80^3 benchmark cube).  0 evaluates cubes term by term
  * like other grids.  Takes effect at the next evaluation.  No reference
  * counterpart: the reference evaluates each voxel's terms directly
  * (mod_intracule.f90:345-358). */

@@ -131,23 +131,28 @@ def test_repeat_is_bitwise_identical(full):
 
 def test_cube_contraction_matches_ray_tiles(full):
     """The 80^3 cube (0.15 bohr steps, centred at the origin) on the same
-    records: the per-axis contraction (every quartet at every voxel) against
-    the term-by-term ray tiles along z (cutoff on).  Runs last: it replaces
-    the grid."""
+    records: the per-axis contraction (every quartet at every voxel) on the
+    vector units and on the matrix cores, against the term-by-term ray tiles
+    along z (cutoff on).  Runs last: it replaces the grid."""
     e = full["e"]
     n = e.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
     assert n == 512000
+    got = {}
     try:
         e.set_cube_contraction(0)
         ray, cnt = e.evaluate(capi.INCA_FP64, THRESH)
         ray = ray.copy()
-        e.set_cube_contraction(1)
-        got, cnt2 = e.evaluate(capi.INCA_FP64, THRESH)
-        ref_n, ex = e.eval_counts()
+        for mode in (1, 2):
+            e.set_cube_contraction(mode)
+            iv, cnt2 = e.evaluate(capi.INCA_FP64, THRESH)
+            ref_n, ex = e.eval_counts()
+            assert list(cnt2) == list(full["cnt"])
+            assert ex == ref_n == int(cnt[2]) * n
+            got[mode] = iv.copy()
     finally:
         e.set_cube_contraction(1)
-    assert list(cnt) == list(cnt2) == list(full["cnt"])
-    assert ex == ref_n == int(cnt[2]) * n
-    er, ea = _close(got, ray)
-    print("80^3 cube, contraction vs ray tiles: rel %.2e abs %.2e" % (er, ea))
-    assert er < 1e-10 and ea < 1e-13
+    assert list(cnt) == list(full["cnt"])
+    for mode, iv in got.items():
+        er, ea = _close(iv, ray)
+        print("80^3 cube, contraction mode %d vs ray tiles: rel %.2e abs %.2e" % (mode, er, ea))
+        assert er < 1e-10 and ea < 1e-13

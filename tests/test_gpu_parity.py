@@ -432,14 +432,16 @@ def test_ray_tiles_on_cube_lines(engine):
             assert er < 1e-11 and ea < 1e-13
 
 
-@pytest.mark.parametrize("dims", [(9, 7, 20), (41, 5, 6)])
+@pytest.mark.parametrize("dims", [(9, 7, 20), (41, 5, 6), (97, 3, 4)])
 def test_cube_contraction_matches_term_by_term(engine, dims):
     """Cube grids by the per-axis contraction (every quartet at every voxel,
     factors exp(-rp_c^2) P_c(rp_c) per axis, batches of quartets added into
-    the same rows) against the term-by-term point kernel with the cutoff
-    off (the reference's loop).  (9, 7, 20): one block of x values; (41, 5,
-    6): two blocks, the second zero-padded.  More than 8192 prepared
-    quartets, so the factors come in two or more batches."""
+    the same rows), on the vector units (mode 1) and on the matrix cores
+    (mode 2), against the term-by-term point kernel with the cutoff off (the
+    reference's loop).  (9, 7, 20): one x tile, padded; (41, 5, 6): three x
+    tiles, fewer (j, k) pairs than one block; (97, 3, 4): two groups of x
+    tiles.  More than 8192 prepared quartets, so the factors come in two or
+    more batches."""
     s = synth.config_cluster(seed=35, natoms=3, nheavy=2, nocc=3, exp_lo=0.05, exp_hi=2e3,
                              shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
     ijkl, d = s.dm2_unique()
@@ -447,6 +449,7 @@ def test_cube_contraction_matches_term_by_term(engine, dims):
     engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
     engine.set_quartets(ijkl=ijkl, d=d)
     engine.grid_cube([0.1, -0.2, 0.3], [0.3, 0.25, 0.15], list(dims))
+    got = {}
     try:
         engine.set_cube_contraction(0)
         engine.set_point_cutoff(0)
@@ -454,24 +457,29 @@ def test_cube_contraction_matches_term_by_term(engine, dims):
         ref, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
         ref = ref.copy()
         assert int(cnt[2]) > 8192
-        engine.set_cube_contraction(1)
         engine.set_point_cutoff(100)  # the contraction evaluates every term regardless
-        got, _ = engine.evaluate(capi.INCA_FP64, THRESH)
-        got = got.copy()
-        ref_n, ex = engine.eval_counts()
-        assert ex == ref_n == int(cnt[2]) * int(np.prod(dims))
-        mixed, _ = engine.evaluate(capi.INCA_MIXED, THRESH)
-        mixed = mixed.copy()
+        for mode in (1, 2):
+            engine.set_cube_contraction(mode)
+            iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
+            ref_n, ex = engine.eval_counts()
+            assert ex == ref_n == int(cnt[2]) * int(np.prod(dims))
+            mixed, _ = engine.evaluate(capi.INCA_MIXED, THRESH)
+            got[mode] = (iv.copy(), mixed.copy())
     finally:
         engine.set_cube_contraction(1)
         engine.set_point_cutoff(100)
         engine.set_tiles(4, 4)
-    er, ea = _close(got, ref)
-    em, eam = _close(mixed, ref)
-    print("cube %s contraction vs term by term: rel %.2e abs %.2e; mixed rel %.2e abs %.2e" % (
-        dims, er, ea, em, eam))
-    assert er < 1e-11 and ea < 1e-13
-    assert em < 1e-11 and eam < 1e-13
+    for mode, (iv, mixed) in got.items():
+        er, ea = _close(iv, ref)
+        em, eam = _close(mixed, ref)
+        print("cube %s contraction mode %d vs term by term: rel %.2e abs %.2e; mixed rel %.2e "
+              "abs %.2e" % (dims, mode, er, ea, em, eam))
+        assert er < 1e-11 and ea < 1e-13
+        assert em < 1e-11 and eam < 1e-13
+    # the matrix cores add the four products of a step as successive fused
+    # multiply-adds, the same chain as the vector kernel's: with equal
+    # chunking the two modes agree bitwise (measured: equal here)
+    print("modes 1 and 2 bitwise equal: %s" % np.array_equal(got[1][0], got[2][0]))
 
 
 def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
