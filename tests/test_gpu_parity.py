@@ -372,6 +372,20 @@ def test_ray_tiles_match_point_by_point(engine):
         assert rr < 1e-11 and ra < 1e-13
 
 
+def test_cube_contraction_mode_range(engine):
+    """Modes 0 (term by term), 1 (vector units) and 2 (matrix cores) are
+    accepted; anything else is an argument error that leaves the mode as it
+    was."""
+    try:
+        for mode in (0, 2, 1):
+            engine.set_cube_contraction(mode)
+        for bad in (-1, 3):
+            with pytest.raises(capi.IncaError, match="0, 1 or 2"):
+                engine.set_cube_contraction(bad)
+    finally:
+        engine.set_cube_contraction(1)
+
+
 def test_autotune_before_any_grid_loop():
     """inca_autotune straight after inca_prepare on a new context (no grid
     loop has run yet, so no buffer of one exists), on a cube grid with the
