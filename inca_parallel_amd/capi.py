@@ -29,7 +29,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
            "inca_get_eval_counts",
-           "inca_prepare", "inca_evaluate_prepared", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
+           "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
@@ -90,6 +90,7 @@ def load():
     L.inca_get_eval_counts.argtypes = [vp, vp, vp]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
     L.inca_evaluate_prepared.argtypes = [vp, C.c_int32, vp]
+    L.inca_evaluate_prepared_noalloc.argtypes = [vp, C.c_int32, vp, vp]
     L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
                                 vp, vp, vp]
     L.inca_batch_create.argtypes = [C.c_int32, C.c_int32, vp, C.POINTER(vp)]
@@ -471,6 +472,14 @@ class Engine:
         self._ck(self._L.inca_evaluate_prepared(self._h, int(precision), _ptr(iv)),
                  "inca_evaluate_prepared")
         return iv[:self.G] if want_ivec else None
+
+    def evaluate_prepared_noalloc(self, precision, ivec_device_ptr, stream=None):
+        """inca_evaluate_prepared_noalloc: the grid loop into a device array of
+        G doubles (an int address), enqueued on `stream` (an int hipStream_t
+        address, or None for the context's stream); returns without waiting."""
+        self._ck(self._L.inca_evaluate_prepared_noalloc(
+            self._h, int(precision), C.c_void_p(int(ivec_device_ptr)),
+            C.c_void_p(int(stream)) if stream else None), "inca_evaluate_prepared_noalloc")
 
     def ivec(self):
         iv = np.zeros(max(self.G, 1))

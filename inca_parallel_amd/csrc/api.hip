@@ -52,6 +52,10 @@ int inca_create(int32_t device, inca_ctx **out) {
     g_create_err = std::string("hipSetDevice failed: ") + hipGetErrorString(e);
     return INCA_EHIP;
   }
+  if ((e = noalloc_scratch_init(device)) != hipSuccess) {
+    g_create_err = std::string("address of the no-allocation scratch: ") + hipGetErrorString(e);
+    return INCA_EHIP;
+  }
   inca_ctx *c = new inca_ctx();
   c->device = device;
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -381,6 +385,21 @@ int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec) {
   if (ivec && ctx->G > 0)
     INCA_HIP(hipMemcpy(ivec, ctx->d_ivec, ctx->G * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int inca_evaluate_prepared_noalloc(inca_ctx *ctx, int32_t precision, double *ivec,
+                                   void *stream) {
+  CHECK_CTX(ctx);
+  if (ctx->grid_kind == GRID_NONE) return set_err(ctx, INCA_EINVAL, "grid not set");
+  if (ctx->qstride == 0) return set_err(ctx, INCA_EINVAL, "quartets not prepared");
+  if (precision != INCA_FP64 && precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "unknown precision %d", precision);
+  if (precision == INCA_MIXED && ctx->prep_precision != INCA_MIXED)
+    return set_err(ctx, INCA_EINVAL, "quartets were prepared for FP64: prepare with INCA_MIXED");
+  if (!ivec && ctx->G > 0) return set_err(ctx, INCA_EINVAL, "null Ivec");
+  INCA_HIP(hipSetDevice(ctx->device));
+  return evaluate_noalloc(ctx, precision, ivec,
+                          stream ? static_cast<hipStream_t>(stream) : ctx->stream);
 }
 
 int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes, int32_t nohydro,

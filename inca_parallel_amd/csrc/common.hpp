@@ -236,5 +236,9 @@ void atom_features(int natoms, const int32_t *an, const double *cartes, int npri
 int evaluate_grid(inca_ctx *ctx, int precision);
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist);
 int autotune(inca_ctx *ctx, int32_t *best);
+// no-allocation evaluation (eval.hip): per-device scratch set-up, called by
+// inca_create with the device current, and the grid loop on that scratch
+hipError_t noalloc_scratch_init(int device);
+int evaluate_noalloc(inca_ctx *ctx, int precision, double *ivec, hipStream_t st);
 
 }  // namespace inca

@@ -22,6 +22,7 @@
 #include <array>
 #include <cmath>
 #include <map>
+#include <mutex>
 
 #include "common.hpp"
 #include "eval_core.hpp"
@@ -1065,6 +1066,153 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   (void)hipEventDestroy(b);
   ctx->have_ivec = 0;  // the partial buffer no longer matches Ivec
   return rc;
+}
+
+}  // namespace inca
+
+// ---- evaluation without device allocation (inca_evaluate_prepared_noalloc) ----
+//
+// The call's device memory beyond its inputs (the context's grid and
+// prepared quartets) and its output (the caller's Ivec) is the tile records
+// and the per-chunk partial rows of the grid loop.  Both live in `scratch`,
+// a file-scope array of which every device holds its own copy from module
+// load to process end.  Host code never passes the array's name to a kernel
+// (on the host it is only a stand-in symbol): inca_create calls
+// noalloc_scratch_init once per device, with the device current, and the
+// device address hipGetSymbolAddress returns is what the launches take.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;  // 32 MiB per device
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace inca {
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // per device: the device address of `scratch`
+
+double *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+// Point-by-point launch (no cutoff, no ray tiles) of one precision range.
+template <int P>
+void launch_plain(bool mixed, dim3 grid, hipStream_t st, const double *q, int qstride, int64_t n,
+                  int64_t qch, const inca_ctx *ctx, const double *tb, int64_t ntile,
+                  int64_t tiles, int64_t nch, int xcd, double *partial) {
+  if (mixed)
+    hipLaunchKernelGGL((k_eval_mixed<P, false>), grid, dim3(kT), 0, st, q, qstride, n, qch,
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, ctx->G, tb,
+                       (const unsigned long long *)nullptr, ntile, tiles, nch, xcd, partial);
+  else
+    hipLaunchKernelGGL((k_eval_fp64<P, false>), grid, dim3(kT), 0, st, q, qstride, n, qch,
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, ctx->G, tb,
+                       (const unsigned long long *)nullptr, ntile, tiles, nch, xcd, partial);
+}
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < (int)g_scratch_ptr.size() && g_scratch_ptr[device]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  g_scratch_ptr[device] = static_cast<double *>(p);
+  return hipSuccess;
+}
+
+// The grid loop of evaluate_grid, point by point with the cutoff off, its
+// tile records and partial rows in `scratch`, reduced in a fixed order into
+// the caller's device array ivec.  Enqueued on `st`; nothing waits.
+int evaluate_noalloc(inca_ctx *ctx, int precision, double *ivec, hipStream_t st) {
+  double *base = noalloc_scratch(ctx->device);
+  if (!base) return set_err(ctx, INCA_EINVAL, "no-allocation scratch not set up on device %d",
+                            ctx->device);
+  const int64_t G = ctx->G, nq = ctx->q2;
+  const int64_t cap = (int64_t)(kNoAllocScratchBytes / sizeof(double));
+  if (G == 0) return 0;
+  if (nq == 0) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((G + kT - 1) / kT)), dim3(kT), 0, st, ivec, G);
+    INCA_HIP(hipGetLastError());
+    return 0;
+  }
+  // ranges: [0, n64) FP64, [n64, nq) FP32 (mixed); points per lane from the
+  // context's tiles (a ray-tile choice of the FP32 range runs 8 per lane)
+  auto valid_p = [](int p) { return p == 1 || p == 2 || p == 4 || p == 6 || p == 8 || p == 10; };
+  const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
+  const int64_t nr[2] = {n64, nq - n64};
+  const int Pk[2] = {valid_p(ctx->tile_p[0]) ? ctx->tile_p[0] : 4,
+                     valid_p(ctx->tile_p[1]) ? ctx->tile_p[1] : 8};
+  int64_t tiles[2] = {0, 0}, ntile_max = 0, nranges = 0;
+  for (int r = 0; r < 2; ++r) {
+    if (nr[r] == 0) continue;
+    ++nranges;
+    tiles[r] = (G + (int64_t)kT * Pk[r] - 1) / ((int64_t)kT * Pk[r]);
+    ntile_max = std::max<int64_t>(ntile_max, tiles[r] * (kT / 64));
+  }
+  const int64_t tb_d = (ntile_max * kTB + 31) / 32 * 32;  // 256-byte aligned rows follow
+  if (tb_d + nranges * G > cap)
+    return set_err(ctx, INCA_ERANGE,
+                   "no-allocation evaluation: a grid of %lld points needs %lld bytes of "
+                   "scratch, more than the %zu it holds",
+                   (long long)G, (long long)((tb_d + nranges * G) * 8), kNoAllocScratchBytes);
+  const int64_t rows = (cap - tb_d) / G;
+  int64_t qch[2] = {1, 1}, nch[2] = {0, 0};
+  for (int r = 0; r < 2; ++r)
+    if (nr[r] > 0) chunking(nr[r], tiles[r], (double)nr[r] / (double)nq, &qch[r], &nch[r]);
+  if (nch[0] + nch[1] > rows) {
+    // fewer, longer chunks: rows shared in proportion to the quartets
+    // (the floors sum to at most `rows`, and nranges <= rows)
+    for (int r = 0; r < 2; ++r) {
+      if (nr[r] == 0) continue;
+      const int64_t budget =
+          std::max<int64_t>(1, (int64_t)((double)rows * (double)nr[r] / (double)nq));
+      qch[r] = (nr[r] + budget - 1) / budget;
+      nch[r] = (nr[r] + qch[r] - 1) / qch[r];
+    }
+  }
+  double *tb = base;
+  double *partial = base + tb_d;
+  hipLaunchKernelGGL(k_fill_tiles, dim3((unsigned)((ntile_max + kT - 1) / kT)), dim3(kT), 0, st,
+                     ntile_max, tb);
+  INCA_HIP(hipGetLastError());
+  int64_t row0 = 0;
+  for (int r = 0; r < 2; ++r) {
+    if (nr[r] == 0) continue;
+    const int xcd = nch[r] >= kXcds;
+    const int64_t nblocks = tiles[r] * (xcd ? (nch[r] + kXcds - 1) / kXcds * kXcds : nch[r]);
+    const dim3 grid((unsigned)nblocks);
+    const double *q = ctx->d_q + (r == 0 ? 0 : n64 * ctx->qstride);
+    const int64_t ntile = tiles[r] * (kT / 64);
+    double *prow = partial + row0 * G;
+    const bool mixed = r == 1;
+    switch (Pk[r]) {
+      case 1: launch_plain<1>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+      case 2: launch_plain<2>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+      case 6: launch_plain<6>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+      case 8: launch_plain<8>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+      case 10: launch_plain<10>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+      default: launch_plain<4>(mixed, grid, st, q, ctx->qstride, nr[r], qch[r], ctx, tb, ntile, tiles[r], nch[r], xcd, prow); break;
+    }
+    INCA_HIP(hipGetLastError());
+    row0 += nch[r];
+  }
+  // chunk rows to Ivec in a fixed order, through slices where the scratch
+  // has room for them
+  const int64_t nchunks = row0;
+  const int64_t pblk = (G + kT - 1) / kT;
+  int64_t nslice = std::min<int64_t>({nchunks, std::max<int64_t>(1, 2048 / pblk), 65535,
+                                      rows - nchunks});
+  if (nslice > 1 && nchunks >= 4 * nslice) {
+    double *red = partial + nchunks * G;
+    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)pblk, (unsigned)nslice), dim3(kT), 0, st,
+                       partial, (int)nchunks, (int)nslice, G, red);
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)pblk), dim3(kT), 0, st, red, (int)nslice, G, ivec);
+  } else {
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)pblk), dim3(kT), 0, st, partial, (int)nchunks, G,
+                       ivec);
+  }
+  INCA_HIP(hipGetLastError());
+  return 0;
 }
 
 }  // namespace inca

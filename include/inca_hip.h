@@ -140,6 +140,28 @@ int inca_prepare(inca_ctx *ctx, int32_t precision, double thresh,
  * with INCA_MIXED (the FP64/FP32 split is made in the pre-pass). */
 int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec);
 
+/* inca_evaluate_prepared without device allocation.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or allocating library routine
+ * and does not wait: the grid loop over the quartets the last inca_prepare /
+ * inca_evaluate prepared, on the current grid, is enqueued on `stream` (a
+ * hipStream_t; NULL: the context's stream), and its result lands in ivec, a
+ * DEVICE pointer to G doubles owned by the caller.  Every grid kind is
+ * evaluated point by point with the per-point cutoff off (cube grids are not
+ * contracted per axis), and the FP32 range of INCA_MIXED always runs the
+ * mixed kernel: the result is that of inca_evaluate_prepared after
+ * inca_set_point_cutoff(ctx, 0), inca_set_cube_contraction(ctx, 0) and
+ * non-zero FP32 tiles, up to summation order.  The tile records and partial
+ * sums live in a static 32 MiB array per device, held until the process
+ * ends: the grid must fit one partial row of G doubles per precision range
+ * plus the tile records (about 4 million points in FP64, 2 million mixed);
+ * a larger grid fails with INCA_ERANGE before anything is enqueued.  Every
+ * call on a device shares that array and leaves its contents behind, so
+ * calls that overlap on one device (any contexts, any streams) race. */
+int inca_evaluate_prepared_noalloc(inca_ctx *ctx, int32_t precision, double *ivec,
+                                   void *stream);
+
 /* Integration centres of the multicentre grid, replacing centercalc
  * (mod_autogrid.f90:12-131): the origin, +-(R_j - R_k) for heavy-atom pairs
  * (all pairs when nohydro = 0), z snapped to 0 below 0.5 bohr, centres
