@@ -23,13 +23,18 @@
  *  - TFVC: partition of unity, inversion symmetry, integrals of normalised
  *    Gaussians;
  *  - centercalc: hand-computed centres for CH4 and two-heavy-atom cases;
- *  - smearing: integral of P(u) against sum w*I.
- * The ill-conditioned polycoef branch (|C| > 1e3 zeroing decided by
- * roundoff) is parity unpinned: no reference output covers it.
+ *  - smearing: the closed form of one point (a normal density on the
+ *    reference's window, I_avg = P / (4 pi u^2)), linearity;
+ *  - the |C| > 1e3 zeroing of ill-conditioned polycoef fits: a 60-digit
+ *    audit (tests/golden/polycoef_audit.py) that classifies every fit as
+ *    decided by the mathematics or by rounding; every decision the
+ *    mathematics fixes is checked in both arithmetic modes (orc_set_arith)
+ *    and in an FMA build (tests/test_polycoef_audit.py).
  */
 #include "inca_oracle.h"
 
 #include <float.h>
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -49,6 +54,27 @@ static double ipow(double x, int e) {
   for (int k = 0; k < e; ++k) r = (k == 0) ? x : r * x;
   return r;
 }
+
+/* Arithmetic mode (orc_set_arith).  0, the default: integer powers by
+ * repeated multiplication and Gauss-Hermite roots as correctly rounded
+ * doubles -- the operation sequence the device pre-pass follows.  1: the
+ * reference's literal forms through libm: x**dble(n) as pow(x, n)
+ * (mod_intrastuff.f90:118-122, 153, 189-192), JA8 as one left-to-right
+ * product with pi**1.5 and (2 aik)**-(L+1.5) (:117-122), the Gauss-Hermite
+ * roots as 16-significant-digit decimals in the reference's node order
+ * (:222-298; x_max of two nodes to 15 digits, :44-113) and the Hermite
+ * polynomials of the weight formula written out in powers (:312-340).
+ * Both are FP64 implementations of the same algorithm; on well-conditioned
+ * fits they agree to rounding, and their spread on ill-conditioned fits is
+ * what mode 1 exists to measure.  Set it before a run: it is global. */
+static int g_arith = 0;
+/* per-thread override of the mode (orc_fit_census), -1 when unset */
+static _Thread_local int t_arith = -1;
+static inline int arith_mode(void) { return t_arith >= 0 ? t_arith : g_arith; }
+void orc_set_arith(int mode) { g_arith = mode; }
+int orc_get_arith(void) { return g_arith; }
+
+static double rpow(double x, int e) { return arith_mode() ? pow(x, (double)e) : ipow(x, e); }
 
 /* Gauss-Hermite roots for n = 1..11 nodes (ascending).  Values are the
  * roots of H_n computed with numpy.polynomial.hermite.hermgauss; they are
@@ -112,6 +138,70 @@ static double hermite_nm1(double x, int n) {
   return h1;
 }
 
+/* The root of H_n nearest x0, refined by Newton steps in long double
+ * (64-bit significand) and rounded to `digits` significant decimal digits,
+ * as the reference writes its root constants. */
+static double root_decimal(double x0, int n, int digits) {
+  long double x = x0;
+  for (int it = 0; it < 4; ++it) {
+    long double h0 = 1.0L, h1 = 2.0L * x; /* H_0, H_1 */
+    for (int k = 2; k <= n; ++k) {
+      long double h2 = 2.0L * x * h1 - 2.0L * (k - 1) * h0;
+      h0 = h1;
+      h1 = h2;
+    }
+    /* H_n' = 2 n H_(n-1) */
+    if (n >= 1 && h0 != 0.0L) x -= h1 / (2.0L * n * h0);
+  }
+  char buf[64];
+  snprintf(buf, sizeof buf, "%.*Lg", digits, x);
+  return strtod(buf, NULL);
+}
+
+/* Gauss-Hermite nodes of mode 1: the roots to 16 significant digits
+ * (0 exactly for the middle node), in the reference's order -- ascending,
+ * except for five nodes: -r2, -r1, 0, r2, r1 with r2 > r1
+ * (mod_intrastuff.f90:222-298). */
+static void gh_roots_ref(int n, double *rh) {
+  double r[11];
+  for (int i = 0; i < n; ++i) {
+    double x0 = GH_ROOT[n - 1][i];
+    r[i] = (x0 == 0.0) ? 0.0 : root_decimal(x0, n, 16);
+  }
+  if (n == 5) {
+    rh[0] = r[0];
+    rh[1] = r[1];
+    rh[2] = r[2];
+    rh[3] = r[4];
+    rh[4] = r[3];
+  } else {
+    for (int i = 0; i < n; ++i) rh[i] = r[i];
+  }
+}
+
+/* H_(n-1)(x) written out in powers of x (physicists' Hermite polynomials,
+ * coefficients from the three-term recurrence), each power through libm pow
+ * as the reference's x**k (mod_intrastuff.f90:312-340). */
+static double hermite_powers(double x, int n) {
+  double c[12] = {1.0}, prev[12] = {0.0}, cur[12];
+  int deg = 0; /* c holds H_deg */
+  for (; deg < n - 1; ++deg) {
+    for (int m = 0; m < 12; ++m) cur[m] = 0.0;
+    for (int m = 0; m <= deg; ++m) cur[m + 1] += 2.0 * c[m];
+    for (int m = 0; m < 12; ++m) cur[m] -= 2.0 * deg * prev[m];
+    for (int m = 0; m < 12; ++m) {
+      prev[m] = c[m];
+      c[m] = cur[m];
+    }
+  }
+  double h = 0.0;
+  for (int m = deg; m >= 0; --m) {
+    if (c[m] == 0.0) continue;
+    h += (m == 0) ? c[m] : c[m] * (m == 1 ? x : pow(x, (double)m));
+  }
+  return h;
+}
+
 /* mod_intrastuff.f90:198-341.  n = floor(L/2)+1 nodes; weight
  * w_i = 2^(n-1) n! sqrt(pi) / (n^2 H_{n-1}(x_i)^2).  Returns n, or -1 for
  * L > 21 (the reference prints an error and continues with garbage). */
@@ -125,6 +215,11 @@ int orc_gauherm(int L, double *rh, double *wh) {
   double factn = 1.0;
   for (int i = 2; i <= n; ++i) factn *= i;
   double cons = pow(2.0, (double)(n - 1)) * factn * sqrt(PI) / (double)(n * n);
+  if (arith_mode()) {
+    gh_roots_ref(n, rh);
+    for (int i = 0; i < n; ++i) wh[i] = cons * (1.0 / pow(hermite_powers(rh[i], n), 2.0));
+    return n;
+  }
   for (int i = 0; i < n; ++i) {
     rh[i] = GH_ROOT[n - 1][i];
     double h = hermite_nm1(rh[i], n);
@@ -133,10 +228,12 @@ int orc_gauherm(int L, double *rh, double *wh) {
   return n;
 }
 
-/* Largest Gauss-Hermite root for nn nodes (JA8's x_max, :44-113). */
+/* Largest Gauss-Hermite root for nn nodes (JA8's x_max, :44-113); mode 1
+ * takes the reference's decimals (16 digits, 15 for two nodes). */
 static double gh_xmax(int nn) {
   if (nn <= 1) return 0.0;
   if (nn > 11) nn = 11; /* undefined in the reference beyond 10 */
+  if (arith_mode()) return root_decimal(GH_ROOT[nn - 1][nn - 1], nn, nn == 2 ? 15 : 16);
   return GH_ROOT[nn - 1][nn - 1];
 }
 
@@ -150,6 +247,11 @@ double orc_ja8(int ti, int tk, int mi, int mk, int ni, int nk, double alfi,
   double L = (double)(ti + tk + mi + mk + ni + nk);
   double dx = fabs(xk - xi), dy = fabs(yk - yi), dz = fabs(zk - zi);
   const double two_a = 2.0 * aik;
+  if (arith_mode()) /* the reference's product, left to right (:117-122) */
+    return pow(PI, 1.5) * pow(two_a, -(L + 1.5)) * exp(-2.0 * eik * rik2) *
+           pow(xm0 + alfk * s * dx, 2.0 * ti) * pow(xm0 + alfi * s * dx, 2.0 * tk) *
+           pow(xm1 + alfk * s * dy, 2.0 * mi) * pow(xm1 + alfi * s * dy, 2.0 * mk) *
+           pow(xm2 + alfk * s * dz, 2.0 * ni) * pow(xm2 + alfi * s * dz, 2.0 * nk);
   double J = 5.568327996831708 * (1.0 / (ipow(two_a, (int)L + 1) * sqrt(two_a))) *
              exp(-2.0 * eik * rik2);
   J *= ipow(xm0 + alfk * s * dx, 2 * ti) * ipow(xm0 + alfi * s * dx, 2 * tk);
@@ -225,8 +327,8 @@ static double wijkl(double ri, double rj, double rk, double rl, double rijkl,
                     double invz, double alf) {
   double a = invz * rhat;
   double ap = (alf + 0.5) * r, am = (alf - 0.5) * r;
-  return ipow(a + am + (rijkl - ri), ti) * ipow(a + ap + (rijkl - rj), tj) *
-         ipow(a + am + (rijkl - rk), tk) * ipow(a + ap + (rijkl - rl), tl);
+  return rpow(a + am + (rijkl - ri), ti) * rpow(a + ap + (rijkl - rj), tj) *
+         rpow(a + am + (rijkl - rk), tk) * rpow(a + ap + (rijkl - rl), tl);
 }
 
 /* mod_intrastuff.f90:127-175 */
@@ -249,7 +351,7 @@ int orc_polycoef(double ri, double rj, double rk, double rl, double rik,
   for (int i = 0; i < np; ++i) {
     int l1 = L;
     for (int j = 0; j < np; ++j) {
-      M[i + np * j] = ipow(sqe * (xx[i] + rik - rjl), l1);
+      M[i + np * j] = rpow(sqe * (xx[i] + rik - rjl), l1);
       --l1;
     }
   }
@@ -305,6 +407,24 @@ static void exact_poly(const double R[4], double rijkl, const int t[4], int L,
   for (int q = 0; q <= L; ++q) E[q] = acc[L - q];
 }
 
+/* mod_intracule.f90:307-312: the Gaussian-product quantities of a quartet
+ * passing screen 1 (R[ax][0..3] = coordinate of primitives i, j, k, l). */
+static void quartet_geometry(double alfi, double alfj, double alfk, double alfl,
+                             double *const R[3], double *sqe, double *invz, double *alfijkl,
+                             double P_ik[3], double P_jl[3], double P[3]) {
+  double aik = alfi + alfk, ajl = alfj + alfl;
+  double zeta = aik + ajl;
+  double eijkl = (aik * ajl) * (1.0 / zeta);
+  *sqe = sqrt(eijkl);
+  for (int ax = 0; ax < 3; ++ax) {
+    P_ik[ax] = (alfi * R[ax][0] + alfk * R[ax][2]) * (1.0 / aik);
+    P_jl[ax] = (alfj * R[ax][1] + alfl * R[ax][3]) * (1.0 / ajl);
+    P[ax] = (aik * P_ik[ax] + ajl * P_jl[ax]) * (1.0 / zeta);
+  }
+  *invz = 1.0 / sqrt(zeta);
+  *alfijkl = 0.5 * (aik - ajl) * (1.0 / zeta);
+}
+
 static inline int TMN(const int32_t *tmn, int nprim, int i, int ax) {
   return tmn[i + nprim * ax];
 }
@@ -345,17 +465,10 @@ void orc_prepare_quartet(int nprim, const double *alpha, const int32_t *tmn,
   if (!(q->screen1 >= lim)) return;
   q->pass1 = 1;
   double zeta = aik + ajl;
-  double eijkl = (aik * ajl) * (1.0 / zeta);
-  double sqe = sqrt(eijkl);
+  double sqe, invz, alfijkl;
   double P_ik[3], P_jl[3], P[3];
   double *R[3] = {X, Y, Z};
-  for (int ax = 0; ax < 3; ++ax) {
-    P_ik[ax] = (alfi * R[ax][0] + alfk * R[ax][2]) * (1.0 / aik);
-    P_jl[ax] = (alfj * R[ax][1] + alfl * R[ax][3]) * (1.0 / ajl);
-    P[ax] = (aik * P_ik[ax] + ajl * P_jl[ax]) * (1.0 / zeta);
-  }
-  double invz = 1.0 / sqrt(zeta);
-  double alfijkl = 0.5 * (aik - ajl) * (1.0 / zeta);
+  quartet_geometry(alfi, alfj, alfk, alfl, R, &sqe, &invz, &alfijkl, P_ik, P_jl, P);
   double A = DMval * (1.0 / (zeta * sqrt(zeta))) * exp(-eik * rik2 - ejl * rjl2);
   double U[3];
   for (int ax = 0; ax < 3; ++ax) {
@@ -839,4 +952,206 @@ void orc_smearing(int64_t G, const double *coords, const double *weights,
   }
   for (int j = 0; j < n_bins; ++j)
     Iavg[j] = (u[j] > 1e-6) ? P[j] * (1.0 / (4.0 * pi * u[j] * u[j])) : 0.0;
+}
+
+/* The polynomial fits (mod_intrastuff.f90:127-175) of the records that pass
+ * screen 1, as rows of their FP64 inputs: for every such record and every
+ * axis of degree L >= 1, ORC_FIT_ROW doubles
+ *   [record (0-based), axis, Ri, Rj, Rk, Rl, P_ik, P_jl, P, ti, tj, tk, tl,
+ *    np = L + 1, sqe, invz, alfijkl]
+ * (what polycoef receives at mod_intracule.f90:323-341).  Returns the number
+ * of rows (rows may be NULL to count them), or -1 if cap is too small.
+ * Test infrastructure for the roundoff audit of the fits. */
+int64_t orc_fit_rows(int nprim, const double *alpha, const int32_t *tmn, const double *x,
+                     const double *y, const double *z, int normalize, double thresh,
+                     int64_t nrec, const int32_t *rec_ijkl, const double *rec_d, double *rows,
+                     int64_t cap) {
+  double *np_ = NULL;
+  if (normalize) {
+    np_ = (double *)malloc(sizeof(double) * nprim);
+    for (int i = 0; i < nprim; ++i)
+      np_[i] = orc_normprim(alpha[i], TMN(tmn, nprim, i, 0), TMN(tmn, nprim, i, 1),
+                           TMN(tmn, nprim, i, 2));
+  }
+  const double lim = thresh / ((double)nprim * (nprim + 1) / 2.0);
+  int64_t n = 0;
+  for (int64_t r = 0; r < nrec; ++r) {
+    const int32_t *q4 = rec_ijkl + 4 * r;
+    int ok = q4[0] >= 1 && q4[0] <= nprim;
+    for (int c = 1; c < 4; ++c) ok = ok && q4[c] >= 1 && q4[c] <= nprim;
+    if (!ok) break;
+    orc_quartet q;
+    orc_prepare_quartet(nprim, alpha, tmn, x, y, z, np_, lim, q4[0], q4[1], q4[2], q4[3],
+                        rec_d[r], &q);
+    if (!q.pass1) continue;
+    const int id[4] = {q4[0] - 1, q4[1] - 1, q4[2] - 1, q4[3] - 1};
+    double X[4], Y[4], Z[4];
+    for (int a = 0; a < 4; ++a) {
+      X[a] = x[id[a]];
+      Y[a] = y[id[a]];
+      Z[a] = z[id[a]];
+    }
+    double *R[3] = {X, Y, Z};
+    double sqe, invz, alf, P_ik[3], P_jl[3], P[3];
+    quartet_geometry(alpha[id[0]], alpha[id[1]], alpha[id[2]], alpha[id[3]], R, &sqe, &invz,
+                     &alf, P_ik, P_jl, P);
+    for (int ax = 0; ax < 3; ++ax) {
+      if (q.L[ax] < 1) continue;
+      if (rows) {
+        if (n >= cap) {
+          free(np_);
+          return -1;
+        }
+        double *w = rows + n * ORC_FIT_ROW;
+        w[0] = (double)r;
+        w[1] = ax;
+        for (int a = 0; a < 4; ++a) w[2 + a] = R[ax][a];
+        w[6] = P_ik[ax];
+        w[7] = P_jl[ax];
+        w[8] = P[ax];
+        for (int a = 0; a < 4; ++a) w[9 + a] = TMN(tmn, nprim, id[a], ax);
+        w[13] = q.L[ax] + 1;
+        w[14] = sqe;
+        w[15] = invz;
+        w[16] = alf;
+      }
+      ++n;
+    }
+  }
+  free(np_);
+  return n;
+}
+
+/* orc_polycoef on every row of orc_fit_rows, in the current arithmetic
+ * mode: C (n x 22, C[0] = highest power) and status (1: the reference would
+ * zero the fit).  Returns the number of zeroed fits. */
+int64_t orc_polycoef_rows(int64_t n, const double *rows, double *C, int32_t *status) {
+  int64_t bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+  for (int64_t i = 0; i < n; ++i) {
+    const double *w = rows + i * ORC_FIT_ROW;
+    double c[22] = {0.0};
+    const int st = orc_polycoef(w[2], w[3], w[4], w[5], w[6], w[7], w[8], (int)w[9], (int)w[10],
+                                (int)w[11], (int)w[12], (int)w[13], w[14], w[15], w[16], c);
+    for (int k = 0; k < 22; ++k) C[i * 22 + k] = c[k];
+    status[i] = st;
+    bad += st;
+  }
+  return bad;
+}
+
+/* Census of the fits near the reference's |C| > 1e3 rule: the rows of
+ * orc_fit_rows (records passing screen 1, axes of degree >= 1) whose fit is
+ * zeroed, or has max |C| >= cmin, in arithmetic mode 0 or mode 1 of this
+ * build.  Records are split over OpenMP threads in contiguous ranges and the
+ * rows come out in record order.  Every record must hold valid indices.
+ * Returns the number of rows written (at most cap), or -1 on a bad index
+ * or allocation failure.  Test infrastructure for the roundoff audit. */
+int64_t orc_fit_census(int nprim, const double *alpha, const int32_t *tmn, const double *x,
+                       const double *y, const double *z, int normalize, double thresh,
+                       int64_t nrec, const int32_t *rec_ijkl, const double *rec_d, double cmin,
+                       double *rows, int64_t cap) {
+  for (int64_t r = 0; r < 4 * nrec; ++r)
+    if (rec_ijkl[r] < 1 || rec_ijkl[r] > nprim) return -1;
+  double *np_ = NULL;
+  if (normalize) {
+    np_ = (double *)malloc(sizeof(double) * nprim);
+    for (int i = 0; i < nprim; ++i)
+      np_[i] = orc_normprim(alpha[i], TMN(tmn, nprim, i, 0), TMN(tmn, nprim, i, 1),
+                           TMN(tmn, nprim, i, 2));
+  }
+  const double lim = thresh / ((double)nprim * (nprim + 1) / 2.0);
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  double **buf = (double **)calloc(nt, sizeof(double *));
+  int64_t *cnt = (int64_t *)calloc(nt, sizeof(int64_t));
+  int fail = 0;
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    const int64_t r0 = nrec * t / nt, r1 = nrec * (t + 1) / nt;
+    int64_t bcap = 1024;
+    double *b = (double *)malloc(sizeof(double) * ORC_FIT_ROW * bcap);
+    for (int64_t r = r0; r < r1 && b; ++r) {
+      const int32_t *q4 = rec_ijkl + 4 * r;
+      orc_quartet q;
+      /* screen 1 in mode 0 (the rows' inputs do not depend on the mode) */
+      t_arith = 0;
+      orc_prepare_quartet(nprim, alpha, tmn, x, y, z, np_, lim, q4[0], q4[1], q4[2], q4[3],
+                          rec_d[r], &q);
+      if (!q.pass1) continue;
+      const int id[4] = {q4[0] - 1, q4[1] - 1, q4[2] - 1, q4[3] - 1};
+      double X[4], Y[4], Z[4];
+      for (int a = 0; a < 4; ++a) {
+        X[a] = x[id[a]];
+        Y[a] = y[id[a]];
+        Z[a] = z[id[a]];
+      }
+      double *R[3] = {X, Y, Z};
+      double sqe, invz, alf, P_ik[3], P_jl[3], P[3];
+      quartet_geometry(alpha[id[0]], alpha[id[1]], alpha[id[2]], alpha[id[3]], R, &sqe, &invz,
+                       &alf, P_ik, P_jl, P);
+      for (int ax = 0; ax < 3; ++ax) {
+        if (q.L[ax] < 1) continue;
+        int t4[4];
+        for (int a = 0; a < 4; ++a) t4[a] = TMN(tmn, nprim, id[a], ax);
+        int keep = 0;
+        for (int mode = 0; mode < 2 && !keep; ++mode) {
+          double c[22];
+          t_arith = mode;
+          const int st = orc_polycoef(R[ax][0], R[ax][1], R[ax][2], R[ax][3], P_ik[ax], P_jl[ax],
+                                      P[ax], t4[0], t4[1], t4[2], t4[3], q.L[ax] + 1, sqe, invz,
+                                      alf, c);
+          double cm = 0.0;
+          for (int k = 0; k <= q.L[ax]; ++k) cm = fmax(cm, fabs(c[k]));
+          keep = st || cm >= cmin;
+        }
+        if (!keep) continue;
+        if (cnt[t] == bcap) {
+          bcap *= 2;
+          double *nb = (double *)realloc(b, sizeof(double) * ORC_FIT_ROW * bcap);
+          if (!nb) {
+            free(b);
+            b = NULL;
+            break;
+          }
+          b = nb;
+        }
+        double *w = b + cnt[t] * ORC_FIT_ROW;
+        w[0] = (double)r;
+        w[1] = ax;
+        for (int a = 0; a < 4; ++a) w[2 + a] = R[ax][a];
+        w[6] = P_ik[ax];
+        w[7] = P_jl[ax];
+        w[8] = P[ax];
+        for (int a = 0; a < 4; ++a) w[9 + a] = t4[a];
+        w[13] = q.L[ax] + 1;
+        w[14] = sqe;
+        w[15] = invz;
+        w[16] = alf;
+        cnt[t]++;
+      }
+    }
+    if (!b) {
+#pragma omp atomic write
+      fail = 1;
+    }
+    t_arith = -1;
+    buf[t] = b;
+  }
+  int64_t n = 0;
+  for (int t = 0; t < nt && !fail; ++t) {
+    for (int64_t i = 0; i < cnt[t] && n < cap; ++i, ++n)
+      if (rows) memcpy(rows + n * ORC_FIT_ROW, buf[t] + i * ORC_FIT_ROW, sizeof(double) * ORC_FIT_ROW);
+  }
+  for (int t = 0; t < nt; ++t) free(buf[t]);
+  free(buf);
+  free(cnt);
+  free(np_);
+  return fail ? -1 : n;
 }

@@ -123,6 +123,29 @@ void orc_smearing(int64_t G, const double *coords, const double *weights,
                   const double *density, double u_max, int n_bins,
                   double sigma, double *u, double *P, double *Iavg);
 
+/* Arithmetic of powers, Gauss-Hermite roots and weights: 0 (default)
+ * repeated multiplication and correctly rounded roots; 1 the reference's
+ * literal libm forms (see inca_oracle.c).  Global; set before a run. */
+void orc_set_arith(int mode);
+int orc_get_arith(void);
+
+/* Inputs of every polynomial fit of the records passing screen 1 (axes of
+ * degree >= 1), ORC_FIT_ROW doubles each; and the fits of such rows in the
+ * current arithmetic mode.  Test infrastructure for the roundoff audit of
+ * mod_intrastuff.f90:127-175. */
+#define ORC_FIT_ROW 17
+int64_t orc_fit_rows(int nprim, const double *alpha, const int32_t *tmn, const double *x,
+                     const double *y, const double *z, int normalize, double thresh,
+                     int64_t nrec, const int32_t *rec_ijkl, const double *rec_d, double *rows,
+                     int64_t cap);
+int64_t orc_polycoef_rows(int64_t n, const double *rows, double *C, int32_t *status);
+/* The rows whose fit is zeroed or has max |C| >= cmin in mode 0 or 1
+ * (OpenMP over records, rows in record order, at most cap written). */
+int64_t orc_fit_census(int nprim, const double *alpha, const int32_t *tmn, const double *x,
+                       const double *y, const double *z, int normalize, double thresh,
+                       int64_t nrec, const int32_t *rec_ijkl, const double *rec_d, double cmin,
+                       double *rows, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,9 @@ def build(force: bool = False) -> str:
     return path
 
 
+_LIB_FMA = None
+
+
 def lib():
     global _LIB
     if _LIB is not None:
@@ -37,7 +40,25 @@ def lib():
     path = os.path.join(_HERE, "liborc.so")
     if not os.path.exists(path):
         build()
-    L = C.CDLL(path)
+    _LIB = _configure(C.CDLL(path))
+    return _LIB
+
+
+def lib_fma():
+    """The same oracle built with fused multiply-adds wherever the compiler
+    contracts them (liborc_fma.so), as the reference's release build does on
+    an FMA host: a second FP64 implementation for the roundoff audit."""
+    global _LIB_FMA
+    if _LIB_FMA is not None:
+        return _LIB_FMA
+    path = os.path.join(_HERE, "liborc_fma.so")
+    if not os.path.exists(path):
+        build(force=True)
+    _LIB_FMA = _configure(C.CDLL(path))
+    return _LIB_FMA
+
+
+def _configure(L):
     L.orc_dfact.argtypes = [C.c_int]
     L.orc_dfact.restype = C.c_int
     L.orc_normprim.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int]
@@ -74,8 +95,86 @@ def lib():
     L.orc_bs_radius.restype = C.c_double
     L.orc_smearing.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_int,
                                C.c_double, _dp, _dp, _dp]
-    _LIB = L
+    L.orc_set_arith.argtypes = [C.c_int]
+    L.orc_get_arith.restype = C.c_int
+    L.orc_fit_rows.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp, C.c_int, C.c_double,
+                               C.c_int64, _ip, _dp, C.c_void_p, C.c_int64]
+    L.orc_fit_rows.restype = C.c_int64
+    L.orc_polycoef_rows.argtypes = [C.c_int64, _dp, _dp, _ip]
+    L.orc_polycoef_rows.restype = C.c_int64
+    L.orc_fit_census.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp, C.c_int, C.c_double,
+                                 C.c_int64, _ip, _dp, C.c_double, C.c_void_p, C.c_int64]
+    L.orc_fit_census.restype = C.c_int64
     return L
+
+
+FIT_ROW = 17  # ORC_FIT_ROW
+
+
+class arith:
+    """Context manager: run the oracle `L` (default liborc.so) in arithmetic
+    mode `mode` (0: repeated multiplication, 1: the reference's libm forms)."""
+
+    def __init__(self, mode, L=None):
+        self.L = L or lib()
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = self.L.orc_get_arith()
+        self.L.orc_set_arith(int(self.mode))
+        return self.L
+
+    def __exit__(self, *a):
+        self.L.orc_set_arith(self.prev)
+
+
+def fit_rows(prims, ijkl, d, thresh, normalize=True, L=None):
+    """(n, FIT_ROW) inputs of every polynomial fit (axes of degree >= 1) of
+    the records passing screen 1: [record, axis, Ri, Rj, Rk, Rl, P_ik, P_jl,
+    P, ti, tj, tk, tl, np, sqe, invz, alfijkl]."""
+    L = L or lib()
+    alpha = _f64(prims["alpha"])
+    nprim = len(alpha)
+    tmn = _i32(np.asarray(prims["tmn"]).reshape(nprim, 3).T.reshape(-1))
+    ijkl = _i32(ijkl).reshape(-1)
+    d = _f64(d)
+    args = (nprim, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]), _f64(prims["z"]),
+            int(normalize), float(thresh), len(d), ijkl, d)
+    n = L.orc_fit_rows(*args, None, 0)
+    rows = np.zeros((max(n, 1), FIT_ROW))
+    m = L.orc_fit_rows(*args, rows.ctypes.data, n)
+    assert m == n
+    return rows[:n]
+
+
+def fit_census(prims, ijkl, d, thresh, cmin, normalize=True, L=None, cap=1 << 20):
+    """Rows (as fit_rows) of the fits zeroed, or with max |C| >= cmin, in
+    arithmetic mode 0 or 1 of library L (OpenMP over records)."""
+    L = L or lib()
+    alpha = _f64(prims["alpha"])
+    nprim = len(alpha)
+    tmn = _i32(np.asarray(prims["tmn"]).reshape(nprim, 3).T.reshape(-1))
+    ijkl = _i32(ijkl).reshape(-1)
+    d = _f64(d)
+    rows = np.zeros((cap, FIT_ROW))
+    n = L.orc_fit_census(nprim, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]),
+                         _f64(prims["z"]), int(normalize), float(thresh), len(d), ijkl, d,
+                         float(cmin), rows.ctypes.data, cap)
+    assert 0 <= n < cap, n
+    return rows[:n]
+
+
+def polycoef_rows(rows, L=None):
+    """Fits of the rows in the current arithmetic mode of `L`: (C (n,22) with
+    C[:,0] the highest power, status (n,) with 1 where the reference would
+    zero the fit)."""
+    L = L or lib()
+    rows = _f64(rows)
+    n = rows.shape[0]
+    Cc = np.zeros((max(n, 1), 22))
+    st = np.zeros(max(n, 1), np.int32)
+    L.orc_polycoef_rows(n, rows.reshape(-1) if n else np.zeros(FIT_ROW), Cc, st)
+    return Cc[:n], st[:n]
 
 
 class OrcQuartet(C.Structure):
@@ -140,10 +239,11 @@ def normprim(alpha, tmn):
 
 def intracule(prims: dict, ijkl: np.ndarray, d: np.ndarray, r: np.ndarray,
               thresh: float, normalize: bool = True, nthreads: int = 0,
-              dynamic: bool = False, exact: bool = False):
+              dynamic: bool = False, exact: bool = False, L=None):
     """Quartet loop of the reference; returns (ivec, counts[4]).
-    exact=True evaluates the exact axis polynomials instead of the fits."""
-    L = lib()
+    exact=True evaluates the exact axis polynomials instead of the fits.
+    L: the oracle library to run (default liborc.so)."""
+    L = L or lib()
     alpha = _f64(prims["alpha"])
     nprim = len(alpha)
     tmn = _i32(np.asarray(prims["tmn"]).reshape(nprim, 3).T.reshape(-1))

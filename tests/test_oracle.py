@@ -263,3 +263,50 @@ def test_smearing_conserves_the_integral():
     u, P, Ia = O.smearing(pts, w, dens, u_max=12.0, n_bins=1000, sigma=0.15)
     bw = u[1] - u[0]
     assert abs(np.sum(P) * bw - np.sum(w * dens)) < 2e-3 * np.sum(w * dens)
+
+
+def _smear_closed_form(u, pts, w, dens, sigma):
+    """P(u_j) of radial_smearing (mod_intracule.f90:576-649) written from its
+    definition: every point p spreads w I onto the bins as a normal density
+    of width sigma centred at |p|, over the bins from 4 sigma above |p| down
+    to 4 sigma plus one bin below it (the window's integer truncation), and
+    nowhere else."""
+    bw = u[1] - u[0]
+    P = np.zeros_like(u)
+    for p, wi, di in zip(np.atleast_2d(pts), np.atleast_1d(w), np.atleast_1d(dens)):
+        u0 = float(np.linalg.norm(p))
+        off = u - u0
+        inside = (off <= 4 * sigma) & (off > -4 * sigma - bw)
+        g = np.exp(-off ** 2 / (2 * sigma ** 2)) / (sigma * np.sqrt(2 * np.pi))
+        P += np.where(inside, wi * di * g, 0.0)
+    return P
+
+
+@pytest.mark.parametrize("u0", [2.3456, 0.1, 9.87, 4.0])
+def test_smearing_single_point_closed_form(u0):
+    """One point of weight w and density I at |p| = u0: P is w I N(u; u0,
+    0.15) on its window, zero elsewhere; I_avg = P / (4 pi u^2), 0 at u = 0;
+    the axis is u_j = j u_max / (n_bins - 1)."""
+    d = np.array([0.3, -0.48, 0.824])
+    p = d / np.linalg.norm(d) * u0
+    u, P, Ia = O.smearing(p[None, :], [0.7], [1.9], u_max=10.0, n_bins=1000, sigma=0.15)
+    assert np.array_equal(u, np.arange(1000) * (10.0 / 999))
+    ref = _smear_closed_form(u, p[None, :], [0.7], [1.9], 0.15)
+    assert np.count_nonzero(P) == np.count_nonzero(ref) > 50
+    assert np.allclose(P, ref, rtol=1e-13, atol=0.0)
+    Iref = np.where(u > 1e-6, ref / (4 * np.pi * np.where(u > 0, u, 1.0) ** 2), 0.0)
+    assert Ia[0] == 0.0
+    assert np.allclose(Ia, Iref, rtol=1e-13, atol=0.0)
+    # the peak sits on the bin nearest u0, at w I / (sigma sqrt(2 pi)) there
+    j = int(np.argmax(P))
+    assert abs(u[j] - u0) <= 0.5 * (u[1] - u[0]) + 1e-12
+
+
+def test_smearing_is_linear_in_the_points():
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-4, 4, size=(40, 3))
+    w = rng.uniform(0.1, 1.0, size=40)
+    dens = rng.uniform(-1, 1, size=40)
+    u, P, _ = O.smearing(pts, w, dens, u_max=9.0, n_bins=700, sigma=0.15)
+    ref = _smear_closed_form(u, pts, w, dens, 0.15)
+    assert np.allclose(P, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
