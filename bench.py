@@ -43,14 +43,28 @@ def flops_per_eval(hist):
     return float(np.sum(hist * (16 + 2 * L))) / max(1, int(hist.sum()))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
     """Oracle (FP64 CPU restatement, OpenMP over quartets, all host cores) on
-    a seeded random subset of the records over the full grid."""
+    a seeded random subset of the records over the full grid, built here
+    with -O3 -march=native -fopenmp for this host."""
+    import tempfile
     from oracle import oracle as O
     rng = np.random.default_rng(12345)
     cores = os.cpu_count() or 1
     prims = sysm.prims()
     n_total = len(rec_host)
+    tmp = tempfile.mkdtemp(prefix="orc_native_")
+    L = O.lib_native(os.path.join(tmp, "liborc_native.so"))
 
     def run(n):
         idx = np.sort(rng.choice(n_total, size=n, replace=False))
@@ -58,7 +72,7 @@ def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
         ijkl = np.ascontiguousarray(sub[:, :4])
         d = np.ascontiguousarray(sub[:, 4:6]).view(np.float64).reshape(-1)
         t0 = time.perf_counter()
-        _, cnt = O.intracule(prims, ijkl, d, grid_r, THRESH, nthreads=cores, dynamic=True)
+        _, cnt = O.intracule(prims, ijkl, d, grid_r, THRESH, nthreads=cores, dynamic=True, L=L)
         return time.perf_counter() - t0, cnt
 
     n = 4000
@@ -67,6 +81,7 @@ def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
     t, cnt = run(n2)
     evals = float(cnt[2]) * grid_r.shape[0]
     return dict(value=evals / t, unit="evals/s", cores=cores, kind="port",
+                cpu=cpu_model(), build="gcc -O3 -march=native -fopenmp (built on this host)",
                 sample="%d seeded random records of %d (%d pass both screens) x %d grid "
                        "points, FP64 oracle, OpenMP dynamic, %.1f s" % (
                            n2, n_total, int(cnt[2]), grid_r.shape[0], t))

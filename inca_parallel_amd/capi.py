@@ -29,7 +29,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
            "inca_get_eval_counts",
-           "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_batch_create", "inca_batch_destroy",
+           "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
@@ -93,6 +93,7 @@ def load():
     L.inca_evaluate_prepared_noalloc.argtypes = [vp, C.c_int32, vp, vp]
     L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
                                 vp, vp, vp]
+    L.inca_bragg_slater.argtypes = [C.c_int32, vp, vp]
     L.inca_batch_create.argtypes = [C.c_int32, C.c_int32, vp, C.POINTER(vp)]
     L.inca_batch_destroy.argtypes = [vp]
     L.inca_batch_last_error.argtypes = [vp]
@@ -167,6 +168,16 @@ def autogrid(an, cartes, nohydro=True, betaone=False, radius=None):
 MLP_IN, MLP_OUT, MLP_NPARAM = 16, 9, 9993
 NRAD_CLASSES = (50, 75, 100, 150)
 NANG_CLASSES = (110, 194, 302, 590)
+
+
+def bragg_slater(an):
+    """Bragg-Slater radii in bohr of the atomic numbers `an` (inca_bragg_slater)."""
+    L = load()
+    an = np.ascontiguousarray(an, np.int32)
+    r = np.zeros(max(len(an), 1))
+    if L.inca_bragg_slater(len(an), an.ctypes.data, r.ctypes.data) != 0:
+        raise IncaError("inca_bragg_slater failed")
+    return r[:len(an)]
 
 
 def _molecule_arrays(an, cartes, alpha, tmn, prim_atom):

@@ -417,6 +417,12 @@ int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes, int32
   return 0;
 }
 
+int inca_bragg_slater(int32_t n, const int32_t *an, double *radius) {
+  if (n < 0 || (n > 0 && (!an || !radius))) return INCA_EINVAL;
+  for (int32_t i = 0; i < n; ++i) radius[i] = bragg_slater_bohr(an[i]);
+  return 0;
+}
+
 int inca_set_mixed_tail(inca_ctx *ctx, double tail) {
   CHECK_CTX(ctx);
   if (!(tail >= 0.0 && tail <= 1.0)) return set_err(ctx, INCA_EINVAL, "tail must be in [0,1]");

@@ -39,11 +39,51 @@ for k in range(1, len(cases), 2):
     assert sum(size[o[0]] for o in orbits) == n, n
     rules.append((n, orbits))
 
+# The numbers come from SciPy's source, which carries this notice.
+SCIPY_NOTICE = """\
+The orbit parameters below are taken from SciPy (scipy/integrate/_lebedev.py),
+distributed under this licence:
+
+Copyright (c) 2001-2002 Enthought, Inc. 2003-2024, SciPy Developers.
+All rights reserved.
+
+Redistribution and use in source and binary forms, with or without
+modification, are permitted provided that the following conditions
+are met:
+
+1. Redistributions of source code must retain the above copyright
+   notice, this list of conditions and the following disclaimer.
+
+2. Redistributions in binary form must reproduce the above
+   copyright notice, this list of conditions and the following
+   disclaimer in the documentation and/or other materials provided
+   with the distribution.
+
+3. Neither the name of the copyright holder nor the names of its
+   contributors may be used to endorse or promote products derived
+   from this software without specific prior written permission.
+
+THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS
+"AS IS" AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT
+LIMITED TO, THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR
+A PARTICULAR PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT
+OWNER OR CONTRIBUTORS BE LIABLE FOR ANY DIRECT, INDIRECT, INCIDENTAL,
+SPECIAL, EXEMPLARY, OR CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT
+LIMITED TO, PROCUREMENT OF SUBSTITUTE GOODS OR SERVICES; LOSS OF USE,
+DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER CAUSED AND ON ANY
+THEORY OF LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT
+(INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY OUT OF THE USE
+OF THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
+"""
+
 out = sys.stdout
 out.write("/* Generated by gen_lebedev_table.py -- do not edit.\n"
           " * Lebedev-Laikov rules as Oh orbits {code, a, b, v}; weights v sum to 1.\n"
           " * code 1: (1,0,0)  2: (0,a,a)  3: (a,a,a)  4: (a,a,b) b=sqrt(1-2a^2)\n"
-          " * 5: (a,b,0) b=sqrt(1-a^2)  6: (a,b,c) c=sqrt(1-a^2-b^2) */\n")
+          " * 5: (a,b,0) b=sqrt(1-a^2)  6: (a,b,c) c=sqrt(1-a^2-b^2)\n *\n")
+for line in SCIPY_NOTICE.splitlines():
+    out.write((" * " + line).rstrip() + "\n")
+out.write(" */\n")
 out.write("#pragma once\n")
 out.write("struct LebOrbit { int code; double a, b, v; };\n")
 out.write("static const int LEB_NRULES = %d;\n" % len(rules))

@@ -161,6 +161,15 @@ module inca_hip
          type(c_ptr), value :: m
          character(kind=c_char), intent(in) :: path(*)
       end function
+      integer(c_int) function inca_autogrid(natoms, an, cartes, nohydro, betaone, radius, cap, &
+            nquad, cent, ps, sfalpha) bind(C, name="inca_autogrid")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         integer(c_int32_t), value :: natoms, nohydro, betaone, cap
+         integer(c_int32_t), intent(in) :: an(*)
+         real(c_double), intent(in) :: cartes(*)
+         type(c_ptr), value :: radius, cent, ps, sfalpha
+         integer(c_int32_t), intent(out) :: nquad
+      end function
       integer(c_int) function inca_learned_autogrid(m, natoms, an, cartes, nprim, alpha, tmn, &
             prim_atom, nohydro, cap, nquad, cent, ps, sfalpha, nradc, nangc, radius) &
             bind(C, name="inca_learned_autogrid")

@@ -45,13 +45,11 @@ PLAIN = (50, 590)          # reference defaults (mod_input_reader.f90:55-66)
 DENSE = (200, 974)         # judge grid
 FALLBACK_TOL = 1e-8
 THRESH = 1e-12
-BS_ANGSTROM = [0.35, 0.35, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50, 0.38, 1.80, 1.50, 1.25,
-               1.10, 1.00, 1.00, 1.00, 1.00, 2.20, 1.80, 1.60, 1.40, 1.35, 1.40, 1.40, 1.40,
-               1.35, 1.35, 1.35, 1.35, 1.30, 1.25, 1.15, 1.15, 1.15, 1.00]
 
 
 def bragg_slater(an):
-    return np.array([(BS_ANGSTROM[z - 1] if 1 <= z <= 36 else 1.0) * 1.8897 for z in an])
+    """The library's Bragg-Slater radii in bohr (get_bs_radius)."""
+    return capi.bragg_slater(an)
 
 
 def molecule(seed):

@@ -256,6 +256,61 @@ class System:
             f.write("%-40s   R     %22.15E\n" % ("Total Energy", -1.0))
 
 
+WFX_TYPES = {(0, 0, 0): 1, (1, 0, 0): 2, (0, 1, 0): 3, (0, 0, 1): 4, (2, 0, 0): 5,
+             (0, 2, 0): 6, (0, 0, 2): 7, (1, 1, 0): 8, (1, 0, 1): 9, (0, 1, 1): 10}
+
+
+def write_wfx(system, path, title="synthetic"):
+    """AIMAll .wfx with the sections the reference's reader uses
+    (src/mod_read_files.f90:12-287): primitives on their atoms, closed-shell
+    occupations, MO coefficients of the unnormalised primitives."""
+    s = system
+    n = s.nprim
+    nocc = s.nelec // 2
+    norm = np.array([normprim(a, t) for a, t in zip(s.alpha, s.tmn)])
+
+    def block(tag, lines):
+        return "<%s>\n%s\n</%s>\n" % (tag, "\n".join(lines), tag)
+
+    def rows(vals, fmt, per):
+        return [" ".join(fmt % v for v in vals[k:k + per]) for k in range(0, len(vals), per)]
+
+    out = [block("Title", [" " + title]), block("Keywords", [" GTO"]),
+           block("Number of Nuclei", [" %d" % len(s.an)]),
+           block("Number of Occupied Molecular Orbitals", [" %d" % nocc]),
+           block("Number of Perturbations", [" 0"]), block("Net Charge", [" 0"]),
+           block("Number of Electrons", [" %d" % s.nelec]),
+           block("Number of Alpha Electrons", [" %d" % nocc]),
+           block("Number of Beta Electrons", [" %d" % nocc]),
+           block("Electronic Spin Multiplicity", [" 1"]),
+           block("Nuclear Names", [" %s%d" % (e, i + 1) for i, e in enumerate(s.elements)]),
+           block("Atomic Numbers", [" %d" % z for z in s.an]),
+           block("Nuclear Charges", [" %.15E" % float(z) for z in s.an]),
+           block("Nuclear Cartesian Coordinates",
+                 [" %.15E %.15E %.15E" % tuple(c) for c in s.coords]),
+           block("Number of Primitives", [" %d" % n]),
+           block("Primitive Centers", rows(list(s.ra), " %d", 10)),
+           block("Primitive Types", rows([WFX_TYPES[tuple(t)] for t in s.tmn], " %d", 10)),
+           block("Primitive Exponents", rows(list(s.alpha), " %.15E", 4)),
+           block("Molecular Orbital Occupation Numbers", [" %.15E" % 2.0] * nocc),
+           block("Molecular Orbital Energies", [" %.15E" % (-1.0)] * nocc)]
+    mo = ["<Molecular Orbital Primitive Coefficients>"]
+    for k in range(nocc):
+        mo += ["<MO Number>", " %d" % (k + 1), "</MO Number>"]
+        mo += rows(list(s.C[:, k] * norm), " %.15E", 4)
+    mo.append("</Molecular Orbital Primitive Coefficients>")
+    out.append("\n".join(mo) + "\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def write_dm2p(path, ijkl, d):
+    """Flat .dm2p stream: 24-byte records (int32 i, j, k, l; float64 D)."""
+    rec = np.empty(len(d), dtype=[("ijkl", "<i4", 4), ("d", "<f8")])
+    rec["ijkl"], rec["d"] = ijkl, d
+    rec.tofile(path)
+
+
 def _folded(P, i, j, k, l):
     """Sum of D over the distinct images of (i,j,k,l) under i<->k, j<->l and
     the electron swap (ik)<->(jl)."""

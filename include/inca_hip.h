@@ -176,6 +176,11 @@ int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes,
                   int32_t cap, int32_t *nquad, double *cent, double *Ps,
                   double *sfalpha);
 
+/* The Bragg-Slater radius in bohr of each atomic number an[0..n): Slater's
+ * radii in angstrom times 1.8897, get_bs_radius (mod_autogrid.f90:142-185);
+ * the table inca_autogrid uses when radius is NULL.  Host code. */
+int inca_bragg_slater(int32_t n, const int32_t *an, double *radius);
+
 /* Batched evaluation of many molecules (no reference counterpart: the
  * reference runs one molecule per process, inca_main.f90:76).  Every
  * context must be on `device` and hold a grid and prepared quartets

@@ -58,6 +58,14 @@ def lib_fma():
     return _LIB_FMA
 
 
+def lib_native(path):
+    """The oracle built for the host it runs on (make native NATIVE_OUT=path),
+    for the CPU baseline's timing."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", "NATIVE_OUT=%s" % path],
+                   check=True)
+    return _configure(C.CDLL(path))
+
+
 def _configure(L):
     L.orc_dfact.argtypes = [C.c_int]
     L.orc_dfact.restype = C.c_int
