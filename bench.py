@@ -313,7 +313,16 @@ def cube_bench(capi, eng):
     grid loop over the prepared quartets (the per-axis contraction: every
     term at every voxel), with the mixed error per voxel."""
     n = eng.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
-    out = {"points": n, "method": "per-axis contraction (every quartet at every voxel, FP64)"}
+    out = {"points": n, "method": "per-axis contraction (every quartet at every voxel) on the "
+                                  "matrix cores, factor tiles staged in LDS: FP64; mixed: its FP32 "
+                                  "range on FP32 factors and FP32 matrix cores, folded into FP64"}
+    # the FP64 contraction on the vector units, for comparison
+    eng.prepare(capi.INCA_FP64, THRESH)
+    eng.set_cube_contraction(1)
+    eng.evaluate_prepared(capi.INCA_FP64, want_ivec=False)
+    tm = eng.timings()
+    out["fp64_vector_units_eval_ms"] = float(tm[1] + tm[2])
+    eng.set_cube_contraction(2)
     for prec, name in ((capi.INCA_FP64, "fp64"), (capi.INCA_MIXED, "mixed")):
         eng.prepare(prec, THRESH)
         t0 = time.perf_counter()

@@ -545,9 +545,9 @@ class Engine:
                  "inca_set_point_cutoff")
 
     def set_cube_contraction(self, on):
-        """1 (default): cube grids by the per-axis contraction (every quartet
-        at every voxel, FP64) on the vector units; 2: the same contraction on
-        the FP64 matrix cores; 0: term by term like other grids."""
+        """2 (default): cube grids by the per-axis contraction (every quartet
+        at every voxel) on the matrix cores; 1: the FP64 range on the vector
+        units; 0: term by term like other grids."""
         self._ck(self._L.inca_set_cube_contraction(self._h, int(on)),
                  "inca_set_cube_contraction")
 

@@ -103,7 +103,7 @@ struct inca_ctx {
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   int poly_exact = 0;         // 1: exact axis polynomials instead of the fits
   double cut_log2 = 100.0;    // per-point cutoff 2^-cut_log2 of a term's peak (0: off)
-  int cube_contract = 1;      // cube grids by the per-axis contraction (1: VALU, 2: MFMA, 0: off)
+  int cube_contract = 2;      // cube grids by the per-axis contraction (2: matrix cores, 1: vector units, 0: off)
   double *d_cf = nullptr;     // per-axis factors of a batch of quartets (cube)
   size_t cf_cap = 0;
   double *d_q = nullptr;      // compacted prepared records

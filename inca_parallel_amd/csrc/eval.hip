@@ -231,88 +231,181 @@ __global__ __launch_bounds__(kT) void k_cube_contract(const double *__restrict__
   }
 }
 
-// The same contraction on the FP64 matrix cores (mode 2; the vector kernel
-// above is the default: on the 80^3 benchmark cube this one takes 1.35-1.42 s
-// against 0.86 s, about 0.31 of the FP64 matrix peak whatever the number of
-// x tiles per wave and the occupancy).  Over the records of a
-// chunk it is the matrix product C[i][jk] = sum_q Fx[q][i] (Fy[q][j] Fz[q][k])
-// (jk = j nz + k, so C is the chunk's partial row in grid order).  A wave
-// computes MI x 2 tiles of 16 x values by 16 (j, k) pairs with
-// v_mfma_f64_16x16x4f64, four records per step; lane l supplies
-//   A: Fx[q0 + l/16][i0 + l%16]  (16 x 4),
-//   B: Fy[q][j] Fz[q][k] for q = q0 + l/16 and pair jk0 + l%16  (4 x 16),
-// and receives C[i0 + l/16 + 4r][jk0 + l%16] in register r (the lane layout
-// of the instruction on gfx950, checked against a host product).  Block x:
-// the group of 16 MI x values (ig = blockIdx.x mod nig) and 128 pairs (4
-// waves of 32); block y: the chunk.  The x factors are zero-padded to
-// nxp = 16 MI nig and records past the chunk supply A = 0, so no operand
-// needs a test.
-template <int MI>
-__global__ __launch_bounds__(kT) void k_cube_contract_mfma(const double *__restrict__ F,
-                                                           int64_t n, int64_t qch, int nx,
-                                                           int nxp, int ny, int nz, int nig,
-                                                           int accumulate,
-                                                           double *__restrict__ partial,
-                                                           int64_t G) {
+// Per-axis factors in FP32 for the FP32 range of a mixed evaluation: rp is
+// formed in FP64 (sqe c + offset cancels), then exp(-rp^2) by the hardware
+// exp2 with -log2(e) split in two floats, and the axis polynomial by FP32
+// Horner (the x factor carries A), as the mixed point kernel does.
+__global__ __launch_bounds__(kT) void k_cube_factors_f32(const double *__restrict__ q, int qstride,
+                                                         int64_t n, int nx, int nxp, int ny,
+                                                         int nz, const double *__restrict__ gx,
+                                                         const double *__restrict__ gy,
+                                                         const double *__restrict__ gz,
+                                                         float *__restrict__ F) {
+  const float kNL2eHi = -1.44269502162933349609375f;
+  const float kNL2eLo = -1.925963033500011e-08f;
   const int64_t nf = (int64_t)nxp + ny + nz;
-  const int64_t njk = (int64_t)ny * nz;
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ig = (int)(blockIdx.x % (unsigned)nig);
-  const int64_t jb = blockIdx.x / (unsigned)nig;
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= n * nf) return;
+  const int64_t r = t / nf;
+  const int a = (int)(t - r * nf);
+  if (a >= nx && a < nxp) {  // padding of the x factors
+    F[t] = 0.0f;
+    return;
+  }
+  const double *__restrict__ rec = q + r * qstride;
+  const QHeader h = load_header(rec);
+  const double *__restrict__ c = rec + kQHead;
+  int L;
+  double coord, off;
+  if (a < nx) {
+    L = h.Lx;
+    coord = gx[(int64_t)a * ny * nz];
+    off = h.ax;
+  } else if (a < nxp + ny) {
+    c += h.Lx + 1;
+    L = h.Ly;
+    coord = gy[(int64_t)(a - nxp) * nz];
+    off = h.ay;
+  } else {
+    c += h.Lx + h.Ly + 2;
+    L = h.Lz;
+    coord = gz[a - nxp - ny];
+    off = h.az;
+  }
+  const float rp = (float)fma(h.sqe, coord, off);
+  float poly = (float)c[0];
+  for (int j = 1; j <= L; ++j) poly = fmaf(poly, rp, (float)c[j]);
+  const float s = rp * rp;
+  F[t] = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo)) * poly;
+}
+
+// The contraction on the matrix cores, factor tiles staged in LDS.  Over the
+// records of a chunk it is the matrix product
+//   C[i][(j,k)] = sum_q Fx[q][i] (Fy[q][j] Fz[q][k]).
+// A block takes 16 x values (i tile) and a 16 x 16 (j, k) tile; wave w holds
+// the four 16 x 16 output tiles of j = j0 + 4w + c (c = 0..3), k = k0..k0+15.
+// Per step of four records lane l supplies A = Fx[q + l/16][i0 + l%16] and,
+// per tile c, B = Fy[q + l/16][j0 + 4w + c] Fz[q + l/16][k0 + l%16]; the 16
+// records of a stage are loaded into LDS by the whole block (zeros past the
+// chunk and outside the cube, and the x factors are zero-padded to whole i
+// tiles, so no operand needs a test).
+//   T = double: v_mfma_f64_16x16x4f64, FP64 accumulators (mode 2);
+//   T = float:  v_mfma_f32_16x16x4f32 on FP32 factors (the FP32 range of a
+//               mixed evaluation), FP32 sums of 64 records at a time folded
+//               into FP64 accumulators.
+// Output register r of lane l holds row Mma<T>::row(l, r), column l%16 (the
+// instructions' lane layouts on gfx950).
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  static constexpr int kFold = 0;  // accumulate in FP64 throughout
+};
+template <>
+struct Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  static constexpr int kFold = 2;  // stages of 32 records per FP32 run
+};
+
+constexpr int kStageQ = 32;  // records per LDS stage
+
+template <typename T>
+__global__ __launch_bounds__(kT) void k_cube_contract_mma(const T *__restrict__ F, int64_t n,
+                                                          int64_t qch, int nx, int nxp, int ny,
+                                                          int nz, int accumulate,
+                                                          double *__restrict__ partial,
+                                                          int64_t G) {
+  typedef typename Mma<T>::v4 v4;
+  __shared__ T sx[kStageQ][16], sy[kStageQ][16], sz[kStageQ][16];
+  const int64_t nf = (int64_t)nxp + ny + nz;
+  const int nit = nxp / 16, njt = (ny + 15) / 16;
+  const int it = (int)(blockIdx.x % (unsigned)nit);
+  const int jkt = (int)(blockIdx.x / (unsigned)nit);
+  const int i0 = it * 16, j0 = (jkt % njt) * 16, k0 = (jkt / njt) * 16;
   const int64_t chunk = blockIdx.y;
   const int64_t q0 = chunk * qch;
   const int64_t q1 = (q0 + qch < n) ? q0 + qch : n;
-  const int i0 = ig * 16 * MI;
-  const int64_t jk0 = jb * 128 + (int64_t)w * 32;
-  const int kq = lane >> 4;             // record of the step
-  const int ia = i0 + (lane & 15);      // x value of A in tile 0
-  int64_t jy[2], kz[2];
-  bool jok[2];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int kr = lane >> 4, col = lane & 15;
+  v4 acc[4];
+  double acc64[4][4];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int64_t jk = jk0 + t * 16 + (lane & 15);
-    jok[t] = jk < njk;
-    const int64_t jc = jok[t] ? jk : 0;
-    jy[t] = nxp + jc / nz;
-    kz[t] = (int64_t)nxp + ny + jc % nz;
+  for (int c = 0; c < 4; ++c) {
+    acc[c] = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[c][r] = 0.0;
   }
-  typedef double d4 __attribute__((ext_vector_type(4)));
-  d4 acc[MI][2];
+  // loader: thread t stages column t%16 of records t/16 and t/16 + 16 of
+  // each 16-wide tile; the next stage is fetched into registers while the
+  // current one is consumed
+  const int lq = threadIdx.x >> 4, lc = threadIdx.x & 15;
+  const bool jok = j0 + lc < ny, kok = k0 + lc < nz;
+  T pre[2][3];
+  auto fetch = [&](int64_t qb) {
 #pragma unroll
-  for (int m = 0; m < MI; ++m)
+    for (int h = 0; h < 2; ++h) {
+      const int64_t qq = qb + lq + 16 * h;
+      const bool ok = qq < q1;
+      const T *__restrict__ row = F + (ok ? qq : q0) * nf;
+      pre[h][0] = ok ? row[i0 + lc] : T(0);
+      pre[h][1] = (ok && jok) ? row[nxp + j0 + lc] : T(0);
+      pre[h][2] = (ok && kok) ? row[nxp + ny + k0 + lc] : T(0);
+    }
+  };
+  if (q0 < q1) fetch(q0);
+  int stage = 0;
+  for (int64_t qb = q0; qb < q1; qb += kStageQ, ++stage) {
+    __syncthreads();  // the previous stage has been consumed
 #pragma unroll
-    for (int t = 0; t < 2; ++t) acc[m][t] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int64_t q = q0; q < q1; q += 4) {
-    const int64_t qq = q + kq;
-    const bool ok = qq < q1;
-    const double *__restrict__ row = F + (ok ? qq : q) * nf;
-    double a[MI], b[2];
+    for (int h = 0; h < 2; ++h) {
+      sx[lq + 16 * h][lc] = pre[h][0];
+      sy[lq + 16 * h][lc] = pre[h][1];
+      sz[lq + 16 * h][lc] = pre[h][2];
+    }
+    __syncthreads();
+    if (qb + kStageQ < q1) fetch(qb + kStageQ);
 #pragma unroll
-    for (int m = 0; m < MI; ++m) a[m] = ok ? row[ia + 16 * m] : 0.0;
+    for (int s = 0; s < kStageQ / 4; ++s) {
+      const int rq = 4 * s + kr;
+      const T a = sx[rq][col];
+      const T fz = sz[rq][col];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) b[t] = jok[t] ? row[jy[t]] * row[kz[t]] : 0.0;
+      for (int c = 0; c < 4; ++c) acc[c] = Mma<T>::step(a, sy[rq][4 * w + c] * fz, acc[c]);
+    }
+    if (Mma<T>::kFold && (stage + 1) % (Mma<T>::kFold > 0 ? Mma<T>::kFold : 1) == 0) {
 #pragma unroll
-    for (int m = 0; m < MI; ++m)
+      for (int c = 0; c < 4; ++c) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
-        acc[m][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[t], acc[m][t], 0, 0, 0);
-  }
-  double *__restrict__ out = partial + chunk * G;
-#pragma unroll
-  for (int m = 0; m < MI; ++m)
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int64_t jk = jk0 + t * 16 + (lane & 15);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = i0 + 16 * m + kq + 4 * r;
-        if (i < nx && jk < njk) {
-          const int64_t g = (int64_t)i * njk + jk;
-          out[g] = accumulate ? out[g] + acc[m][t][r] : acc[m][t][r];
-        }
+        for (int r = 0; r < 4; ++r) acc64[c][r] += (double)acc[c][r];
+        acc[c] = v4{0, 0, 0, 0};
       }
     }
+  }
+  double *__restrict__ out = partial + chunk * G;
+  const int64_t njk = (int64_t)ny * nz;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int j = j0 + 4 * w + c, k = k0 + col;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + Mma<T>::row(lane, r);
+      const double v = Mma<T>::kFold ? acc64[c][r] + (double)acc[c][r] : (double)acc[c][r];
+      if (i < nx && j < ny && k < nz) {
+        const int64_t g = (int64_t)i * njk + (int64_t)j * nz + k;
+        out[g] = accumulate ? out[g] + v : v;
+      }
+    }
+  }
 }
 
 // rho = exp(-2 sqe^2 h^2) of every prepared record into word 7 (ray tiles).
@@ -771,36 +864,30 @@ void chunking(int64_t n, int64_t tiles, double share, int64_t *qch, int64_t *nch
 }  // namespace
 
 // The cube contraction over every prepared quartet: factors for batches of
-// quartets (at most 2^28 doubles at a time), each batch contracted into the
-// same `*nrows` partial rows, one per chunk, added in batch order.
-int cube_contract(inca_ctx *ctx, int64_t *nrows) {
+// quartets (at most 2^28 doubles of them at a time), each batch contracted
+// into the same `*nrows` partial rows, one per chunk, added in batch order.
+// The FP64 range (all quartets in FP64; the leading q_fp64 of a mixed
+// evaluation) runs on the vector kernel (mode 1) or the FP64 matrix cores
+// (mode 2); the FP32 range of a mixed evaluation runs on FP32 factors and
+// the FP32 matrix cores, folded into FP64.
+int cube_contract(inca_ctx *ctx, int precision, int64_t *nrows) {
   hipStream_t st = ctx->stream;
   const int nx = ctx->cube_n[0], ny = ctx->cube_n[1], nz = ctx->cube_n[2];
   const int64_t G = ctx->G, nq = ctx->q2;
   const int64_t njk = (int64_t)ny * nz;
   if ((int64_t)nx * njk != G) return set_err(ctx, INCA_EINVAL, "cube grid of unexpected size");
-  const bool mfma = ctx->cube_contract == 2;
+  const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
+  const int64_t n32 = nq - n64;
+  const bool mma64 = ctx->cube_contract == 2;
   int ncu = 0, per_cu = 0;
   INCA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
   hipError_t oe = hipSuccess;
-  int IB = 8, MI = 1, nig = 1, nxp = 0;
-  int64_t ni = 1, per_chunk = 1;
-  if (mfma) {
-    // matrix cores (mode 2): x tiles of 16 in groups of at most 5 per wave, 128
-    // (j, k) pairs per block
-    const int ntl = (nx + 15) / 16;
-    nig = (ntl + 4) / 5;
-    MI = (ntl + nig - 1) / nig;
-    nxp = 16 * MI * nig;
-    per_chunk = (int64_t)nig * ((njk + 127) / 128);
-    switch (MI) {
-      case 1: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<1>, kT, 0); break;
-      case 2: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<2>, kT, 0); break;
-      case 3: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<3>, kT, 0); break;
-      case 4: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<4>, kT, 0); break;
-      default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mfma<5>, kT, 0); break;
-    }
-  } else {
+  // matrix-core tiles: 16 x values by a 16 x 16 (j, k) tile per block
+  const int nx16 = (nx + 15) / 16 * 16;
+  const int64_t per_mma = (int64_t)(nx16 / 16) * ((ny + 15) / 16) * ((nz + 15) / 16);
+  int IB = 8, nxp = nx16;
+  int64_t ni = 1, per_chunk = per_mma;
+  if (n64 > 0 && !mma64) {
     // vector units: x values per lane, the choice that wastes the fewest,
     // the larger on ties
     const int ibs[5] = {8, 16, 24, 32, 40};
@@ -823,47 +910,48 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
       case 32: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<32>, kT, 0); break;
       default: oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract<40>, kT, 0); break;
     }
+  } else if (n64 > 0) {
+    oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mma<double>, kT, 0);
+  } else {
+    oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cube_contract_mma<float>, kT, 0);
   }
   if (oe != hipSuccess) return hip_fail(ctx, oe, "occupancy query", __LINE__);
-  const int64_t nfp = (int64_t)nxp + ny + nz;
+  const int64_t nfp = (int64_t)nxp + ny + nz;     // FP64 factors per quartet
+  const int64_t nfp32 = (int64_t)nx16 + ny + nz;  // FP32 factors per quartet
   // chunks: the blocks of a batch fill two rounds of the device (every block
   // does the same work, so a partial last round would idle most CUs)
   const int64_t resident = (int64_t)std::max(1, ncu) * std::max(1, per_cu);
   // quartets per factor batch: within 2^28 doubles (2 GB; 2^26 ran as
   // fast, 2^30 slower as each chunk's rows left the caches), and at most half
   // of them beyond 8192 (no batch need hold every quartet)
-  const int64_t nb = std::max<int64_t>(
-      1, std::min<int64_t>({nq, ((int64_t)1 << 28) / nfp,
-                            std::max<int64_t>(4096, (nq + 1) / 2)}));
+  auto batch = [&](int64_t n, int64_t per) {
+    return std::max<int64_t>(1, std::min<int64_t>({n, ((int64_t)1 << 28) / per,
+                                                   std::max<int64_t>(4096, (n + 1) / 2)}));
+  };
+  const int64_t nb = n64 > 0 ? batch(n64, nfp) : 0;
+  const int64_t nb32 = n32 > 0 ? batch(n32, (nfp32 + 1) / 2) : 0;  // two floats per double
   int64_t cpb = std::max<int64_t>(1, 2 * resident / per_chunk);
-  cpb = std::min<int64_t>(cpb, std::max<int64_t>(1, (nb + 63) / 64));
+  cpb = std::min<int64_t>(cpb, std::max<int64_t>(1, (std::max(nb, nb32) + 63) / 64));
   if (cpb > 65535) cpb = 65535;
   int rc;
-  if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)(nb * nfp)))) return rc;
+  const size_t cf_doubles = std::max<size_t>((size_t)(nb * nfp), (size_t)((nb32 * nfp32 + 1) / 2));
+  if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, cf_doubles))) return rc;
   if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(cpb * G)))) return rc;
-  for (int64_t qb = 0; qb < nq; qb += nb) {
-    const int64_t n = std::min(nb, nq - qb);
+  for (int64_t qb = 0; qb < n64; qb += nb) {
+    const int64_t n = std::min(nb, n64 - qb);
     const int64_t nt = n * nfp;
     hipLaunchKernelGGL(k_cube_factors, dim3((unsigned)((nt + kT - 1) / kT)), dim3(kT), 0, st,
                        ctx->d_q + qb * ctx->qstride, ctx->qstride, n, nx, nxp, ny, nz, ctx->d_gx,
                        ctx->d_gy, ctx->d_gz, ctx->d_cf);
     // every batch fills all cpb rows (a chunk may be empty: it writes zeros)
     const int64_t qch = (n + cpb - 1) / cpb;
-    const dim3 grid((unsigned)per_chunk, (unsigned)cpb);
     const int accumulate = qb > 0;
-    if (mfma) {
-#define INCA_CUBEM(MIV)                                                                      \
-  hipLaunchKernelGGL(k_cube_contract_mfma<MIV>, grid, dim3(kT), 0, st, ctx->d_cf, n, qch, nx, \
-                     nxp, ny, nz, nig, accumulate, ctx->d_partial, G)
-      switch (MI) {
-        case 1: INCA_CUBEM(1); break;
-        case 2: INCA_CUBEM(2); break;
-        case 3: INCA_CUBEM(3); break;
-        case 4: INCA_CUBEM(4); break;
-        default: INCA_CUBEM(5); break;
-      }
-#undef INCA_CUBEM
+    if (mma64) {
+      hipLaunchKernelGGL(k_cube_contract_mma<double>, dim3((unsigned)per_mma, (unsigned)cpb),
+                         dim3(kT), 0, st, ctx->d_cf, n, qch, nx, nxp, ny, nz, accumulate,
+                         ctx->d_partial, G);
     } else {
+      const dim3 grid((unsigned)per_chunk, (unsigned)cpb);
 #define INCA_CUBE(IBV)                                                                         \
   hipLaunchKernelGGL(k_cube_contract<IBV>, grid, dim3(kT), 0, st, ctx->d_cf, n, qch, nx, ny, nz, \
                      ni, accumulate, ctx->d_partial, G)
@@ -876,6 +964,20 @@ int cube_contract(inca_ctx *ctx, int64_t *nrows) {
       }
 #undef INCA_CUBE
     }
+    INCA_HIP(hipGetLastError());
+  }
+  float *cf32 = reinterpret_cast<float *>(ctx->d_cf);
+  for (int64_t qb = 0; qb < n32; qb += nb32) {
+    const int64_t n = std::min(nb32, n32 - qb);
+    const int64_t nt = n * nfp32;
+    hipLaunchKernelGGL(k_cube_factors_f32, dim3((unsigned)((nt + kT - 1) / kT)), dim3(kT), 0, st,
+                       ctx->d_q + (n64 + qb) * ctx->qstride, ctx->qstride, n, nx, nx16, ny, nz,
+                       ctx->d_gx, ctx->d_gy, ctx->d_gz, cf32);
+    const int64_t qch = (n + cpb - 1) / cpb;
+    const int accumulate = n64 > 0 || qb > 0;
+    hipLaunchKernelGGL(k_cube_contract_mma<float>, dim3((unsigned)per_mma, (unsigned)cpb),
+                       dim3(kT), 0, st, (const float *)cf32, n, qch, nx, nx16, ny, nz, accumulate,
+                       ctx->d_partial, G);
     INCA_HIP(hipGetLastError());
   }
   *nrows = cpb;
@@ -905,7 +1007,7 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   int64_t nchunks = 0;
   bool rays_used = false;  // a range ran on the ray lists (duplicate points skipped)
   if (contracted) {
-    if ((rc = cube_contract(ctx, &nchunks))) return rc;
+    if ((rc = cube_contract(ctx, precision, &nchunks))) return rc;
   } else {
     // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
     const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;

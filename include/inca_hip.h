@@ -295,21 +295,23 @@ int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32,
  * counterpart. */
 int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
 
-/* Cube grids by the per-axis contraction (on = 1, the default).  A cube
- * (mod_build_grid.f90:412-440) is a tensor product of nx x-, ny y- and nz
- * z-coordinates, and exp(-|rp|^2) is the product of its per-axis factors,
- * so every term is A Fx(x) Fy(y) Fz(z) with F(c) = exp(-rp_c^2) P_c(rp_c).
- * The factors are formed once per quartet and coordinate (nx + ny + nz of
- * them) and Ivec(i,j,k) = sum_q Fx_q(i) Fy_q(j) Fz_q(k) is contracted with
- * one FMA per quartet and voxel: every quartet at every voxel, in FP64 (a
- * mixed evaluation too), with no cutoff.  For each chunk of quartets the
+/* Cube grids by the per-axis contraction.  A cube (mod_build_grid.f90:
+ * 412-440) is a tensor product of nx x-, ny y- and nz z-coordinates, and
+ * exp(-|rp|^2) is the product of its per-axis factors, so every term is
+ * A Fx(x) Fy(y) Fz(z) with F(c) = exp(-rp_c^2) P_c(rp_c).  The factors are
+ * formed once per quartet and coordinate (nx + ny + nz of them) and
+ * Ivec(i,j,k) = sum_q Fx_q(i) Fy_q(j) Fz_q(k) is contracted: every quartet
+ * at every voxel, with no cutoff.  For each chunk of quartets the
  * contraction is the matrix product of the x factors with the row-wise
- * Kronecker product of the y and z factors; on = 1 runs it on the vector
- * units, on = 2 on the FP64 matrix cores (v_mfma_f64_16x16x4f64: the same
- * chain of fused multiply-adds, bitwise equal when the chunking matches;
- * 1.6x slower on the 80^3 benchmark cube).  0 evaluates cubes term by term
- * like other grids.  Takes effect at the next evaluation.  No reference
- * counterpart: the reference evaluates each voxel's terms directly
+ * Kronecker product of the y and z factors.  on = 2 (the default) runs it on
+ * the matrix cores with factor tiles staged in LDS (v_mfma_f64_16x16x4f64;
+ * 0.67 s on the 80^3 benchmark cube), on = 1 on the vector units (0.86 s);
+ * both agree with the term-by-term FP64 evaluation to 1e-11 relative.  A
+ * mixed evaluation contracts its FP32 range on FP32 factors and the FP32
+ * matrix cores (v_mfma_f32_16x16x4f32), sums of 64 records folded into FP64,
+ * whatever `on` is (1 or 2).  0 evaluates cubes term by term like other
+ * grids.  Takes effect at the next evaluation.  No reference counterpart:
+ * the reference evaluates each voxel's terms directly
  * (mod_intracule.f90:345-358). */
 int inca_set_cube_contraction(inca_ctx *ctx, int32_t on);
 /* Evaluations of the last grid loop: reference = quartets x points (the

@@ -150,9 +150,29 @@ def test_cube_contraction_matches_ray_tiles(full):
             assert ex == ref_n == int(cnt[2]) * n
             got[mode] = iv.copy()
     finally:
-        e.set_cube_contraction(1)
+        e.set_cube_contraction(2)
     assert list(cnt) == list(full["cnt"])
     for mode, iv in got.items():
         er, ea = _close(iv, ray)
         print("80^3 cube, contraction mode %d vs ray tiles: rel %.2e abs %.2e" % (mode, er, ea))
         assert er < 1e-10 and ea < 1e-13
+    full["cube_fp64"] = got[1]
+
+
+def test_cube_mixed_per_voxel(full):
+    """The 80^3 cube evaluated mixed: the FP32 range on FP32 factors and the
+    FP32 matrix cores, folded into FP64, against the FP64 contraction at
+    every voxel (runs after the cube test above, on its grid)."""
+    e = full["e"]
+    if "cube_fp64" not in full:
+        pytest.skip("needs the FP64 cube of test_cube_contraction_matches_ray_tiles")
+    ref = full["cube_fp64"]
+    mixed, cnt = e.evaluate(capi.INCA_MIXED, THRESH)
+    n64, n32, _ = e.mixed_split()
+    er, ea = _close(mixed, ref)
+    es = abs(mixed.sum() - ref.sum()) / abs(ref.sum())
+    print("80^3 cube mixed (%d FP64 + %d FP32 quartets) vs FP64: rel %.2e abs %.2e sum %.2e"
+          % (n64, n32, er, ea, es))
+    assert n32 > n64 > 0
+    assert ea < 1e-9 and es < 1e-9
+    assert not np.array_equal(mixed, ref)

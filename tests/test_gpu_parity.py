@@ -383,7 +383,7 @@ def test_cube_contraction_mode_range(engine):
             with pytest.raises(capi.IncaError, match="0, 1 or 2"):
                 engine.set_cube_contraction(bad)
     finally:
-        engine.set_cube_contraction(1)
+        engine.set_cube_contraction(2)
 
 
 def test_autotune_before_any_grid_loop():
@@ -435,7 +435,7 @@ def test_ray_tiles_on_cube_lines(engine):
     finally:
         engine.set_tiles(4, 4)
         engine.set_point_cutoff(100)
-        engine.set_cube_contraction(1)
+        engine.set_cube_contraction(2)
     for cut in (100, 0):
         for P in (4, 6, 8, 10):
             er, ea = _close(out[(cut, P)], out[(cut, 1)])
@@ -446,16 +446,20 @@ def test_ray_tiles_on_cube_lines(engine):
             assert er < 1e-11 and ea < 1e-13
 
 
-@pytest.mark.parametrize("dims", [(9, 7, 20), (41, 5, 6), (97, 3, 4)])
+@pytest.mark.parametrize("dims", [(9, 7, 20), (41, 5, 6), (97, 3, 4), (25, 6, 7)])
 def test_cube_contraction_matches_term_by_term(engine, dims):
     """Cube grids by the per-axis contraction (every quartet at every voxel,
     factors exp(-rp_c^2) P_c(rp_c) per axis, batches of quartets added into
-    the same rows), on the vector units (mode 1) and on the matrix cores
+    the same rows), on the vector units (mode 1) and on the FP64 matrix cores
     (mode 2), against the term-by-term point kernel with the cutoff off (the
-    reference's loop).  (9, 7, 20): one x tile, padded; (41, 5, 6): three x
-    tiles, fewer (j, k) pairs than one block; (97, 3, 4): two groups of x
-    tiles.  More than 8192 prepared quartets, so the factors come in two or
-    more batches."""
+    reference's loop), FP64 bar.  A mixed evaluation contracts its FP32 range
+    on FP32 factors and the FP32 matrix cores (sums of 64 records folded into
+    FP64): against the term-by-term FP64 result within 1e-9 of max|I| per
+    voxel and 1e-9 relative on the sum, and not equal to the FP64
+    contraction.  (9, 7, 20): one x tile, padded; (41, 5, 6): three x tiles,
+    fewer (j, k) pairs than one block; (97, 3, 4): seven x tiles; (25, 6, 7):
+    two x tiles.  More than 8192 prepared quartets, so the factors come in
+    two or more batches."""
     s = synth.config_cluster(seed=35, natoms=3, nheavy=2, nocc=3, exp_lo=0.05, exp_hi=2e3,
                              shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
     ijkl, d = s.dm2_unique()
@@ -478,22 +482,22 @@ def test_cube_contraction_matches_term_by_term(engine, dims):
             ref_n, ex = engine.eval_counts()
             assert ex == ref_n == int(cnt[2]) * int(np.prod(dims))
             mixed, _ = engine.evaluate(capi.INCA_MIXED, THRESH)
+            n64, n32, _ = engine.mixed_split()
+            assert n64 > 0 and n32 > 0
             got[mode] = (iv.copy(), mixed.copy())
     finally:
-        engine.set_cube_contraction(1)
+        engine.set_cube_contraction(2)
         engine.set_point_cutoff(100)
         engine.set_tiles(4, 4)
     for mode, (iv, mixed) in got.items():
         er, ea = _close(iv, ref)
         em, eam = _close(mixed, ref)
+        es = abs(mixed.sum() - ref.sum()) / abs(ref.sum())
         print("cube %s contraction mode %d vs term by term: rel %.2e abs %.2e; mixed rel %.2e "
-              "abs %.2e" % (dims, mode, er, ea, em, eam))
+              "abs %.2e sum %.2e" % (dims, mode, er, ea, em, eam, es))
         assert er < 1e-11 and ea < 1e-13
-        assert em < 1e-11 and eam < 1e-13
-    # the matrix cores add the four products of a step as successive fused
-    # multiply-adds, the same chain as the vector kernel's: with equal
-    # chunking the two modes agree bitwise (measured: equal here)
-    print("modes 1 and 2 bitwise equal: %s" % np.array_equal(got[1][0], got[2][0]))
+        assert eam < 1e-9 and es < 1e-9
+        assert not np.array_equal(mixed, iv)  # the FP32 range ran in FP32
 
 
 def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
