@@ -30,12 +30,12 @@ HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 FC = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
 
 HIP_SOURCES = ["util.hip", "grid.hip", "quartet.hip", "eval.hip", "autogrid.hip", "batch.hip",
-               "mlp.hip",
+               "mlp.hip", "onee.hip",
                "api.hip"]
 # the Fortran modules in dependency order
 F_SOURCES = ["mod_inca_hip.f90", "mod_state.f90", "mod_locate.f90",
              "mod_readers.f90", "mod_autogrid.f90", "mod_input.f90",
-             "mod_intracule.f90", "inca_main.f90"]
+             "mod_intracule.f90", "mod_cubefile.f90", "inca_main.f90"]
 
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

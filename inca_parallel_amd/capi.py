@@ -29,7 +29,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
            "inca_get_eval_counts",
-           "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_batch_create", "inca_batch_destroy",
+           "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_onee", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
@@ -94,6 +94,7 @@ def load():
     L.inca_autogrid.argtypes = [C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, vp,
                                 vp, vp, vp]
     L.inca_bragg_slater.argtypes = [C.c_int32, vp, vp]
+    L.inca_onee.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
     L.inca_batch_create.argtypes = [C.c_int32, C.c_int32, vp, C.POINTER(vp)]
     L.inca_batch_destroy.argtypes = [vp]
     L.inca_batch_last_error.argtypes = [vp]
@@ -491,6 +492,18 @@ class Engine:
         self._ck(self._L.inca_evaluate_prepared_noalloc(
             self._h, int(precision), C.c_void_p(int(ivec_device_ptr)),
             C.c_void_p(int(stream)) if stream else None), "inca_evaluate_prepared_noalloc")
+
+    def onee(self, kind, index=0, T=None, w=None):
+        """One-electron function on the current grid (inca_onee): kind 1
+        primitive `index`, 3 orbital `index`, 5 density, 6 the reference's
+        Laplacian; T (nmo, nprim), w (nmo)."""
+        out = np.zeros(max(self.G, 1))
+        Tf = None if T is None else np.asfortranarray(T, dtype=np.float64)
+        wv = None if w is None else np.ascontiguousarray(w, np.float64)
+        nmo = 0 if Tf is None else Tf.shape[0]
+        self._ck(self._L.inca_onee(self._h, int(kind), int(index), nmo, _ptr(Tf), _ptr(wv),
+                                   out.ctypes.data), "inca_onee")
+        return out[:self.G]
 
     def ivec(self):
         iv = np.zeros(max(self.G, 1))

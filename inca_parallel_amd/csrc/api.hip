@@ -417,6 +417,29 @@ int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes, int32
   return 0;
 }
 
+int inca_onee(inca_ctx *ctx, int32_t kind, int32_t index, int32_t nmo, const double *T,
+              const double *w, double *values) {
+  CHECK_CTX(ctx);
+  if (ctx->nprim < 1) return set_err(ctx, INCA_EINVAL, "primitives not set");
+  if (ctx->grid_kind == GRID_NONE) return set_err(ctx, INCA_EINVAL, "grid not set");
+  if (!values && ctx->G > 0) return set_err(ctx, INCA_EINVAL, "null output");
+  if (kind == 1) {
+    if (index < 1 || index > ctx->nprim)
+      return set_err(ctx, INCA_EINVAL, "primitive %d outside 1..%d", index, ctx->nprim);
+  } else if (kind == 3 || kind == 5 || kind == 6) {
+    if (nmo < 1 || !T) return set_err(ctx, INCA_EINVAL, "orbital coefficients missing");
+    if (kind == 3 && (index < 1 || index > nmo))
+      return set_err(ctx, INCA_EINVAL, "orbital %d outside 1..%d", index, nmo);
+    if (kind != 3 && !w) return set_err(ctx, INCA_EINVAL, "orbital weights missing");
+  } else {
+    return set_err(ctx, INCA_EINVAL,
+                   "one-electron function kind %d: 1 primitive, 3 orbital, 5 density, "
+                   "6 Laplacian", kind);
+  }
+  INCA_HIP(hipSetDevice(ctx->device));
+  return onee(ctx, kind, index, nmo, T, w, values);
+}
+
 int inca_bragg_slater(int32_t n, const int32_t *an, double *radius) {
   if (n < 0 || (n > 0 && (!an || !radius))) return INCA_EINVAL;
   for (int32_t i = 0; i < n; ++i) radius[i] = bragg_slater_bohr(an[i]);

@@ -236,6 +236,9 @@ void atom_features(int natoms, const int32_t *an, const double *cartes, int npri
 int evaluate_grid(inca_ctx *ctx, int precision);
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist);
 int autotune(inca_ctx *ctx, int32_t *best);
+// one-electron functions on the current grid (onee.hip)
+int onee(inca_ctx *ctx, int kind, int index, int nmo, const double *T, const double *w,
+         double *values);
 // no-allocation evaluation (eval.hip): per-device scratch set-up, called by
 // inca_create with the device current, and the grid loop on that scratch
 hipError_t noalloc_scratch_init(int device);

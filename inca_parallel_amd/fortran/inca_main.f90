@@ -7,6 +7,7 @@ program inca
    use input_reader
    use read_files
    use intracule
+   use cubefile, only: onee_cubes
    use inca_hip, only: inca_close
    implicit none
    character(len=256) :: input_file_name
@@ -27,7 +28,7 @@ program inca
    if (readfchk) call filefchk(fchkfilename)
    if (readbas) call filebas(basfilename)
    if (readlog) write(*,*) "NOTE: .log input is not used by this build (intracules only)."
-   if (cube) write(*,*) "NOTE: one-electron cube files (&cube_job) are not part of this build."
+   if (cube) call onee_cubes()
 
    if (intracalc) then
       if (.not. readbas) then

@@ -161,6 +161,12 @@ module inca_hip
          type(c_ptr), value :: m
          character(kind=c_char), intent(in) :: path(*)
       end function
+      integer(c_int) function inca_onee(ctx, kind, idx, nmo, t, w, values) &
+            bind(C, name="inca_onee")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx, t, w, values
+         integer(c_int32_t), value :: kind, idx, nmo
+      end function
       integer(c_int) function inca_autogrid(natoms, an, cartes, nohydro, betaone, radius, cap, &
             nquad, cent, ps, sfalpha) bind(C, name="inca_autogrid")
          import :: c_int, c_ptr, c_int32_t, c_double

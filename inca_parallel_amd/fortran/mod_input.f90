@@ -26,11 +26,12 @@ module input_reader
    character(len=16) :: engine_precision = 'fp64'
    character(len=16) :: engine_polynomials = 'reference'
 
-   ! &cube_job fields (one-electron cubes are not part of this build; the
-   ! block is parsed so existing inputs keep working)
+   ! &cube_job fields (one-electron cubes, mod_cubefile.f90 of this host)
    integer :: mo = 0, npr = 0, cao = 0
    double precision :: center(3) = 0.d0, step(3) = 0.d0
    integer :: np(3) = 0
+   ! the input file name without its extension (names of the one-electron cubes)
+   character(len=256) :: job_stem = ''
 
 contains
 
@@ -137,6 +138,7 @@ contains
       if (len_trim(outname) == 0) outname = stem(filename, dot_idx)//".out"
       if (len_trim(r_plot_name) == 0) r_plot_name = stem(filename, dot_idx)//".rad"
       if (len_trim(cubeintraname) == 0) cubeintraname = stem(filename, dot_idx)//"_vector.cube"
+      job_stem = stem(filename, dot_idx)
 
       if (radial_plot) then
          if (nblock < 1 .or. nblock > 20) then

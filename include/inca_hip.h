@@ -176,6 +176,24 @@ int inca_autogrid(int32_t natoms, const int32_t *an, const double *cartes,
                   int32_t cap, int32_t *nquad, double *cent, double *Ps,
                   double *sfalpha);
 
+/* One-electron functions on the current grid: the cube files of &cube_job
+ * (cubegen/fwrite, mod_cubefile.f90:16-128) with the functions of
+ * mod_functions.f90, on the primitives of inca_set_primitives (centres x, y,
+ * z; the normalisation is not applied, as in Prim):
+ *   kind 1: primitive `index` (1-based), Prim (:17-31);
+ *   kind 3: orbital `index`, MoOr (:199-210): sum_p T(index,p) Prim_p over
+ *           |T| > 1e-10;
+ *   kind 5: density sum_i w_i MoOr_i^2, Density / Dens_a / Dens_b
+ *           (:312-352; w_i = 2 for the closed shell, 1 per spin);
+ *   kind 6: the reference's Laplacian sum_i 2 w_i MoOr_i sum_p T(i,p) sd_p,
+ *           Lapl (:351-372; sd the analytic Laplacian of a primitive,
+ *           :113-152).
+ * T(nmo, nprim) column-major (wfxinfo T: coefficients of the unnormalised
+ * primitives), w(nmo); both unused for kind 1.  values: host array of G.
+ * kind 2 (atomic orbitals) needs the .log reader and is not provided. */
+int inca_onee(inca_ctx *ctx, int32_t kind, int32_t index, int32_t nmo, const double *T,
+              const double *w, double *values);
+
 /* The Bragg-Slater radius in bohr of each atomic number an[0..n): Slater's
  * radii in angstrom times 1.8897, get_bs_radius (mod_autogrid.f90:142-185);
  * the table inca_autogrid uses when radius is NULL.  Host code. */

@@ -1155,3 +1155,59 @@ int64_t orc_fit_census(int nprim, const double *alpha, const int32_t *tmn, const
   free(np_);
   return fail ? -1 : n;
 }
+
+/* One-electron functions at explicit points (mod_functions.f90): Prim
+ * (:17-31, an unnormalised Cartesian primitive at its centre), sd (:113-152,
+ * its Laplacian), MoOr (:199-210, sum over |T| > 1e-10), Density (:312-329,
+ * sum_i w_i MoOr_i^2, w_i = 2 closed shell), Lapl (:351-372, sum_i 2 w_i
+ * MoOr_i sum_p T(i,p) sd_p).  kind 1 primitive `index`, 3 orbital `index`,
+ * 5 density, 6 Laplacian (1-based index).  T(nmo, nprim) column-major. */
+static double onee_prim(int p, const double *alpha, const int32_t *tmn, int nprim,
+                        const double *cx, const double *cy, const double *cz, const double *r,
+                        double *lap) {
+  const double d[3] = {r[0] - cx[p], r[1] - cy[p], r[2] - cz[p]};
+  const int t[3] = {TMN(tmn, nprim, p, 0), TMN(tmn, nprim, p, 1), TMN(tmn, nprim, p, 2)};
+  const double a = alpha[p];
+  const double e = exp(-a * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+  double f[3], f2[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const double x = d[ax];
+    const int n = t[ax];
+    f[ax] = ipow(x, n);
+    /* d2/dx2 of x^n exp(-a x^2), without the exponential */
+    double v = 4.0 * a * a * ipow(x, n + 2) - 2.0 * a * (2 * n + 1) * ipow(x, n);
+    if (n >= 2) v += (double)(n * (n - 1)) * ipow(x, n - 2);
+    f2[ax] = v;
+  }
+  if (lap)
+    *lap = (f2[0] * f[1] * f[2] + f[0] * f2[1] * f[2] + f[0] * f[1] * f2[2]) * e;
+  return f[0] * f[1] * f[2] * e;
+}
+
+void orc_onee(int kind, int index, int nprim, const double *alpha, const int32_t *tmn,
+              const double *cx, const double *cy, const double *cz, int nmo, const double *T,
+              const double *w, int64_t npts, const double *r, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < npts; ++g) {
+    const double *rg = r + 3 * g;
+    if (kind == 1) {
+      out[g] = onee_prim(index - 1, alpha, tmn, nprim, cx, cy, cz, rg, NULL);
+      continue;
+    }
+    double acc = 0.0;
+    for (int i = (kind == 3 ? index - 1 : 0); i < (kind == 3 ? index : nmo); ++i) {
+      double phi = 0.0, psi = 0.0;
+      for (int p = 0; p < nprim; ++p) {
+        double lp;
+        const double v = onee_prim(p, alpha, tmn, nprim, cx, cy, cz, rg, &lp);
+        const double c = T[i + (int64_t)nmo * p];
+        if (fabs(c) > 1e-10) phi += c * v;
+        psi += c * lp;
+      }
+      if (kind == 3) acc = phi;
+      else if (kind == 5) acc += w[i] * phi * phi;
+      else acc += 2.0 * w[i] * phi * psi;
+    }
+    out[g] = acc;
+  }
+}

@@ -123,6 +123,15 @@ void orc_smearing(int64_t G, const double *coords, const double *weights,
                   const double *density, double u_max, int n_bins,
                   double sigma, double *u, double *P, double *Iavg);
 
+/* One-electron functions at points r(3,npts) (mod_functions.f90: Prim,
+ * MoOr, Density, Lapl): kind 1 primitive `index`, 3 orbital `index`,
+ * 5 density sum_i w_i phi_i^2, 6 the reference's Laplacian
+ * sum_i 2 w_i phi_i sum_p T(i,p) lap(Prim_p).  T(nmo,nprim) column-major;
+ * cx/cy/cz the primitive centres. */
+void orc_onee(int kind, int index, int nprim, const double *alpha, const int32_t *tmn,
+              const double *cx, const double *cy, const double *cz, int nmo, const double *T,
+              const double *w, int64_t npts, const double *r, double *out);
+
 /* Arithmetic of powers, Gauss-Hermite roots and weights: 0 (default)
  * repeated multiplication and correctly rounded roots; 1 the reference's
  * literal libm forms (see inca_oracle.c).  Global; set before a run. */

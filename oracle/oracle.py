@@ -103,6 +103,8 @@ def _configure(L):
     L.orc_bs_radius.restype = C.c_double
     L.orc_smearing.argtypes = [C.c_int64, _dp, _dp, _dp, C.c_double, C.c_int,
                                C.c_double, _dp, _dp, _dp]
+    L.orc_onee.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _dp, _dp, C.c_int,
+                           C.c_void_p, C.c_void_p, C.c_int64, _dp, _dp]
     L.orc_set_arith.argtypes = [C.c_int]
     L.orc_get_arith.restype = C.c_int
     L.orc_fit_rows.argtypes = [C.c_int, _dp, _ip, _dp, _dp, _dp, C.c_int, C.c_double,
@@ -363,3 +365,23 @@ def smearing(coords, weights, density, u_max, n_bins=1000, sigma=0.15):
     lib().orc_smearing(G, coords, _f64(weights), _f64(density), float(u_max),
                        int(n_bins), float(sigma), u, P, Ia)
     return u, P, Ia
+
+
+def onee(kind, index, prims, r, T=None, w=None):
+    """One-electron function at points r (n,3): kind 1 primitive `index`, 3
+    orbital `index`, 5 density, 6 the reference's Laplacian (orc_onee).
+    T (nmo, nprim) coefficients of the unnormalised primitives, w (nmo)."""
+    L = lib()
+    alpha = _f64(prims["alpha"])
+    nprim = len(alpha)
+    tmn = _i32(np.asarray(prims["tmn"]).reshape(nprim, 3).T.reshape(-1))
+    r = _f64(r).reshape(-1)
+    n = r.size // 3
+    Tf = None if T is None else np.asfortranarray(T, dtype=np.float64)
+    nmo = 0 if T is None else Tf.shape[0]
+    wv = None if w is None else _f64(w)
+    out = np.zeros(max(n, 1))
+    L.orc_onee(int(kind), int(index), nprim, alpha, tmn, _f64(prims["x"]), _f64(prims["y"]),
+               _f64(prims["z"]), nmo, None if Tf is None else Tf.ctypes.data,
+               None if wv is None else wv.ctypes.data, n, r, out)
+    return out[:n]

@@ -502,9 +502,11 @@ def test_cube_contraction_matches_term_by_term(engine, dims):
 
 def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
     """inca_set_tiles(p64, 0): a mixed evaluation runs its FP32 range on the
-    FP64 ray tiles, so on a radial scan and a cube it agrees with the FP64
-    evaluation up to summation order; on a multicentre grid (no rays) the
-    FP32 range keeps the mixed kernel and the mixed tolerance."""
+    FP64 ray tiles, so on a radial scan and a cube evaluated term by term it
+    agrees with the FP64 evaluation up to summation order; on a multicentre
+    grid (no rays) the FP32 range keeps the mixed kernel and the mixed
+    tolerance.  (Contracted cubes run their FP32 range on the FP32 matrix
+    cores whatever the tiles: test_cube_contraction_matches_term_by_term.)"""
     s, ijkl, d = small_cluster
     p = s.prims()
     engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
@@ -518,6 +520,7 @@ def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
     ]
     try:
         engine.set_mixed_tail(0.05)
+        engine.set_cube_contraction(0)
         for rays, make in grids:
             make()
             engine.set_tiles(8, 4)
@@ -535,6 +538,7 @@ def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
     finally:
         engine.set_tiles(4, 4)
         engine.set_mixed_tail(0.02)
+        engine.set_cube_contraction(2)
 
 
 def test_point_cutoff_radius_bounds_the_terms(engine, small_cluster):

@@ -310,3 +310,79 @@ def test_smearing_is_linear_in_the_points():
     u, P, _ = O.smearing(pts, w, dens, u_max=9.0, n_bins=700, sigma=0.15)
     ref = _smear_closed_form(u, pts, w, dens, 0.15)
     assert np.allclose(P, ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max())
+
+
+# ---- one-electron functions (mod_functions.f90) -----------------------------
+
+def _onee_system():
+    from inca_parallel_amd import synth
+    rng = np.random.default_rng(41)
+    elements = ["O", "H", "H"]
+    coords = np.array([[0.0, 0.0, 0.0], [1.43, 1.1, 0.0], [-1.43, 1.1, 0.3]])
+    alpha, tmn, cen, ra = synth.make_basis(rng, elements, coords, {0: 2, 1: 1, 2: 1}, {0: 2},
+                                           0.2, 3.0)
+    C = synth.orbitals(rng, synth.overlap_matrix(alpha, tmn, cen), 3)
+    s = synth.System(elements, coords, alpha, tmn, cen, ra, C)
+    norm = np.array([synth.normprim(a, t) for a, t in zip(alpha, tmn)])
+    T = (s.C * norm[:, None]).T  # MO coefficients of the unnormalised primitives
+    return s, T
+
+
+def test_onee_primitive_laplacian_by_finite_differences():
+    """sd (the Laplacian of a primitive, mod_functions.f90:113-152) against a
+    fourth-order central difference of Prim, for s, p and d primitives: the
+    Laplacian of one primitive p is Lapl with T = e_p and w = 1/2 divided by
+    Prim_p."""
+    s, _ = _onee_system()
+    p = s.prims()
+    rng = np.random.default_rng(2)
+    pts = rng.uniform(-1.5, 1.5, size=(6, 3))
+    h = 1e-3
+    for q in range(s.nprim):
+        T = np.zeros((1, s.nprim))
+        T[0, q] = 1.0
+        v = O.onee(1, q + 1, p, pts)
+        lap_v = O.onee(6, 0, p, pts, T, [0.5])
+        fd = np.zeros(len(pts))
+        for ax in range(3):
+            e = np.zeros(3)
+            e[ax] = h
+            f = [O.onee(1, q + 1, p, pts + k * e) for k in (-2, -1, 1, 2)]
+            fd += (-f[0] + 16 * f[1] - 30 * v + 16 * f[2] - f[3]) / (12 * h * h)
+        ok = np.abs(v) > 1e-3 * np.abs(v).max()
+        assert np.allclose(lap_v[ok] / v[ok], fd[ok], rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+
+
+def test_onee_orbitals_and_density_integrate_to_their_counts():
+    """On a fine cube the S-orthonormal orbitals (MoOr) integrate to 1 in
+    square, and the closed-shell density (Density, w = 2) to N = 6; the
+    primitive (Prim) of an s function to (pi/alpha)^(3/2)."""
+    s, T = _onee_system()
+    p = s.prims()
+    h = 0.12
+    ax = np.arange(-7.0, 7.0 + h / 2, h)
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    dv = h ** 3
+    rho = O.onee(5, 0, p, pts, T, [2.0] * 3)
+    assert abs(rho.sum() * dv - 6.0) < 1e-5
+    for i in range(3):
+        phi = O.onee(3, i + 1, p, pts, T)
+        assert abs((phi ** 2).sum() * dv - 1.0) < 1e-5
+    q = int(np.nonzero(np.asarray(p["tmn"]).sum(1) == 0)[0][0])
+    g = O.onee(1, q + 1, p, pts)
+    assert abs(g.sum() * dv - (np.pi / p["alpha"][q]) ** 1.5) < 1e-6 * (np.pi / p["alpha"][q]) ** 1.5
+    # the reference's Laplacian integrates to -4 sum_i int |grad phi_i|^2;
+    # check it pointwise against 2 w phi lap(phi) by central differences
+    lap = O.onee(6, 0, p, pts[::997], T, [2.0] * 3)
+    hd = 1e-3
+    ref = np.zeros(len(lap))
+    for i in range(3):
+        phi0 = O.onee(3, i + 1, p, pts[::997], T)
+        l2 = -6 * phi0
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = hd
+            l2 = l2 + O.onee(3, i + 1, p, pts[::997] + e, T) + O.onee(3, i + 1, p, pts[::997] - e, T)
+        ref += 4.0 * phi0 * l2 / hd ** 2
+    assert np.allclose(lap, ref, rtol=1e-4, atol=1e-5 * np.abs(ref).max())
