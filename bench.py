@@ -87,65 +87,54 @@ def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
                            n2, n_total, int(cnt[2]), grid_r.shape[0], t))
 
 
-def batch_bench(capi, synth, torch, nmol=48, reps=3):
-    """Batched launch on a set of small molecules (scaled-down batch workload:
-    3-6 atoms, s/p/d primitives, autogrid centres with nrad 30-50 x LD0110 or
-    LD0146, nosym, symmetry-unique records generated in HBM).  Compares one
-    replay of the captured persistent launch with the same molecules
-    evaluated one context at a time."""
-    engines, keep = [], []
-    evals = 0.0
-    t_unb_kernel = 0.0
-    t_unb_wall = 0.0
-    for i in range(nmol):
-        nat = 3 + i % 4
-        s = synth.config_cluster(seed=1000 + i, natoms=nat, nheavy=max(1, nat - 1),
-                                 nocc=2 + i % 4, shells_heavy={0: 2, 1: 1, 2: i % 2},
-                                 shells_h={0: 2})
-        recs = s.dm2_unique_torch(device="cuda")
-        keep.append(recs)
-        p = s.prims()
-        e = capi.Engine(0)
-        e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
-        cent, Ps, sf = capi.autogrid(p["an"], p["cartes"], nohydro=True)
-        nq = len(Ps)
-        e.grid_multicenter(cent, [30 + 10 * (i % 3)] * nq, [110 if i % 2 else 146] * nq, sf, Ps,
-                           nosym=True)
-        e.set_quartets(device_ptr=recs.data_ptr(), n=recs.shape[0])
-        e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)  # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, cnt = e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
-        t_unb_wall += time.perf_counter() - t0
-        tm = e.timings()
-        t_unb_kernel += (tm[1] + tm[2]) * 1e-3
-        evals += float(cnt[2]) * e.G
-        engines.append(e)
-    b = capi.Batch(engines)
-    for e in engines:
-        e.close()
-    del keep
-    best = b.autotune()
-    b.evaluate()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ms = []
-    for _ in range(reps):
-        b.evaluate()
-        ms.append(b.info()["last_ms"])
-    wall = (time.perf_counter() - t0) / reps
-    info = b.info()
-    b.close()
+def batch_bench(capi, synth, torch, count=32):
+    """Batched molecules (the batched workload: seeded molecules of 4-20
+    atoms, 30-300 s/p/d primitives, each on its own autogrid with nrad
+    50-150 and nang 110-590, nosym, symmetry-unique records generated in
+    HBM): the `count` cheapest of its 256 molecules by records x grid points,
+    streamed through the persistent batch in arena-sized groups, in FP64 and
+    mixed; against the same molecules evaluated one context at a time."""
+    from inca_parallel_amd import batch_stream as BS
+    seeds = BS.config4_by_cost(count)
+    mols = []
+    for sd in seeds:
+        s, nrad, nang = synth.config4_molecule(sd)
+        mols.append(("seed%d" % sd, s, nrad, nang))
+    # every molecule also one context at a time, or every 4th beyond 32
+    names = {m[0] for k, m in enumerate(mols) if len(mols) <= 32 or k % 4 == 0}
+    r64, g64 = BS.evaluate_stream(mols, capi.INCA_FP64, single=names, log=log)
+    rmx, gmx = BS.evaluate_stream(mols, capi.INCA_MIXED, log=log)
+    evals = sum(g["evals_per_replay"] for g in g64)
+    ms64 = sum(g["replay_ms"] for g in g64)
+    msmx = sum(g["replay_ms"] for g in gmx)
+    singles = [m for m in r64 if "single_fp64" in m]
+    single_ms = sum(m["single_kernel_ms"] for m in singles)
+    single_evals = sum(m["reference_evals"] for m in singles)
+    err_single = max(abs(m["integral"] - m["single_fp64"]) / abs(m["single_fp64"])
+                     for m in singles)
+    err_mixed = max(abs(a["integral"] - b["integral"]) / abs(b["integral"])
+                    for a, b in zip(rmx, r64))
     return {
-        "molecules": nmol, "evals_per_replay": evals, "work_items": info["items"],
-        "persistent_blocks": info["blocks"], "points_per_lane": best,
-        "value": evals / (np.mean(ms) * 1e-3), "unit": "evals/s",
-        "replay_ms": float(np.mean(ms)), "replay_wall_ms": wall * 1e3,
-        "unbatched_kernel_evals_per_s": evals / t_unb_kernel,
-        "unbatched_wall_evals_per_s": evals / t_unb_wall,
-        "note": "batched: one hipGraph replay (persistent class-sorted launch + reductions) over "
-                "prepared quartets; unbatched: the same molecules one inca_evaluate at a time "
-                "(kernel = eval + reduce time, wall = whole call incl. pre-pass and syncs)",
+        "workload": "batched molecules: the %d cheapest of the 256 seeded molecules (seeds "
+                    "1000-1255, 4-20 atoms, 30-300 s/p/d primitives, nrad 50-150 x nang "
+                    "110-590 autogrids, nosym)" % len(mols),
+        "molecules": len(mols), "groups": len(g64),
+        "atoms": [min(m["atoms"] for m in r64), max(m["atoms"] for m in r64)],
+        "primitives": [min(m["nprim"] for m in r64), max(m["nprim"] for m in r64)],
+        "evals_per_replay": evals,
+        "value": evals / (ms64 * 1e-3), "unit": "evals/s",
+        "replay_ms": ms64,
+        "mixed_value": evals / (msmx * 1e-3), "mixed_replay_ms": msmx,
+        "unbatched_kernel_evals_per_s": single_evals / (single_ms * 1e-3),
+        "unbatched_molecules": len(singles),
+        "max_rel_err_batch_vs_single_fp64": err_single,
+        "max_rel_err_mixed_vs_fp64": err_mixed,
+        "replays_bitwise_equal": all(g["replays_bitwise_equal"] for g in g64 + gmx),
+        "median_abs_sum_rule_err": float(np.median([abs(m["sum_rule_err"]) for m in r64])),
+        "points_per_lane": [g["points_per_lane"] for g in g64],
+        "note": "replay = one hipGraph replay of a group's persistent class-sorted launch; "
+                "unbatched = the same prepared quartets evaluated one context at a time "
+                "(eval + reduce kernel time)",
     }
 
 
@@ -350,6 +339,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--natoms", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-molecules", type=int, default=32,
+                    help="molecules of the batched workload in the batch leg (cheapest first; "
+                         "256 is the whole workload, several hours)")
     ap.add_argument("--no-cube", action="store_true")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the learned-grid + mixed + batched leg against the plain FP64 kernel")
@@ -487,7 +479,7 @@ def main():
         out["cube"] = cube_bench(capi, eng)
         log("cube: %s" % out["cube"])
     if not a.no_batch:
-        out["batch"] = batch_bench(capi, synth, torch)
+        out["batch"] = batch_bench(capi, synth, torch, a.batch_molecules)
         log("batch: %s" % out["batch"])
     if not a.no_target:
         out["target"] = target_bench(capi, torch)

@@ -24,14 +24,14 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_grid_points", "inca_get_grid", "inca_get_radial_layout",
            "inca_load_dm2p", "inca_set_quartets", "inca_evaluate", "inca_get_ivec",
            "inca_integrate", "inca_radial_values", "inca_smearing", "inca_get_timings",
-           "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split",
+           "inca_class_histogram", "inca_set_mixed_tail", "inca_get_mixed_split", "inca_set_prepare_chunk", "inca_release_scratch",
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
            "inca_get_eval_counts",
            "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_onee", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
-           "inca_batch_autotune", "inca_batch_info",
+           "inca_batch_autotune", "inca_batch_set_points", "inca_batch_info",
            "inca_mlp_create", "inca_mlp_destroy", "inca_mlp_last_error", "inca_mlp_get_params",
            "inca_mlp_set_params", "inca_mlp_load", "inca_mlp_forward", "inca_mlp_loss_grad", "inca_mlp_adam_step",
            "inca_atom_features", "inca_learned_autogrid",
@@ -79,6 +79,8 @@ def load():
     L.inca_get_timings.argtypes = [vp, _dp]
     L.inca_class_histogram.argtypes = [vp, C.c_int32, vp]
     L.inca_set_mixed_tail.argtypes = [vp, C.c_double]
+    L.inca_set_prepare_chunk.argtypes = [vp, C.c_int64]
+    L.inca_release_scratch.argtypes = [vp]
     L.inca_set_tiles.argtypes = [vp, C.c_int32, C.c_int32]
     L.inca_autotune.argtypes = [vp, vp]
     L.inca_get_prepared.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
@@ -102,6 +104,7 @@ def load():
     L.inca_batch_evaluate.argtypes = [vp, vp, vp]
     L.inca_batch_get_ivec.argtypes = [vp, C.c_int32, vp]
     L.inca_batch_autotune.argtypes = [vp, vp]
+    L.inca_batch_set_points.argtypes = [vp, C.c_int32]
     L.inca_batch_info.argtypes = [vp, vp, vp, vp, vp, vp]
     L.inca_mlp_create.argtypes = [C.c_int32, C.c_uint64, C.POINTER(vp)]
     L.inca_mlp_destroy.argtypes = [vp]
@@ -323,6 +326,10 @@ class Batch:
         self._ck(self._L.inca_batch_autotune(self._h, C.addressof(b)), "inca_batch_autotune")
         return b.value
 
+    def set_points(self, points_per_lane):
+        self._ck(self._L.inca_batch_set_points(self._h, int(points_per_lane)),
+                 "inca_batch_set_points")
+
     def info(self):
         n, p, nb, r, ms = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
         self._ck(self._L.inca_batch_info(self._h, C.addressof(n), C.addressof(p),
@@ -534,6 +541,14 @@ class Engine:
         self._ck(self._L.inca_class_histogram(self._h, int(nbins), h.ctypes.data),
                  "inca_class_histogram")
         return h
+
+    def release_scratch(self):
+        """Free the records and scratch, keeping grid and prepared quartets."""
+        self._ck(self._L.inca_release_scratch(self._h), "inca_release_scratch")
+
+    def set_prepare_chunk(self, records):
+        """Records per pre-pass chunk (inca_set_prepare_chunk)."""
+        self._ck(self._L.inca_set_prepare_chunk(self._h, int(records)), "inca_set_prepare_chunk")
 
     def set_mixed_tail(self, tail):
         self._ck(self._L.inca_set_mixed_tail(self._h, float(tail)), "inca_set_mixed_tail")

@@ -446,6 +446,37 @@ int inca_bragg_slater(int32_t n, const int32_t *an, double *radius) {
   return 0;
 }
 
+int inca_release_scratch(inca_ctx *ctx) {
+  CHECK_CTX(ctx);
+  INCA_HIP(hipSetDevice(ctx->device));
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  auto drop = [](auto *&p, size_t &cap) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  };
+  drop(ctx->d_rec, ctx->rec_cap);
+  ctx->nrec = 0;
+  drop(ctx->d_qtmp, ctx->qtmp_cap);
+  drop(ctx->d_flag, ctx->flag_cap);
+  drop(ctx->d_idx, ctx->idx_cap);
+  drop(ctx->d_tmp, ctx->tmp_cap);
+  drop(ctx->d_partial, ctx->partial_cap);
+  drop(ctx->d_red, ctx->red_cap);
+  drop(ctx->d_mask, ctx->mask_cap);
+  drop(ctx->d_wcount, ctx->wcount_cap);
+  drop(ctx->d_cf, ctx->cf_cap);
+  drop(ctx->d_qf, ctx->qf_cap);
+  return 0;
+}
+
+int inca_set_prepare_chunk(inca_ctx *ctx, int64_t records) {
+  CHECK_CTX(ctx);
+  if (records < 1024) return set_err(ctx, INCA_EINVAL, "pre-pass chunk below 1024 records");
+  ctx->prep_chunk = records;
+  return 0;
+}
+
 int inca_set_mixed_tail(inca_ctx *ctx, double tail) {
   CHECK_CTX(ctx);
   if (!(tail >= 0.0 && tail <= 1.0)) return set_err(ctx, INCA_EINVAL, "tail must be in [0,1]");

@@ -763,6 +763,19 @@ int inca_batch_autotune(inca_batch *b, int32_t *best) {
   return 0;
 }
 
+int inca_batch_set_points(inca_batch *b, int32_t points_per_lane) {
+  if (!b) return INCA_EINVAL;
+  const int P = points_per_lane;
+  if (P != 1 && P != 2 && P != 4 && P != 8)
+    return bfail(b, INCA_EINVAL, "points per lane " + std::to_string(P) + ": 1, 2, 4 or 8");
+  B_HIP(hipSetDevice(b->device));
+  b->P = P;
+  int rc;
+  if ((rc = persistent_blocks(b, b->P)) || (rc = build_items(b, b->P)) || (rc = capture(b)))
+    return rc;
+  return 0;
+}
+
 int inca_batch_info(inca_batch *b, int64_t *nitems, int32_t *points_per_lane, int32_t *blocks,
                     int64_t *replays, double *last_ms) {
   if (!b) return INCA_EINVAL;

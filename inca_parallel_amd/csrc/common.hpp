@@ -102,6 +102,7 @@ struct inca_ctx {
   double term_integral = 0.0; // sum of the prepared terms' integrals over all space
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   int poly_exact = 0;         // 1: exact axis polynomials instead of the fits
+  int64_t prep_chunk = (int64_t)1 << 26;  // records per pre-pass chunk
   double cut_log2 = 100.0;    // per-point cutoff 2^-cut_log2 of a term's peak (0: off)
   int cube_contract = 2;      // cube grids by the per-axis contraction (2: matrix cores, 1: vector units, 0: off)
   double *d_cf = nullptr;     // per-axis factors of a batch of quartets (cube)

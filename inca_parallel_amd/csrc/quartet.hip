@@ -626,8 +626,9 @@ __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict_
   dst[t] = src[idx[r] * stride + w];
 }
 
-// Histogram of the binary exponent of B (header word 5) over records that
-// passed both screens (LDS per block, then one global atomic per non-empty bin).
+// Histogram of the binary exponent of B (header word 5) over the records
+// whose flag is set (all of them when flag is null), LDS per block, then one
+// global atomic per non-empty bin.
 constexpr int kMagBins = 2100;  // ilogb in [-1075, 1024]
 constexpr int kMagOff = 1075;
 
@@ -638,7 +639,7 @@ __global__ __launch_bounds__(kT) void k_mag_hist(const double *__restrict__ qtmp
   for (int b = threadIdx.x; b < kMagBins; b += kT) h[b] = 0;
   __syncthreads();
   for (int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x; r < n; r += (int64_t)gridDim.x * kT) {
-    if (!flag[r]) continue;
+    if (flag && !flag[r]) continue;
     const double u = qtmp[r * stride + 5];
     int b = (u > 0.0) ? ilogb(u) + kMagOff : 0;
     b = b < 0 ? 0 : (b >= kMagBins ? kMagBins - 1 : b);
@@ -649,7 +650,8 @@ __global__ __launch_bounds__(kT) void k_mag_hist(const double *__restrict__ qtmp
     if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
 }
 
-// heavy (B >= tau) or light (B < tau) among the records that passed
+// heavy (B >= tau) or light (B < tau) among the records that passed (all of
+// them when pass is null)
 __global__ __launch_bounds__(kT) void k_split_flags(const double *__restrict__ qtmp, int stride,
                                                     const uint8_t *__restrict__ pass, int64_t n,
                                                     double tau, int heavy,
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(kT) void k_split_flags(const double *__restrict__ q
   const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (r >= n) return;
   uint8_t f = 0;
-  if (pass[r]) {
+  if (!pass || pass[r]) {
     const bool big = qtmp[r * stride + 5] >= tau;
     f = heavy ? big : !big;
   }
@@ -722,6 +724,79 @@ int build_pair_table(inca_ctx *ctx) {
   return 0;
 }
 
+// Grow a device buffer to hold `need` elements, keeping its first `used`.
+// The new size is 1.5 times the old one, or `need` when that does not fit
+// beside the old buffer; when `need` does not fit either, the kept elements
+// wait in pinned host memory while the old buffer is freed, so that a set
+// of prepared quartets as large as the free device memory can be built.
+template <class T>
+int grow_keep(inca_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t used) {
+  if (need <= cap && ptr) return 0;
+  size_t n = std::max<size_t>(need, cap + cap / 2);
+  T *p = nullptr;
+  hipError_t e = hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess && n > need) {
+    (void)hipGetLastError();
+    n = need;
+    e = hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+  }
+  if (e != hipSuccess && used && ptr) {
+    (void)hipGetLastError();
+    T *h = nullptr;
+    e = hipHostMalloc((void **)&h, used * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess)
+      return set_err(ctx, INCA_ENOMEM, "pinned staging of %zu bytes failed: %s", used * sizeof(T),
+                     hipGetErrorString(e));
+    e = hipMemcpyAsync(h, ptr, used * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipHostFree(h);
+      return hip_fail(ctx, e, "buffer growth (to host)", __LINE__);
+    }
+    (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    e = hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipHostFree(h);
+      return set_err(ctx, INCA_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                     hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(p, h, used * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipHostFree(h);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return hip_fail(ctx, e, "buffer growth (from host)", __LINE__);
+    }
+    ptr = p;
+    cap = n;
+    return 0;
+  }
+  if (e != hipSuccess)
+    return set_err(ctx, INCA_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                   hipGetErrorString(e));
+  if (used && ptr) {
+    e = hipMemcpyAsync(p, ptr, used * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return hip_fail(ctx, e, "buffer growth", __LINE__);
+    }
+  }
+  if (ptr) (void)hipFree(ptr);
+  ptr = p;
+  cap = n;
+  return 0;
+}
+
+// The pre-pass runs over chunks of ctx->prep_chunk records (default 2^26).
+// A chunk's uncompacted prepared slots (one record of `stride` doubles per
+// DM2 record) live in d_qtmp, and the survivors of every chunk are appended
+// to d_q in file order, so a stream of any length is prepared in bounded
+// scratch (2^26 records: 19 GB of slots for s/p/d bases; the radial-plot
+// benchmark's 65.9M records are one chunk).
+
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
   hipStream_t st = ctx->stream;
   ctx->rho_h = 0.0;  // word 7 of the new records holds no rho yet
@@ -743,53 +818,56 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   INCA_HIP(hipStreamSynchronize(st));
   const int64_t nvalid = (int64_t)hcnt[0];
-  if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)(nvalid > 0 ? nvalid : 1) * stride)))
-    return rc;
-  if ((rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)(nvalid > 0 ? nvalid : 1)))) return rc;
-  if ((rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)(nvalid > 0 ? nvalid : 1)))) return rc;
+  const int64_t chunk = std::max<int64_t>(1, std::min(nvalid, ctx->prep_chunk));
+  if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)chunk * stride))) return rc;
+  if ((rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)chunk))) return rc;
+  if ((rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)chunk))) return rc;
   // mod_intracule.f90:163
   const double lim = thresh * (1.0 / ((double)nprim * ((double)nprim + 1.0) * 0.5));
-  const bool want_mom = nvalid > 0;
   // block partials of sum M (the small-degree pass, then the deferred
-  // records' pass), then the total after them
-  const int64_t nb1 = nblk(nvalid);
+  // records' pass), then the chunk's total after them
+  const int64_t nb1 = nblk(chunk);
   // temporaries from the context's scratch block: the term-integral
-  // partials, the deferral flags, and the mixed split's histogram and flags
+  // partials, the deferral flags and the mixed split's histogram
   Carve cv;
-  const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(nvalid),
-               o_hist = cv.add<unsigned long long>(kMagBins), o_f2 = cv.add<uint8_t>(nvalid);
+  const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(chunk),
+               o_hist = cv.add<unsigned long long>(kMagBins);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
-  double *d_mom = want_mom ? Carve::at<double>(ctx->d_tmp, o_mom) : nullptr;
-  uint8_t *d_defer = nullptr;
-  ctx->term_integral = 0.0;
-  if (nvalid > 0) {
-    // Fitted polynomials: records whose axes are all of degree <= 4 (most
-    // of them) go through the register-resident fits; the rest are
-    // compacted in file order and handled by the scratch kernel sized for
-    // the basis.  Exact polynomials: the scratch kernel takes every record.
-    const bool split = !ctx->poly_exact;
+  double *d_mom = Carve::at<double>(ctx->d_tmp, o_mom);
+  // Fitted polynomials: records whose axes are all of degree <= 4 (most
+  // of them) go through the register-resident fits; the rest are compacted
+  // in file order and handled by the scratch kernel sized for the basis.
+  // Exact polynomials: the scratch kernel takes every record.
+  const bool split = !ctx->poly_exact;
+  uint8_t *d_defer = (split && la > 4) ? Carve::at<uint8_t>(ctx->d_tmp, o_def) : nullptr;
+  double mom_total = 0.0;  // chunk totals added in file order
+  int64_t q2 = 0;
+  for (int64_t r0 = 0; r0 < nvalid; r0 += chunk) {
+    const int64_t n = std::min(chunk, nvalid - r0);
+    const int32_t *recs = (const int32_t *)ctx->d_rec + 6 * r0;
     hipError_t e = hipSuccess;
-    if (split && la > 4) d_defer = Carve::at<uint8_t>(ctx->d_tmp, o_def);
-    if (split && e == hipSuccess) {
-      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(nvalid)), dim3(kT), 0, st,
-                         (const int32_t *)ctx->d_rec, nvalid, (const int64_t *)nullptr, nprim,
-                         ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim, ctx->normalize, stride, 0,
-                         ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,
-                         d_defer, ctx->d_cnt, d_mom);
+    // the small-degree pass's nblk(n) partials end at d_mom + nb1, where the
+    // deferred pass's begin, so that one fixed-order sum covers both
+    double *small_mom = d_mom + (nb1 - (int64_t)nblk(n));
+    if (split) {
+      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n)), dim3(kT), 0, st, recs, n,
+                         (const int64_t *)nullptr, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair,
+                         lim, ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
+                         ctx->d_qtmp, ctx->d_flag, d_defer, ctx->d_cnt, small_mom);
       e = hipGetLastError();
     }
-    int64_t nrun = split ? 0 : nvalid;  // records for the scratch kernel
+    int64_t nrun = split ? 0 : n;  // records for the scratch kernel
     if (e == hipSuccess && d_defer) {
-      rc = compact_flags(ctx, d_defer, nvalid, ctx->d_idx, &nrun);
+      rc = compact_flags(ctx, d_defer, n, ctx->d_idx, &nrun);
       if (rc) return rc;
     }
     const int64_t *run_idx = split ? ctx->d_idx : nullptr;
     double *run_mom = split ? d_mom + nb1 : d_mom;
     if (e == hipSuccess && nrun > 0) {
 #define INCA_PREP(NPM)                                                                        \
-  hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st,                \
-                     (const int32_t *)ctx->d_rec, nrun, run_idx, nprim, ctx->d_prim,          \
-                     ctx->d_tmn, ctx->d_pair, lim, ctx->normalize, stride, ctx->poly_exact,   \
+  hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st, recs, nrun,    \
+                     run_idx, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,               \
+                     ctx->normalize, stride, ctx->poly_exact,                                 \
                      ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
                      (uint8_t *)nullptr, ctx->d_cnt, run_mom)
       if (la <= 4) INCA_PREP(5);
@@ -800,24 +878,39 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
 #undef INCA_PREP
       e = hipGetLastError();
     }
-    const int64_t nbt =
-        split ? nb1 + (nrun > 0 ? (int64_t)nblk(nrun) : 0) : (int64_t)nblk(nrun);
-    if (e == hipSuccess && want_mom) {
-      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, d_mom, nbt, d_mom + nbt);
+    const int64_t nbt = split ? nblk(n) + (nrun > 0 ? (int64_t)nblk(nrun) : 0)
+                              : (int64_t)nblk(nrun);
+    const double *sum_from = split ? small_mom : d_mom;
+    double chunk_mom = 0.0;
+    if (e == hipSuccess && nbt > 0) {
+      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, sum_from, nbt, d_mom + 2 * nb1);
       e = hipGetLastError();
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(&chunk_mom, d_mom + 2 * nb1, sizeof(double), hipMemcpyDeviceToHost,
+                           st);
     }
-    if (e == hipSuccess && want_mom)
-      e = hipMemcpyAsync(&ctx->term_integral, d_mom + nbt, sizeof(double),
-                         hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && want_mom) e = hipStreamSynchronize(st);
-    // in the grids' convention: their weights carry 2 pi where the volume
-    // element has 4 pi (mod_build_grid.f90:132)
-    ctx->term_integral *= 0.5;
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e, "quartet preparation", __LINE__);
+    mom_total += chunk_mom;
+    // survivors of the chunk, appended in file order
+    int64_t qc = 0;
+    if ((rc = compact_flags(ctx, ctx->d_flag, n, ctx->d_idx, &qc))) return rc;
+    if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 + qc) * stride,
+                        (size_t)q2 * stride)))
+      return rc;
+    if (qc > 0) {
+      hipLaunchKernelGGL(k_gather_records, dim3(nblk(qc * stride)), dim3(kT), 0, st,
+                         ctx->d_qtmp, ctx->d_idx, qc, stride, ctx->d_q + q2 * stride);
+      INCA_HIP(hipGetLastError());
+    }
+    q2 += qc;
   }
-  int64_t q2 = 0;
-  if ((rc = compact_flags(ctx, ctx->d_flag, nvalid, ctx->d_idx, &q2))) return rc;
-  if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride))) return rc;
+  // in the grids' convention: their weights carry 2 pi where the volume
+  // element has 4 pi (mod_build_grid.f90:132)
+  ctx->term_integral = 0.5 * mom_total;
+  if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride,
+                      (size_t)q2 * stride)))
+    return rc;
   ctx->q_fp64 = q2;
   ctx->mixed_tau = 0.0;
   ctx->prep_precision = precision;
@@ -830,14 +923,13 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     std::vector<unsigned long long> h(kMagBins, 0);
     INCA_HIP(hipMemcpyAsync(d_h, h.data(), kMagBins * sizeof(unsigned long long),
                             hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_mag_hist, dim3(1024), dim3(kT), 0, st, ctx->d_qtmp, stride,
-                       ctx->d_flag, nvalid, d_h);
+    hipLaunchKernelGGL(k_mag_hist, dim3(1024), dim3(kT), 0, st, ctx->d_q, stride,
+                       (const uint8_t *)nullptr, q2, d_h);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
       e = hipMemcpyAsync(h.data(), d_h, kMagBins * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-
     if (e != hipSuccess) return hip_fail(ctx, e, "magnitude histogram", __LINE__);
     // mass of a bin ~ count * 2^(exponent + 1/2)
     std::vector<double> mass(kMagBins);
@@ -855,32 +947,39 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     int cut = 0;
     while (cut < kMagBins && cum + mass[cut] <= budget) cum += mass[cut++];
     const double tau = std::ldexp(1.0, cut - kMagOff);
-    ctx->mixed_tau = tau;
-    uint8_t *d_f2 = Carve::at<uint8_t>(ctx->d_tmp, o_f2);
-    int64_t nh = 0, nl = 0;
-    hipLaunchKernelGGL(k_split_flags, dim3(nblk(nvalid)), dim3(kT), 0, st, ctx->d_qtmp, stride,
-                       ctx->d_flag, nvalid, tau, 1, d_f2);
-    rc = compact_flags(ctx, d_f2, nvalid, ctx->d_idx, &nh);
-    if (!rc && nh > 0)
-      hipLaunchKernelGGL(k_gather_records, dim3(nblk(nh * stride)), dim3(kT), 0, st,
-                         ctx->d_qtmp, ctx->d_idx, nh, stride, ctx->d_q);
-    if (!rc) {
-      hipLaunchKernelGGL(k_split_flags, dim3(nblk(nvalid)), dim3(kT), 0, st, ctx->d_qtmp,
-                         stride, ctx->d_flag, nvalid, tau, 0, d_f2);
-      rc = compact_flags(ctx, d_f2, nvalid, ctx->d_idx, &nl);
+    // stable partition of the survivors into d_qtmp (heavy, then light), which
+    // then becomes the quartet list.  Without device room for that second
+    // copy every survivor stays FP64 (mixed_tau 0, no FP32 range).
+    if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)q2 * stride)) ||
+        (rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)q2)) ||
+        (rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)q2))) {
+      if (rc != INCA_ENOMEM) return rc;
+      (void)hipGetLastError();
+      rc = 0;
+    } else {
+      ctx->mixed_tau = tau;
+      int64_t nh = 0, nl = 0;
+      hipLaunchKernelGGL(k_split_flags, dim3(nblk(q2)), dim3(kT), 0, st, ctx->d_q, stride,
+                         (const uint8_t *)nullptr, q2, tau, 1, ctx->d_flag);
+      rc = compact_flags(ctx, ctx->d_flag, q2, ctx->d_idx, &nh);
+      if (!rc && nh > 0)
+        hipLaunchKernelGGL(k_gather_records, dim3(nblk(nh * stride)), dim3(kT), 0, st, ctx->d_q,
+                           ctx->d_idx, nh, stride, ctx->d_qtmp);
+      if (!rc) {
+        hipLaunchKernelGGL(k_split_flags, dim3(nblk(q2)), dim3(kT), 0, st, ctx->d_q, stride,
+                           (const uint8_t *)nullptr, q2, tau, 0, ctx->d_flag);
+        rc = compact_flags(ctx, ctx->d_flag, q2, ctx->d_idx, &nl);
+      }
+      if (!rc && nl > 0)
+        hipLaunchKernelGGL(k_gather_records, dim3(nblk(nl * stride)), dim3(kT), 0, st, ctx->d_q,
+                           ctx->d_idx, nl, stride, ctx->d_qtmp + nh * stride);
+      e = hipStreamSynchronize(st);
+      if (rc) return rc;
+      if (e != hipSuccess) return hip_fail(ctx, e, "mixed split", __LINE__);
+      std::swap(ctx->d_q, ctx->d_qtmp);
+      std::swap(ctx->q_cap, ctx->qtmp_cap);
+      ctx->q_fp64 = nh;
     }
-    if (!rc && nl > 0)
-      hipLaunchKernelGGL(k_gather_records, dim3(nblk(nl * stride)), dim3(kT), 0, st,
-                         ctx->d_qtmp, ctx->d_idx, nl, stride, ctx->d_q + nh * stride);
-    e = hipStreamSynchronize(st);
-
-    if (rc) return rc;
-    if (e != hipSuccess) return hip_fail(ctx, e, "mixed split", __LINE__);
-    ctx->q_fp64 = nh;
-  } else if (q2 > 0) {
-    hipLaunchKernelGGL(k_gather_records, dim3(nblk(q2 * stride)), dim3(kT), 0, st, ctx->d_qtmp,
-                       ctx->d_idx, q2, stride, ctx->d_q);
-    INCA_HIP(hipGetLastError());
   }
   INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(hcnt), hipMemcpyDeviceToHost, st));
   INCA_HIP(hipStreamSynchronize(st));

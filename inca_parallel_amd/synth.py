@@ -359,3 +359,67 @@ def config_cluster(seed=2, natoms=10, nheavy=7, nocc=20, exp_lo=0.05, exp_hi=500
     S = overlap_matrix(alpha, tmn, cen)
     C = orbitals(rng, S, nocc)
     return System(elements, coords, alpha, tmn, cen, ra, C)
+
+
+CONFIG4_SEEDS = range(1000, 1256)
+CONFIG4_NRAD = (50, 75, 100, 150)
+CONFIG4_NANG = (110, 194, 302, 590)
+
+
+def _config4_layout(seed):
+    """Atoms, geometry, basis and grid orders of a batch molecule (the rng
+    draws before the orbitals')."""
+    rng = np.random.default_rng(seed)
+    nat = int(rng.integers(4, 21))
+    nheavy = max(1, min(nat, int(round(nat * rng.uniform(0.4, 0.8)))))
+    elements = list(rng.choice(["C", "N", "O"], size=nheavy)) + ["H"] * (nat - nheavy)
+    nelec = sum(ZNUM[e] for e in elements)
+    nocc = nelec // 2
+    target = int(rng.integers(max(30, nocc + 4, nat), 301))
+    nrad = int(rng.choice(CONFIG4_NRAD))
+    nang = int(rng.choice(CONFIG4_NANG))
+    coords = grow_molecule(rng, elements)
+    # shells: one s per atom, then random s/p/d (heavy) or s/p (H) shells
+    shells = [[0] for _ in elements]
+    count = nat
+    while count < target:
+        a = int(rng.integers(nat))
+        L = int(rng.choice([0, 1, 2])) if elements[a] != "H" else int(rng.choice([0, 1]))
+        if count + len(SHELLS[L]) > 300:
+            L = 0
+        shells[a].append(L)
+        count += len(SHELLS[L])
+    alpha, tmn, cen, ra = [], [], [], []
+    for a, (el, c) in enumerate(zip(elements, coords)):
+        for L in shells[a]:
+            e = float(np.exp(rng.uniform(np.log(0.05), np.log(500.0))))
+            for t in SHELLS[L]:
+                alpha.append(e)
+                tmn.append(t)
+                cen.append(c)
+                ra.append(a + 1)
+    return (rng, elements, coords, np.array(alpha), np.array(tmn, dtype=np.int32),
+            np.array(cen), np.array(ra, dtype=np.int32), nocc, nrad, nang)
+
+
+def config4_molecule(seed):
+    """One molecule of the batched workload: 4-20 atoms (C/N/O/H, 40-80%
+    heavy), bonds 1.8-2.9 bohr, 30-300 Cartesian s/p/d primitives with
+    exponents log-uniform in [0.05, 500] (shells added at random until the
+    seeded target count is reached; every atom keeps one s shell), an HF-like
+    DM2 from N/2 seeded occupied orbitals (N the electron count of the
+    neutral molecule, rounded down to even), and the seeded autogrid orders nrad in
+    {50, 75, 100, 150} and nang in {110, 194, 302, 590}.  Returns
+    (System, nrad, nang)."""
+    rng, elements, coords, alpha, tmn, cen, ra, nocc, nrad, nang = _config4_layout(seed)
+    C = orbitals(rng, overlap_matrix(alpha, tmn, cen), nocc)
+    return System(elements, coords, alpha, tmn, cen, ra, C), nrad, nang
+
+
+def config4_sizes(seed):
+    """(atoms, primitives, electrons, unique records, nrad, nang) of a batch
+    molecule, without its orbitals."""
+    _, elements, _, alpha, _, _, _, nocc, nrad, nang = _config4_layout(seed)
+    nprim = len(alpha)
+    npair = nprim * (nprim + 1) // 2
+    return len(elements), nprim, 2 * nocc, npair * (npair + 1) // 2, nrad, nang

@@ -109,6 +109,23 @@ int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec,
 int inca_set_quartets(inca_ctx *ctx, int64_t n, const void *records,
                       int32_t on_device);
 
+/* Free the DM2 records and the scratch of the pre-pass and of past grid
+ * loops, keeping the grid and the prepared quartets: for holding many
+ * prepared molecules at once (a batch being assembled).  Later grid loops
+ * allocate their scratch again; inca_prepare needs new records
+ * (inca_load_dm2p / inca_set_quartets) and inca_autotune a new
+ * inca_prepare.  No reference counterpart. */
+int inca_release_scratch(inca_ctx *ctx);
+
+/* Records per chunk of the pre-pass (default 2^26, at least 1024).  The
+ * records are decoded, screened and fitted a chunk at a time and each
+ * chunk's survivors appended in file order, so the pre-pass scratch is
+ * bounded by the chunk (stride doubles per record) whatever the length of
+ * the stream; results do not depend on the chunk size (the term-integral
+ * sum is grouped by chunk).  The reference reads 40,960 records per batch
+ * (mod_intracule.f90:66, 229-262); no counterpart as a setting. */
+int inca_set_prepare_chunk(inca_ctx *ctx, int64_t records);
+
 /* The quartet loop of calculate_intracule (mod_intracule.f90:198-388):
  * decode + normalise (:276-285), screen 1 (:297-305), prefactors and
  * polycoef on three axes (:307-341, mod_intrastuff.f90:127-194), screen 2
@@ -226,6 +243,9 @@ int inca_batch_get_ivec(inca_batch *b, int32_t mol, double *ivec);
 /* Times points-per-lane 1, 2, 4, 8 on the batch, keeps the fastest in
  * *best and re-captures the graph with it. */
 int inca_batch_autotune(inca_batch *b, int32_t *best);
+/* Re-captures the graph with points-per-lane P (1, 2, 4 or 8): a value
+ * the autotune found on a like batch, without timing all four again. */
+int inca_batch_set_points(inca_batch *b, int32_t points_per_lane);
 /* Work items, points per lane, persistent workgroups, graph replays and
  * the device time of the last replay in ms. */
 int inca_batch_info(inca_batch *b, int64_t *nitems, int32_t *points_per_lane,

@@ -14,6 +14,24 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
+def pytest_collection_finish(session):
+    """On a machine with a GPU, start torch's HIP runtime before any test
+    creates an engine context: torch ships its own HIP runtime next to the
+    system one that libinca_hip.so uses, and tests that generate records
+    with torch after an engine exists found no device in torch when torch
+    came second."""
+    if not os.path.exists("/dev/kfd"):
+        return
+    if not any(item.get_closest_marker("gpu") for item in session.items):
+        return
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 @pytest.fixture(scope="session")
 def ch4():
     """The reference's CH4 inputs (tests/data of markelgtx/INCA_parallel)."""

@@ -96,6 +96,12 @@ def test_batch_autotune_recaptures(molecules):
         _check(b, molecules, "autotuned")
         _check(b, molecules, "autotuned replay")
         print("autotuned points per lane %d, %s" % (best, b.info()))
+        for P in (1, 8, best):
+            b.set_points(P)
+            assert b.info()["points_per_lane"] == P
+            _check(b, molecules, "points per lane %d" % P)
+        with pytest.raises(capi.IncaError, match="1, 2, 4 or 8"):
+            b.set_points(3)
     finally:
         b.close()
 

@@ -654,3 +654,64 @@ def test_fp64_exp_accuracy_through_one_s_primitive(engine):
     print("exp over s in [0, %.1f]: max rel err %.2e at s = %.3f (%d points)" % (
         (x * x).max(), err.max(), (x[keep] ** 2)[np.argmax(err)], keep.sum()))
     assert iv[0] > 0 and err.max() < 1e-15
+
+
+def test_prepare_in_chunks_matches_one_pass(engine, small_cluster):
+    """The pre-pass over chunks of records (survivors appended in file order)
+    gives the one-pass result: equal counters, bitwise-equal prepared
+    records in FP64 and in the mixed split, and the term integral within
+    1e-14 (its partial sums are grouped by chunk)."""
+    s, ijkl, d = small_cluster
+    p = s.prims()
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=ijkl, d=d)
+    engine.grid_points(np.zeros((1, 3)))
+    out = {}
+    try:
+        for chunk in (1 << 26, 4096, 7777):
+            engine.set_prepare_chunk(chunk)
+            for prec in (capi.INCA_FP64, capi.INCA_MIXED):
+                cnt = engine.prepare(prec, THRESH)
+                out[(chunk, prec)] = (list(cnt), engine.prepared(), engine.analytic_integral(),
+                                      engine.mixed_split()[:2])
+    finally:
+        engine.set_prepare_chunk(1 << 26)
+    assert len(d) > 8 * 7777
+
+    def used(rec):
+        """the header and the coefficient words of each record (the slots past
+        a record's degrees are never written)"""
+        cls = rec[:, 4].view(np.int64)
+        n = 8 + (cls & 0xff) + ((cls >> 8) & 0xff) + ((cls >> 16) & 0xff) + 3
+        mask = np.arange(rec.shape[1])[None, :] < n[:, None]
+        return np.where(mask, rec, 0.0)
+
+    for chunk in (4096, 7777):
+        for prec in (capi.INCA_FP64, capi.INCA_MIXED):
+            a, b = out[(chunk, prec)], out[(1 << 26, prec)]
+            assert a[0] == b[0] and a[3] == b[3]
+            assert np.array_equal(used(a[1]), used(b[1]))
+            assert abs(a[2] - b[2]) <= 1e-14 * abs(b[2])
+
+
+def test_release_scratch_keeps_grid_and_prepared(engine, small_cluster):
+    """inca_release_scratch frees the records and the pre-pass / grid-loop
+    scratch; the grid loop then allocates its scratch again and gives the
+    same result bit for bit, and a new inca_prepare asks for records."""
+    s, ijkl, d = small_cluster
+    p = s.prims()
+    engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
+    engine.set_quartets(ijkl=ijkl, d=d)
+    engine.grid_points(np.array([[0.3, -0.2, 0.1], [1.5, 0.4, -0.7], [0.0, 0.0, 2.2]]))
+    out = []
+    for prec in (capi.INCA_FP64, capi.INCA_MIXED):
+        engine.set_quartets(ijkl=ijkl, d=d)
+        engine.prepare(prec, THRESH)
+        before = engine.evaluate_prepared(prec).copy()
+        engine.release_scratch()
+        after = engine.evaluate_prepared(prec)
+        assert np.array_equal(before, after)
+        with pytest.raises(capi.IncaError, match="records not set"):
+            engine.prepare(prec, THRESH)
+        out.append(before)
+    assert np.abs(out[0]).max() > 0
