@@ -29,7 +29,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 FC = shutil.which("amdflang") or os.path.join(ROCM, "bin", "amdflang")
 
-HIP_SOURCES = ["util.hip", "grid.hip", "quartet.hip", "eval.hip", "autogrid.hip", "batch.hip",
+HIP_SOURCES = ["util.hip", "grid.hip", "quartet.hip", "group.hip", "eval.hip", "autogrid.hip",
+               "batch.hip",
                "mlp.hip", "onee.hip",
                "api.hip"]
 # the Fortran modules in dependency order

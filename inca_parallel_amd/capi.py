@@ -28,7 +28,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_set_tiles", "inca_autotune", "inca_get_prepared",
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
-           "inca_get_eval_counts",
+           "inca_get_eval_counts", "inca_set_grouping", "inca_get_groups", "inca_get_eval_flops",
            "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_onee", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_set_points", "inca_batch_info",
@@ -88,6 +88,9 @@ def load():
     L.inca_get_mixed_split.argtypes = [vp, vp, vp, vp]
     L.inca_get_analytic_integral.argtypes = [vp, vp]
     L.inca_set_point_cutoff.argtypes = [vp, C.c_double]
+    L.inca_set_grouping.argtypes = [vp, C.c_int32]
+    L.inca_get_groups.argtypes = [vp, vp, vp, vp]
+    L.inca_get_eval_flops.argtypes = [vp, vp]
     L.inca_set_cube_contraction.argtypes = [vp, C.c_int32]
     L.inca_get_eval_counts.argtypes = [vp, vp, vp]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
@@ -585,6 +588,24 @@ class Engine:
         self._ck(self._L.inca_get_eval_counts(self._h, C.addressof(a), C.addressof(b)),
                  "inca_get_eval_counts")
         return a.value, b.value
+
+    def set_grouping(self, on):
+        """1 (default): evaluate by shell-quartet groups (built at the next
+        preparation); 0: record by record (inca_set_grouping)."""
+        self._ck(self._L.inca_set_grouping(self._h, int(bool(on))), "inca_set_grouping")
+
+    def groups(self):
+        """(groups, FP64-range groups, pool coefficients) of the last preparation."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self._ck(self._L.inca_get_groups(self._h, C.addressof(a), C.addressof(b),
+                                         C.addressof(c)), "inca_get_groups")
+        return a.value, b.value, c.value
+
+    def eval_flops(self):
+        """FP64-equivalent operations of the last grid loop when it ran by groups."""
+        f = C.c_double()
+        self._ck(self._L.inca_get_eval_flops(self._h, C.addressof(f)), "inca_get_eval_flops")
+        return f.value
 
     def set_polynomials(self, exact):
         self._ck(self._L.inca_set_polynomials(self._h, int(bool(exact))), "inca_set_polynomials")

@@ -82,7 +82,8 @@ int inca_destroy(inca_ctx *c) {
                   c->d_wvee, c->d_map, c->d_rec, c->d_q, c->d_qtmp, c->d_qf, c->d_flag,
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
                   c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup,
-                  c->d_red, c->d_tmp, c->d_cf};
+                  c->d_red, c->d_tmp, c->d_cf, c->d_gh, c->d_gpool, c->d_gpool32,
+                  c->d_wflop, c->d_flops};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
@@ -134,6 +135,9 @@ int inca_set_primitives(inca_ctx *ctx, int32_t nprim, const double *alpha, const
   ctx->nprim = nprim;
   ctx->normalize = normalize ? 1 : 0;
   ctx->lmax_axis = 4 * tmax;
+  ctx->lmax_prim = 0;
+  for (int i = 0; i < nprim; ++i)
+    ctx->lmax_prim = std::max(ctx->lmax_prim, tmn[i] + tmn[nprim + i] + tmn[2 * nprim + i]);
   int rc = build_pair_table(ctx);
   if (rc) return rc;
   INCA_HIP(hipStreamSynchronize(ctx->stream));
@@ -486,6 +490,8 @@ int inca_set_mixed_tail(inca_ctx *ctx, double tail) {
 
 int inca_get_mixed_split(inca_ctx *ctx, int64_t *n_fp64, int64_t *n_fp32, double *tau) {
   CHECK_CTX(ctx);
+  INCA_HIP(hipSetDevice(ctx->device));
+  if (int rc = split_records(ctx)) return rc;
   if (n_fp64) *n_fp64 = ctx->q_fp64;
   if (n_fp32) *n_fp32 = ctx->q2 - ctx->q_fp64;
   if (tau) *tau = ctx->mixed_tau;
@@ -514,6 +520,29 @@ int inca_get_eval_counts(inca_ctx *ctx, int64_t *reference, int64_t *executed) {
   return 0;
 }
 
+int inca_set_grouping(inca_ctx *ctx, int32_t on) {
+  CHECK_CTX(ctx);
+  if (on != 0 && on != 1) return set_err(ctx, INCA_EINVAL, "grouping must be 0 or 1");
+  ctx->grouping = on;
+  return 0;
+}
+
+int inca_get_groups(inca_ctx *ctx, int64_t *ngroups, int64_t *ngroups_fp64,
+                    int64_t *ncoefficients) {
+  CHECK_CTX(ctx);
+  if (ngroups) *ngroups = ctx->ng;
+  if (ngroups_fp64) *ngroups_fp64 = ctx->ng_fp64;
+  if (ncoefficients) *ncoefficients = ctx->gpool_n;
+  return 0;
+}
+
+int inca_get_eval_flops(inca_ctx *ctx, double *flops) {
+  CHECK_CTX(ctx);
+  if (!flops) return set_err(ctx, INCA_EINVAL, "null output");
+  *flops = ctx->grouped_last ? ctx->flops_executed : 0.0;
+  return 0;
+}
+
 int inca_get_analytic_integral(inca_ctx *ctx, double *total) {
   CHECK_CTX(ctx);
   if (!total) return set_err(ctx, INCA_EINVAL, "null output");
@@ -536,6 +565,9 @@ int inca_set_tiles(inca_ctx *ctx, int32_t p64, int32_t p32) {
                    "points per lane must be 1, 2, 4, 6, 8 or 10 (or 0 for the FP32 range on ray tiles)");
   ctx->tile_p[0] = p64;
   ctx->tile_p[1] = p32;
+  // the group kernels take 1, 2, 4, 6 or 8 (10 -> 8; 0 keeps their choice)
+  ctx->gtile_p[0] = p64 == 10 ? 8 : p64;
+  if (p32 != 0) ctx->gtile_p[1] = p32 == 10 ? 8 : p32;
   return 0;
 }
 
@@ -550,6 +582,10 @@ int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out, int3
   if (stride) *stride = ctx->qstride;
   if (first < 0 || n < 0 || first + n > ctx->q2)
     return set_err(ctx, INCA_EINVAL, "prepared-record range out of bounds");
+  if (n > 0 && out) {
+    INCA_HIP(hipSetDevice(ctx->device));
+    if (int rc = split_records(ctx)) return rc;  // the FP64 range first
+  }
   if (n > 0 && out)
     INCA_HIP(hipMemcpy(out, ctx->d_q + first * ctx->qstride, n * ctx->qstride * sizeof(double),
                        hipMemcpyDeviceToHost));

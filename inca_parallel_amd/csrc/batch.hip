@@ -9,7 +9,11 @@
 //     and its cost is known in advance.  One device radix sort over
 //     (molecule, precision, class) keys orders the whole batch, and a
 //     run-length encoding of the sorted keys gives the class segments;
-//   * all grid points back to back, with per-molecule offsets.
+//   * all grid points back to back, with per-molecule offsets;
+//   * for contexts prepared with shell-quartet groups (group.hip), their
+//     group headers take the place of records (the class is then the
+//     group's total degree) and their coefficients are copied into one
+//     pool, the headers' pool offsets moved accordingly.
 // Work items are (molecule, block tile of 256 P points, class chunk of at most
 // 4096 quartets).  All items of one chunk go to one of 8 queues (chunk mod
 // 8), each ordered by the chunks' estimated cost, largest first, and
@@ -21,7 +25,7 @@
 // molecule.  Each item writes its own partial-sum slot and a
 // fixed-order reduction follows, so results do not depend on which
 // workgroup took which item.  The per-item arithmetic is the single-context
-// FP64 kernel's (eval_core.hpp).  Points per lane are autotuned over
+// FP64 kernel's, or its group evaluation's (eval_core.hpp).  Points per lane are autotuned over
 // {1, 2, 4, 8}; the launch sequence (counter reset, persistent kernel,
 // reduction, per-molecule integrals) is captured once into a hipGraph and
 // replayed by every inca_batch_evaluate.  The counter is cleared by a kernel
@@ -54,6 +58,13 @@ struct inca_batch {
   std::vector<int32_t> stride;
   std::vector<std::vector<std::array<int64_t, 3>>> segs;  // (first, count, class key)
   int64_t Gtot = 0, Qdoubles = 0;
+  // molecules evaluated by shell-quartet groups (their headers take the
+  // place of records, their coefficients sit in one pool)
+  std::vector<int> grouped;
+  std::vector<int64_t> pool_off;
+  int64_t pool_total = 0;
+  double *d_pool = nullptr;
+  float *d_pool32 = nullptr;
   // device arena
   double *d_q = nullptr, *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
   double *d_w = nullptr, *d_wvee = nullptr, *d_ivec = nullptr, *d_res = nullptr;
@@ -90,8 +101,9 @@ namespace inca {
 namespace {
 
 // int64 words per item: mol, tile, qoff, n, stride, partial, local q, and
-// the precision of its quartets (0: FP64, 1: the FP32 range of a molecule
-// prepared for INCA_MIXED)
+// the kind of its quartets: bit 0 the precision (0: FP64, 1: the FP32 range
+// of a molecule prepared for INCA_MIXED), bit 1 set for shell-quartet
+// groups (headers of kGStride doubles, coefficients in the batch's pool)
 constexpr int kItem = 8;
 constexpr int kMol = 6;   // int64 words per molecule: g_off, G, part_off, nch, mask off, tiles
 constexpr int64_t kMaxChunk = 4096;  // quartets per work item at most
@@ -115,6 +127,8 @@ __global__ void k_batch_reset(unsigned int *counter) {
 template <int P>
 __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ items,
                                                    int64_t nitems, const double *__restrict__ q,
+                                                   const double *__restrict__ pool,
+                                                   const float *__restrict__ pool32,
                                                    const double *__restrict__ gx,
                                                    const double *__restrict__ gy,
                                                    const double *__restrict__ gz,
@@ -155,7 +169,7 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
     const int64_t it = queues[2 * qf] + pos;
     const int64_t *__restrict__ d = items + it * kItem;
     const int64_t m = d[0], bt = d[1], qo = d[2], n = d[3], pb = d[5], lq = d[6];
-    const bool f32 = d[7] != 0;
+    const bool f32 = (d[7] & 1) != 0, grp = (d[7] & 2) != 0;
     const int stride = (int)d[4];
     const int64_t goff = mol[kMol * m], G = mol[kMol * m + 1];
     const int64_t moff = mol[kMol * m + 4], nt = mol[kMol * m + 5];
@@ -171,7 +185,15 @@ __global__ __launch_bounds__(kT) void k_batch_eval(const int64_t *__restrict__ i
       acc[k] = 0.0;
     }
     const double *__restrict__ q0 = q + qo - lq * stride;  // the molecule's first quartet
-    if (f32) {
+    if (grp && f32) {
+      for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
+        group_mixed<P>(q0 + l * stride, pool32, x, y, z, acc);
+      });
+    } else if (grp) {
+      for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
+        group_fp64<P>(q0 + l * stride, pool, x, y, z, acc, s_exp2);
+      });
+    } else if (f32) {
       for_live(mask ? mask + moff : nullptr, nt, bt * (kT / 64) + wv, lq, lq + n, [&](int64_t l) {
         const double *__restrict__ rec = q0 + l * stride;
         accumulate_mixed_any<P>(rec, load_header(rec), x, y, z, acc);
@@ -293,6 +315,16 @@ __global__ __launch_bounds__(kT) void k_fill_i32(int32_t *p, int64_t n, int32_t 
   if (t < n) p[t] = v;
 }
 
+// Group headers copied into the batch: their pool offsets (word 8) move by
+// the molecule's place in the batch's pool.
+__global__ __launch_bounds__(kT) void k_rebase_pool(double *__restrict__ gh, int64_t n,
+                                                    int64_t by) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g < n)
+    gh[g * kGStride + 8] =
+        __longlong_as_double((long long)(__double_as_longlong(gh[g * kGStride + 8]) + by));
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
 template <class T>
@@ -338,14 +370,20 @@ int build_items(inca_batch *b, int P) {
     for (const auto &sg : b->segs[m]) {
       const int lsum = (int)((sg[2] >> 24) & 0xffff);
       const int64_t f32 = (sg[2] >> 40) & 1;
+      // operations per point: 16 + 2 Lsum for a record; 14 + 2 M for a group
+      // of total degree Ls (Lx = Ly = Lz = Ls in its class bits)
+      const int ls = lsum / 3;
+      const double ops = b->grouped[m] ? 14.0 + (ls + 1) * (ls + 2) * (ls + 3) / 3.0
+                                       : 16.0 + 2.0 * lsum;
       for (int64_t s = 0; s < sg[1]; s += qchunk, ++c, ++chunk_id) {
         const int64_t n = std::min(qchunk, sg[1] - s);
         const int64_t qo = b->q_off[m] + (sg[0] + s) * b->stride[m];
         // FP32 terms cost about half of the FP64 ones
-        const double cost = (double)n * (16.0 + 2.0 * lsum) *
-                            (double)std::min(tile_pts, b->G[m]) * (f32 ? 0.5 : 1.0);
+        const double cost = (double)n * ops * (double)std::min(tile_pts, b->G[m]) *
+                            (f32 ? 0.5 : 1.0);
         order[chunk_id % kXcds].push_back((int64_t)ch.size());
-        ch.push_back({m, qo, n, part + c * b->G[m], sg[0] + s, f32, tiles, cost});
+        ch.push_back({m, qo, n, part + c * b->G[m], sg[0] + s, f32 | (b->grouped[m] ? 2 : 0),
+                      tiles, cost});
       }
     }
     b->nch[m] = c;
@@ -434,7 +472,8 @@ int enqueue(inca_batch *b, int P) {
   if (b->nitems > 0) {
 #define INCA_BEVAL(PP)                                                                        \
   hipLaunchKernelGGL(k_batch_eval<PP>, dim3(b->nblocks), dim3(kT), 0, st, b->d_items,         \
-                     b->nitems, b->d_q, b->d_gx, b->d_gy, b->d_gz, b->d_mol,                   \
+                     b->nitems, b->d_q, b->d_pool, b->d_pool32, b->d_gx, b->d_gy, b->d_gz,     \
+                     b->d_mol,                                                                 \
                      b->cut ? b->d_mask : nullptr, b->d_queues, b->d_partial, b->d_counter)
     switch (P) {
       case 1: INCA_BEVAL(1); break;
@@ -518,11 +557,19 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   b->ntile.assign(nmol, 0);
   b->nword.assign(nmol, 0);
   b->moff.assign(nmol, 0);
+  b->grouped.assign(nmol, 0);
+  b->pool_off.assign(nmol, 0);
+  b->pool_total = 0;
   b->cut = true;
   int64_t ncent = 0;
   for (int m = 0; m < nmol; ++m) {
     const inca_ctx *c = mols[m];
     if (!c) return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": null context");
+    // shell-quartet groups where the context has them, else records (a
+    // mixed preparation's record split is made now)
+    b->grouped[m] = c->grouping && c->ng > 0;
+    if (!b->grouped[m] && c->device == b->device && split_records(mols[m]))
+      return bfail(b, INCA_EHIP, "molecule " + std::to_string(m) + ": " + c->err);
     if (c->device != b->device)
       return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + " is on another device");
     if (c->grid_kind == GRID_NONE)
@@ -532,10 +579,14 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     b->G[m] = c->G;
     b->g_off[m] = b->Gtot;
     b->Gtot += c->G;
-    b->nq[m] = c->q2;
-    b->stride[m] = c->qstride;
+    b->nq[m] = b->grouped[m] ? c->ng : c->q2;
+    b->stride[m] = b->grouped[m] ? kGStride : c->qstride;
     b->q_off[m] = b->Qdoubles;
-    b->Qdoubles += c->q2 * c->qstride;
+    b->Qdoubles += b->nq[m] * b->stride[m];
+    if (b->grouped[m]) {
+      b->pool_off[m] = b->pool_total;
+      b->pool_total += c->gpool_n;
+    }
     b->cut = b->cut && c->cut_log2 > 0.0;
     b->mode[m] = c->grid_kind == GRID_RADIAL ? 0 : (c->grid_kind == GRID_MULTI ? 1 : 2);
     b->c_off[m] = ncent;
@@ -549,6 +600,13 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
   for (double **a : garr)
     if ((cap = 0, rc = bensure(b, *a, cap, gt))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_q, cap, qd))) return rc;
+  if (b->pool_total > 0) {
+    bool any32 = false;
+    for (int m = 0; m < nmol; ++m)
+      any32 = any32 || (b->grouped[m] && mols[m]->prep_precision == INCA_MIXED);
+    if ((cap = 0, rc = bensure(b, b->d_pool, cap, (size_t)b->pool_total))) return rc;
+    if (any32 && (cap = 0, rc = bensure(b, b->d_pool32, cap, (size_t)b->pool_total))) return rc;
+  }
   if ((cap = 0, rc = bensure(b, b->d_res, cap, 2 * (size_t)nmol))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_mol, cap, kMol * (size_t)nmol))) return rc;
   if ((cap = 0, rc = bensure(b, b->d_pmol, cap, gt))) return rc;
@@ -604,11 +662,13 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     B_HIP(tmp.get(d_nruns, 1));
     for (int m = 0; m < nmol; ++m) {
       const inca_ctx *c = mols[m];
-      const int64_t n = c->q2;
+      const int64_t n = b->nq[m];
       if (n == 0) continue;
-      const int64_t nfp64 = c->prep_precision == INCA_MIXED ? c->q_fp64 : n;
-      hipLaunchKernelGGL(k_class_keys, dim3(nblk(n)), dim3(kT), 0, st, c->d_q, n, c->qstride,
-                         nfp64, (uint64_t)m, d_kin + qstart[m], d_vin + qstart[m]);
+      const int64_t nfp64 = c->prep_precision != INCA_MIXED ? n
+                            : (b->grouped[m] ? c->ng_fp64 : c->q_fp64);
+      hipLaunchKernelGGL(k_class_keys, dim3(nblk(n)), dim3(kT), 0, st,
+                         b->grouped[m] ? c->d_gh : c->d_q, n, b->stride[m], nfp64, (uint64_t)m,
+                         d_kin + qstart[m], d_vin + qstart[m]);
     }
     B_HIP(hipGetLastError());
     unsigned int end_bit = kKeyMolShift + 1;
@@ -626,10 +686,20 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
                                      d_cnt, d_nruns, st));
     for (int m = 0; m < nmol; ++m) {
       const inca_ctx *c = mols[m];
-      const int64_t n = c->q2;
+      const int64_t n = b->nq[m];
       if (n == 0) continue;
-      hipLaunchKernelGGL(k_batch_gather, dim3(nblk(n * c->qstride)), dim3(kT), 0, st, c->d_q,
-                         d_vout + qstart[m], n, c->qstride, b->d_q + b->q_off[m]);
+      hipLaunchKernelGGL(k_batch_gather, dim3(nblk(n * b->stride[m])), dim3(kT), 0, st,
+                         b->grouped[m] ? c->d_gh : c->d_q, d_vout + qstart[m], n, b->stride[m],
+                         b->d_q + b->q_off[m]);
+      if (b->grouped[m]) {
+        hipLaunchKernelGGL(k_rebase_pool, dim3(nblk(n)), dim3(kT), 0, st, b->d_q + b->q_off[m], n,
+                           b->pool_off[m]);
+        B_HIP(hipMemcpyAsync(b->d_pool + b->pool_off[m], c->d_gpool,
+                             (size_t)c->gpool_n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (c->prep_precision == INCA_MIXED && c->d_gpool32)
+          B_HIP(hipMemcpyAsync(b->d_pool32 + b->pool_off[m], c->d_gpool32,
+                               (size_t)c->gpool_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      }
     }
     B_HIP(hipGetLastError());
     unsigned int nruns = 0;
@@ -688,7 +758,7 @@ int inca_batch_destroy(inca_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void *bufs[] = {b->d_q,     b->d_gx,    b->d_gy,   b->d_gz,    b->d_w,
+  void *bufs[] = {b->d_q,     b->d_pool,  b->d_pool32, b->d_gx,  b->d_gy,   b->d_gz,    b->d_w,
                   b->d_wvee,  b->d_ivec,  b->d_res,  b->d_partial, b->d_items,
                   b->d_mol,   b->d_pmol,  b->d_counter, b->d_pmap, b->d_cent,
                   b->d_tb,    b->d_mask,  b->d_queues, b->d_chunks};

@@ -24,6 +24,10 @@ constexpr int kNpMax = kLmaxAxis + 1;
 // k_ray_rho, 0 before), then the coefficients A*Cx(0..Lx), Cy(0..Ly),
 // Cz(0..Lz) (C(0) = highest power).
 constexpr int kQHead = 8;
+// Shell-quartet group header (group.hip): words 0-7 as a prepared record's,
+// [8] coefficient pool offset, [9] coefficient count, [10] member count,
+// [11] FP64 operations per grid point.
+constexpr int kGStride = 12;
 
 enum GridKind { GRID_NONE = 0, GRID_MULTI = 1, GRID_RADIAL = 2, GRID_CUBE = 3, GRID_POINTS = 4 };
 
@@ -54,6 +58,7 @@ struct inca_ctx {
   int nprim = 0;
   int normalize = 1;
   int lmax_axis = 0;  // 4 * max(t) over primitives
+  int lmax_prim = 0;  // max(t + m + n) over primitives
   double *d_prim = nullptr;  // SoA: alpha | x | y | z | norm  (5*nprim)
   int32_t *d_tmn = nullptr;  // axis-major t: tmn[ax*nprim + i]
   inca::PairEntry *d_pair = nullptr;
@@ -99,6 +104,7 @@ struct inca_ctx {
   int64_t q_fp64 = 0;         // mixed mode: leading records evaluated in FP64
   double mixed_tail = 0.02;   // FP32 tail budget: sum of its L1 bounds / |integral|
   double mixed_tau = 0.0;     // L1-bound threshold chosen for the split
+  int rec_split_pending = 0;  // mixed preparation whose record split is not made yet
   double term_integral = 0.0; // sum of the prepared terms' integrals over all space
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   int poly_exact = 0;         // 1: exact axis polynomials instead of the fits
@@ -113,6 +119,25 @@ struct inca_ctx {
   size_t qtmp_cap = 0;
   float *d_qf = nullptr;      // FP32 copy for the mixed kernel
   size_t qf_cap = 0;
+  // shell-quartet groups of the prepared records (group.hip): headers of
+  // kGStride doubles (FP64 range first), one coefficient pool, its FP32 copy
+  int grouping = 1;            // evaluate by groups where the grid loop allows
+  int64_t ng = 0, ng_fp64 = 0; // groups, of them evaluated in FP64
+  double *d_gh = nullptr;
+  size_t gh_cap = 0;
+  double *d_gpool = nullptr;
+  size_t gpool_cap = 0;
+  float *d_gpool32 = nullptr;
+  size_t gpool32_cap = 0;
+  int64_t gpool_n = 0;
+  double gflop_fp64 = 0.0, gflop_fp32 = 0.0;  // FP64 operations per point, summed per range
+  double flops_executed = 0.0;                // of the last grid loop (group evaluation)
+  int grouped_last = 0;                       // the last grid loop ran by groups
+  int gtile_p[2] = {4, 4};                    // points per lane of the group kernels
+  double *d_wflop = nullptr;                  // per 64-group word: operations kept
+  size_t wflop_cap = 0;
+  double *d_flops = nullptr;                  // their sum over the last grid loop
+  size_t flops_cap = 0;
 
   // scratch
   uint8_t *d_flag = nullptr;
@@ -224,6 +249,10 @@ int grid_smearing(inca_ctx *ctx, double u_max, int n_bins, double sigma,
 int build_pair_table(inca_ctx *ctx);
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision,
                      int64_t *counts);
+// the records' FP64/FP32 split of a mixed preparation, made on first need
+int split_records(inca_ctx *ctx);
+// group.hip
+int build_groups(inca_ctx *ctx, int precision);
 
 // autogrid.hip
 double bragg_slater_bohr(int z);

@@ -71,6 +71,53 @@ __global__ __launch_bounds__(kT) void k_eval_fp64(const double *__restrict__ q, 
   }
 }
 
+// Shell-quartet groups (group.hip) with the mapping of k_eval_fp64: a
+// group's exponential and polynomial per point in place of its records'
+// terms.  MIXED: the FP32 range (group_mixed).
+template <int P, bool CUT, bool MIXED>
+__global__ __launch_bounds__(kT) void k_geval(const double *__restrict__ gh, int64_t ng,
+                                              int64_t gchunk, const double *__restrict__ pool,
+                                              const float *__restrict__ pool32,
+                                              const double *__restrict__ gx,
+                                              const double *__restrict__ gy,
+                                              const double *__restrict__ gz, int64_t G,
+                                              const double *__restrict__ tb,
+                                              const unsigned long long *__restrict__ mask,
+                                              int64_t ntile, int64_t tiles, int64_t nch, int xcd,
+                                              double *__restrict__ partial) {
+  const BlockTask task = block_task(tiles, xcd);
+  if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
+  const int64_t tile = task.tile_block * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const bool empty = !(tb[tile * kTB + 5] > 0.0);
+  __shared__ double s_exp2[kExpTable];
+  if (!MIXED) stage_exp_table(s_exp2);
+  double x[P], y[P], z[P], acc[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * 64;
+    const bool ok = g < G;
+    x[k] = ok ? gx[g] : 0.0;
+    y[k] = ok ? gy[g] : 0.0;
+    z[k] = ok ? gz[g] : 0.0;
+    acc[k] = 0.0;
+  }
+  const int64_t q0 = task.chunk * gchunk;
+  const int64_t q1 = empty ? q0 : (q0 + gchunk < ng ? q0 + gchunk : ng);
+  for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t gi) {
+    if (MIXED)
+      group_mixed<P>(gh + gi * kGStride, pool32, x, y, z, acc);
+    else
+      group_fp64<P>(gh + gi * kGStride, pool, x, y, z, acc, s_exp2);
+  });
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int64_t g = base + (int64_t)k * 64;
+    if (g < G) partial[task.chunk * G + g] = acc[k];
+  }
+}
+
 // Ray tiles: lane `lane` of tile t holds the points idx[t][k][lane], k < P
 // (-1: none), equally spaced by h w from its first point o (or o alone):
 // one direction w = o / |o| at consecutive radii of a radial scan (kind 1),
@@ -843,6 +890,75 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   return 0;
 }
 
+// Launch the group kernel over groups [g0, g0 + n) (the headers at
+// ctx->d_gh) in `nch` chunks of `gch` groups, P points per lane; partial
+// sums to partial[chunk][g].  With the cutoff, the skip mask also counts the
+// FP64 operations the launch executes (header word 11) into ctx->d_flops.
+int launch_geval(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, int64_t gch,
+                 int64_t nch, double *partial) {
+  if (P != 1 && P != 2 && P != 6 && P != 8) P = 4;
+  const int64_t G = ctx->G;
+  const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  const int xcd = nch >= kXcds;
+  const int64_t nblocks = tiles * (xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch);
+  dim3 grid((unsigned)nblocks);
+  hipStream_t st = ctx->stream;
+  const bool cut = ctx->cut_log2 > 0.0;
+  const int64_t ntile = tiles * (kT / 64);
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_tb, ctx->tb_cap, (size_t)ntile * kTB))) return rc;
+  if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
+  if ((rc = ensure(ctx, ctx->d_flops, ctx->flops_cap, 1))) return rc;
+  if (cut) {
+    const int mode = ctx->grid_kind == GRID_RADIAL ? 0 : (ctx->grid_kind == GRID_MULTI ? 1 : 2);
+    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
+                       ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
+                       ctx->d_tb, (const int32_t *)nullptr);
+    const int64_t nwords = (n + 63) / 64;
+    const size_t nw = (size_t)(nwords > 0 ? nwords : 1);
+    if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, nw * ntile))) return rc;
+    if ((rc = ensure(ctx, ctx->d_wcount, ctx->wcount_cap, nw))) return rc;
+    if ((rc = ensure(ctx, ctx->d_wflop, ctx->wflop_cap, nw))) return rc;
+    if (nwords > 0) {
+      hipLaunchKernelGGL(k_skip_mask, dim3((unsigned)((nwords + 3) / 4)), dim3(kT), 0, st, gh,
+                         kGStride, n, ctx->d_tb, ntile, nwords, ctx->d_mask, ctx->d_wcount, 11,
+                         ctx->d_wflop);
+      hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(kT), 0, st, ctx->d_wcount, nwords, ctx->d_evals);
+      hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(kT), 0, st, ctx->d_wflop, nwords, ctx->d_flops);
+    }
+  } else {
+    hipLaunchKernelGGL(k_fill_tiles, dim3((unsigned)((ntile + kT - 1) / kT)), dim3(kT), 0, st,
+                       ntile, ctx->d_tb);
+  }
+  INCA_HIP(hipGetLastError());
+#define INCA_GEVAL2(PP, CC)                                                                    \
+  if (mixed)                                                                                   \
+    hipLaunchKernelGGL((k_geval<PP, CC, true>), grid, dim3(kT), 0, st, gh, n, gch,             \
+                       ctx->d_gpool, ctx->d_gpool32, ctx->d_gx, ctx->d_gy, ctx->d_gz, G,       \
+                       ctx->d_tb, ctx->d_mask, ntile, tiles, nch, xcd, partial);               \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_geval<PP, CC, false>), grid, dim3(kT), 0, st, gh, n, gch,            \
+                       ctx->d_gpool, ctx->d_gpool32, ctx->d_gx, ctx->d_gy, ctx->d_gz, G,       \
+                       ctx->d_tb, ctx->d_mask, ntile, tiles, nch, xcd, partial);
+#define INCA_GEVAL(PP)     \
+  if (cut) {               \
+    INCA_GEVAL2(PP, true)  \
+  } else {                 \
+    INCA_GEVAL2(PP, false) \
+  }
+  switch (P) {
+    case 1: INCA_GEVAL(1) break;
+    case 2: INCA_GEVAL(2) break;
+    case 6: INCA_GEVAL(6) break;
+    case 8: INCA_GEVAL(8) break;
+    default: INCA_GEVAL(4) break;
+  }
+#undef INCA_GEVAL
+#undef INCA_GEVAL2
+  INCA_HIP(hipGetLastError());
+  return 0;
+}
+
 // Chunking of n quartets into `share` of a budget of (tile, chunk) blocks.
 // Small chunks keep the quartet stream of the blocks resident on one XCD
 // inside its L2 (measured on the radial-plot benchmark: 8192 blocks 689 ms,
@@ -1002,12 +1118,39 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
     return 0;
   }
   if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
+  if ((rc = ensure(ctx, ctx->d_flops, ctx->flops_cap, 1))) return rc;
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ctx->d_evals);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kT), 0, st, ctx->d_flops, (int64_t)1);
   const bool contracted = ctx->grid_kind == GRID_CUBE && ctx->cube_contract;
+  // shell-quartet groups where they exist (every grid the contraction does
+  // not take); FP64 groups first, then the FP32 range of a mixed preparation
+  const bool grouped = !contracted && ctx->grouping && ctx->ng > 0;
+  ctx->grouped_last = grouped;
+  if (precision == INCA_MIXED && !grouped && (rc = split_records(ctx))) return rc;
   int64_t nchunks = 0;
   bool rays_used = false;  // a range ran on the ray lists (duplicate points skipped)
   if (contracted) {
     if ((rc = cube_contract(ctx, precision, &nchunks))) return rc;
+  } else if (grouped) {
+    const int64_t ng = ctx->ng;
+    const int64_t n64 = (precision == INCA_MIXED) ? ctx->ng_fp64 : ng;
+    const int64_t nr[2] = {n64, ng - n64};
+    int64_t nch[2] = {0, 0}, gch[2] = {1, 1};
+    for (int r = 0; r < 2; ++r) {
+      if (nr[r] == 0) continue;
+      const int P = ctx->gtile_p[r];
+      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      chunking(nr[r], tiles, (double)nr[r] / (double)ng, &gch[r], &nch[r]);
+    }
+    nchunks = nch[0] + nch[1];
+    if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nchunks * G)))) return rc;
+    if (nch[0] > 0 && (rc = launch_geval(ctx, false, ctx->gtile_p[0], ctx->d_gh, nr[0], gch[0],
+                                         nch[0], ctx->d_partial)))
+      return rc;
+    if (nch[1] > 0 &&
+        (rc = launch_geval(ctx, true, ctx->gtile_p[1], ctx->d_gh + n64 * kGStride, nr[1], gch[1],
+                           nch[1], ctx->d_partial + nch[0] * G)))
+      return rc;
   } else {
     // ranges: [0, n64) in FP64, [n64, nq) in the mixed kernel
     const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
@@ -1075,21 +1218,88 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   INCA_HIP(hipEventRecord(ctx->ev[3], st));
   ctx->have_ivec = 1;
   ctx->evals_reference = (int64_t)nq * G;
-  ctx->evals_executed = ctx->evals_reference;
+  // executed evaluations: (record, point) pairs, or (group, point) pairs
+  // when the loop ran by groups
+  ctx->evals_executed = grouped ? ctx->ng * G : ctx->evals_reference;
+  ctx->flops_executed = 0.0;
+  if (grouped)
+    ctx->flops_executed = (double)G * (ctx->gflop_fp64 + ctx->gflop_fp32);
   if (ctx->cut_log2 > 0.0 && !contracted) {
     unsigned long long ex = 0;
+    double fl = 0.0;
     INCA_HIP(hipMemcpyAsync(&ex, ctx->d_evals, sizeof(ex), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipMemcpyAsync(&fl, ctx->d_flops, sizeof(fl), hipMemcpyDeviceToHost, st));
     INCA_HIP(hipStreamSynchronize(st));
     ctx->evals_executed = (int64_t)ex;
+    if (grouped) ctx->flops_executed = fl;
   }
   return 0;
 }
 
 // Time every points-per-lane choice for both kernels on a slice of the
 // prepared quartets of the current grid and keep the fastest.
+// The group kernels' points per lane, on an evenly strided sample of the
+// groups of each range (FP64 groups, and the FP32 ones of a mixed
+// preparation), timed like the record kernels below.
+int autotune_groups(inca_ctx *ctx, int32_t *best) {
+  const int64_t G = ctx->G;
+  const int cand[] = {1, 2, 4, 6, 8};
+  hipEvent_t a, b;
+  INCA_HIP(hipEventCreate(&a));
+  INCA_HIP(hipEventCreate(&b));
+  int rc = 0;
+  for (int kind = 0; kind < 2 && !rc; ++kind) {
+    const int64_t g0 = kind == 0 ? 0 : ctx->ng_fp64;
+    const int64_t nr = kind == 0 ? ctx->ng_fp64 : ctx->ng - ctx->ng_fp64;
+    if (nr == 0 || (kind == 1 && !ctx->d_gpool32)) {
+      if (best) best[kind] = ctx->gtile_p[kind];
+      continue;
+    }
+    int64_t n = ((int64_t)1 << 33) / (G > 0 ? G : 1);
+    n = n < 1024 ? 1024 : (n > (1 << 20) ? (1 << 20) : n);
+    if (n > nr) n = nr;
+    if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)n * kGStride))) break;
+    hipLaunchKernelGGL(k_gather_strided, dim3((unsigned)((n * kGStride + kT - 1) / kT)), dim3(kT),
+                       0, ctx->stream, ctx->d_gh + g0 * kGStride, nr, n, kGStride, ctx->d_qtmp);
+    INCA_HIP(hipGetLastError());
+    float best_ms = 1e30f;
+    int best_p = 4;
+    for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])) && !rc; ++c) {
+      const int P = cand[c];
+      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      int64_t qch, nch;
+      chunking(nr, tiles, (double)nr / (double)ctx->ng, &qch, &nch);
+      nch = (n + qch - 1) / qch;
+      if ((rc = ensure(ctx, ctx->d_partial, ctx->partial_cap, (size_t)(nch * G)))) break;
+      float ms = 1e30f;
+      for (int rep = 0; rep < 3 && !rc; ++rep) {
+        float t = 0.f;
+        INCA_HIP(hipEventRecord(a, ctx->stream));
+        rc = launch_geval(ctx, kind == 1, P, ctx->d_qtmp, n, qch, nch, ctx->d_partial);
+        INCA_HIP(hipEventRecord(b, ctx->stream));
+        INCA_HIP(hipEventSynchronize(b));
+        INCA_HIP(hipEventElapsedTime(&t, a, b));
+        if (rep > 0 && t < ms) ms = t;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_p = P;
+      }
+    }
+    ctx->gtile_p[kind] = best_p;
+    if (best) best[kind] = best_p;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->have_ivec = 0;
+  return rc;
+}
+
 int autotune(inca_ctx *ctx, int32_t *best) {
   const int64_t G = ctx->G, nq = ctx->q2;
   if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
+  if (ctx->grouping && ctx->ng > 0 && !(ctx->grid_kind == GRID_CUBE && ctx->cube_contract))
+    return autotune_groups(ctx, best);
   const int cand[] = {1, 2, 4, 6, 8, 10};
   // an evenly strided sample of the prepared quartets (file order groups
   // quartets by primitive, and with the cutoff their cost depends on where
@@ -1240,7 +1450,9 @@ int evaluate_noalloc(inca_ctx *ctx, int precision, double *ivec, hipStream_t st)
   // ranges: [0, n64) FP64, [n64, nq) FP32 (mixed); points per lane from the
   // context's tiles (a ray-tile choice of the FP32 range runs 8 per lane)
   auto valid_p = [](int p) { return p == 1 || p == 2 || p == 4 || p == 6 || p == 8 || p == 10; };
-  const int64_t n64 = (precision == INCA_MIXED) ? ctx->q_fp64 : nq;
+  // a mixed preparation whose record split is not made yet (it allocates)
+  // runs every record in FP64
+  const int64_t n64 = (precision == INCA_MIXED && !ctx->rec_split_pending) ? ctx->q_fp64 : nq;
   const int64_t nr[2] = {n64, nq - n64};
   const int Pk[2] = {valid_p(ctx->tile_p[0]) ? ctx->tile_p[0] : 4,
                      valid_p(ctx->tile_p[1]) ? ctx->tile_p[1] : 8};

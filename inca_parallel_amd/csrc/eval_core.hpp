@@ -476,6 +476,203 @@ __device__ __forceinline__ void accumulate_mixed_any(const double *__restrict__ 
   }
 }
 
+// ---- shell-quartet groups (group.hip) ----
+// A group's term at rp is exp(-|rp|^2) P(rp) with P(x, y, z) = sum K_abc
+// x^a y^b z^c over a <= Lx, b <= Ly, c <= Lz, a + b + c <= Ls, evaluated by
+// nested Horner: for a = Lx..0 the polynomial in (y, z) of the x^a terms,
+// itself Horner in y over polynomials in z, the coefficients in that order
+// in the group's pool (highest powers first).  The coefficients are
+// wave-uniform (scalar loads); every loop has a uniform trip count.
+struct GHeader {
+  double sqe, ax, ay, az;
+  int Lx, Ly, Lz, Ls;
+  int64_t off;
+};
+
+__device__ __forceinline__ GHeader load_gheader(const double *__restrict__ g) {
+  GHeader h;
+  h.sqe = g[0];
+  h.ax = g[1];
+  h.ay = g[2];
+  h.az = g[3];
+  const uint64_t cls = (uint64_t)__double_as_longlong(g[4]);
+  h.Lx = (int)(cls & 0xff);
+  h.Ly = (int)((cls >> 8) & 0xff);
+  h.Lz = (int)((cls >> 16) & 0xff);
+  h.Ls = (int)((cls >> 32) & 0xff);
+  h.off = (int64_t)__double_as_longlong(g[8]);
+  return h;
+}
+
+__device__ __forceinline__ double mad(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float mad(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <int P, typename T, typename C>
+__device__ __forceinline__ void group_poly(const C *__restrict__ c, const GHeader &h,
+                                           const T (&px)[P], const T (&py)[P], const T (&pz)[P],
+                                           T (&vx)[P]) {
+  for (int a = h.Lx; a >= 0; --a) {
+    const int bm = min(h.Ly, h.Ls - a);
+    T vy[P];
+    for (int b = bm; b >= 0; --b) {
+      const int cm = min(h.Lz, h.Ls - a - b);
+      T vz[P];
+      if (cm == 0) {
+        const T c0 = (T)c[0];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vz[k] = c0;
+        c += 1;
+      } else {
+        const T c0 = (T)c[0], c1 = (T)c[1];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vz[k] = mad(pz[k], c0, c1);
+        for (int j = 2; j <= cm; ++j) {
+          const T cj = (T)c[j];
+#pragma unroll
+          for (int k = 0; k < P; ++k) vz[k] = mad(vz[k], pz[k], cj);
+        }
+        c += cm + 1;
+      }
+      if (b == bm) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) vy[k] = vz[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < P; ++k) vy[k] = mad(vy[k], py[k], vz[k]);
+      }
+    }
+    if (a == h.Lx) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) vx[k] = vy[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < P; ++k) vx[k] = mad(vx[k], px[k], vy[k]);
+    }
+  }
+}
+
+// The same polynomial with every monomial of total degree <= LS present
+// (Lx = Ly = Lz = Ls = LS, as the group builder lays out groups of total
+// degree up to 8): the loops unroll, the coefficient offsets are constants
+// and the loads merge into wide scalar loads.
+template <int P, int LS, typename T, typename C>
+__device__ __forceinline__ void group_poly_fixed(const C *__restrict__ c, const T (&px)[P],
+                                                 const T (&py)[P], const T (&pz)[P],
+                                                 T (&vx)[P]) {
+  int i = 0;
+#pragma unroll
+  for (int a = LS; a >= 0; --a) {
+    T vy[P];
+#pragma unroll
+    for (int b = LS - a; b >= 0; --b) {
+      const int cm = LS - a - b;
+      T vz[P];
+      if (cm == 0) {
+        const T c0 = (T)c[i];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vz[k] = c0;
+        i += 1;
+      } else {
+        const T c0 = (T)c[i], c1 = (T)c[i + 1];
+#pragma unroll
+        for (int k = 0; k < P; ++k) vz[k] = mad(pz[k], c0, c1);
+#pragma unroll
+        for (int j = 2; j <= cm; ++j) {
+          const T cj = (T)c[i + j];
+#pragma unroll
+          for (int k = 0; k < P; ++k) vz[k] = mad(vz[k], pz[k], cj);
+        }
+        i += cm + 1;
+      }
+      if (b == LS - a) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) vy[k] = vz[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < P; ++k) vy[k] = mad(vy[k], py[k], vz[k]);
+      }
+    }
+    if (a == LS) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) vx[k] = vy[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < P; ++k) vx[k] = mad(vx[k], px[k], vy[k]);
+    }
+  }
+}
+
+// Dispatch on the group's layout: dense total degree 0..6 unrolled (7 and 8
+// unrolled would hold 120-165 coefficients and set the register count of
+// the whole kernel), any other layout by the general loops.
+template <int P, typename T, typename C>
+__device__ __forceinline__ void group_poly_any(const C *__restrict__ c, const GHeader &h,
+                                               const T (&px)[P], const T (&py)[P],
+                                               const T (&pz)[P], T (&v)[P]) {
+  const bool dense = h.Lx == h.Ls && h.Ly == h.Ls && h.Lz == h.Ls;
+  switch (dense ? h.Ls : -1) {
+    case 0:
+#pragma unroll
+      for (int k = 0; k < P; ++k) v[k] = (T)c[0];
+      break;
+    case 1: group_poly_fixed<P, 1, T, C>(c, px, py, pz, v); break;
+    case 2: group_poly_fixed<P, 2, T, C>(c, px, py, pz, v); break;
+    case 3: group_poly_fixed<P, 3, T, C>(c, px, py, pz, v); break;
+    case 4: group_poly_fixed<P, 4, T, C>(c, px, py, pz, v); break;
+    case 5: group_poly_fixed<P, 5, T, C>(c, px, py, pz, v); break;
+    case 6: group_poly_fixed<P, 6, T, C>(c, px, py, pz, v); break;
+    default: group_poly<P, T, C>(c, h, px, py, pz, v); break;
+  }
+}
+
+// One group at the P points of a lane, FP64: acc[k] += exp(-|rp|^2) P(rp).
+template <int P>
+__device__ __forceinline__ void group_fp64(const double *__restrict__ gh,
+                                           const double *__restrict__ pool, const double (&x)[P],
+                                           const double (&y)[P], const double (&z)[P],
+                                           double (&acc)[P], const double *__restrict__ tbl) {
+  const GHeader h = load_gheader(gh);
+  double px[P], py[P], pz[P], v[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    px[k] = fma(h.sqe, x[k], h.ax);
+    py[k] = fma(h.sqe, y[k], h.ay);
+    pz[k] = fma(h.sqe, z[k], h.az);
+  }
+  group_poly_any<P, double, double>(pool + h.off, h, px, py, pz, v);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const double s = fma(px[k], px[k], fma(py[k], py[k], pz[k] * pz[k]));
+    acc[k] = fma(exp_neg(s, tbl), v[k], acc[k]);
+  }
+}
+
+// The same in mixed precision: rp in FP64, then FP32 (hardware exp2, the
+// FP32 copy of the coefficients), each term added to the FP64 accumulator.
+template <int P>
+__device__ __forceinline__ void group_mixed(const double *__restrict__ gh,
+                                            const float *__restrict__ pool32, const double (&x)[P],
+                                            const double (&y)[P], const double (&z)[P],
+                                            double (&acc)[P]) {
+  const float kNL2eHi = -1.44269502162933349609375f;
+  const float kNL2eLo = -1.925963033500011e-08f;
+  const GHeader h = load_gheader(gh);
+  float px[P], py[P], pz[P], v[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    px[k] = (float)fma(h.sqe, x[k], h.ax);
+    py[k] = (float)fma(h.sqe, y[k], h.ay);
+    pz[k] = (float)fma(h.sqe, z[k], h.az);
+  }
+  group_poly_any<P, float, float>(pool32 + h.off, h, px, py, pz, v);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const float s = fmaf(px[k], px[k], fmaf(py[k], py[k], pz[k] * pz[k]));
+    const float e = __builtin_amdgcn_exp2f(fmaf(s, kNL2eHi, s * kNL2eLo));
+    acc[k] += (double)(e * v[k]);
+  }
+}
+
 // XCD-aware block order.  The device dispatches workgroups to its 8 XCDs
 // round-robin by linear index, and each XCD has its own L2.  With `xcd` set,
 // block L runs on XCD L mod 8 and the blocks of XCD x take the chunks
@@ -614,11 +811,16 @@ __device__ __forceinline__ bool tile_far(const QHeader &h, double cx, double cy,
 //
 // wcount[w] (optional) receives the evaluations the mask keeps for the word:
 // the sum over tiles of kept quartets x the tile's points.
+//
+// wflop[w] (optional, with the operations per point in record word `fword`):
+// the sum over the word's kept (record, tile) pairs of points x operations.
 __global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, int qstride,
                                                   int64_t nq, const double *__restrict__ tb,
                                                   int64_t ntile, int64_t nwords,
                                                   unsigned long long *__restrict__ mask,
-                                                  unsigned long long *__restrict__ wcount) {
+                                                  unsigned long long *__restrict__ wcount,
+                                                  int fword = -1,
+                                                  double *__restrict__ wflop = nullptr) {
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
   if (w >= nwords) return;  // uniform per wave
@@ -628,6 +830,7 @@ __global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, 
   QHeader h{};
   if (valid) h = load_header(q + qi * qstride);
   unsigned long long kept = 0;
+  double mine = 0.0;  // points of the tiles this lane's record keeps
   for (int64_t t0 = 0; t0 < ntile; t0 += 64) {
     unsigned long long keep = 0;
     const int nt = (int)((ntile - t0) < 64 ? (ntile - t0) : 64);
@@ -638,10 +841,30 @@ __global__ __launch_bounds__(kT) void k_skip_mask(const double *__restrict__ q, 
       const unsigned long long m = __ballot(far);
       if (lane == j) keep = m;
       kept += (unsigned long long)__popcll(vbits & ~m) * (unsigned long long)npts;
+      if (valid && !far) mine += npts;
     }
     if (lane < nt) mask[w * ntile + t0 + lane] = keep;
   }
   if (wcount && lane == 0) wcount[w] = kept;
+  if (wflop) {
+    const double f = wave_sum(valid ? mine * q[qi * qstride + fword] : 0.0);
+    if (lane == 0) wflop[w] = f;
+  }
+}
+
+// Fixed-order sum of n doubles by one block, added to out[0].
+__global__ __launch_bounds__(kT) void k_sum_f64(const double *__restrict__ x, int64_t n,
+                                                double *__restrict__ out) {
+  __shared__ double red[kT];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kT) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] += red[0];
 }
 
 // Fixed-order sum of n counters by one block.

@@ -1,0 +1,453 @@
+// Shell-quartet groups of the prepared records (no reference counterpart;
+// the sum they evaluate is the grid loop of mod_intracule.f90:345-358).
+//
+// A prepared record's term is A exp(-|rp|^2) Vx(rp_x) Vy(rp_y) Vz(rp_z) with
+// rp = sqe r + sqe (P_ik - P_jl).  sqe and the offset depend only on the
+// exponents and centres of the four primitives, so the Cartesian components
+// of one shell quartet -- and any other records that happen to agree -- share
+// rp and the Gaussian, and their terms add into one:
+//   sum_q A_q exp(-|rp|^2) Vx_q Vy_q Vz_q = exp(-|rp|^2) sum_abc K_abc rpx^a rpy^b rpz^c,
+//   K_abc = sum_q (A Cx_q)[a] Cy_q[b] Cz_q[c],
+// one exponential and one three-variable polynomial per group and grid point
+// in place of one exponential, three axis polynomials and their product per
+// record and point.  A shell quartet of d functions holds up to 1296 records
+// and needs at most 165 coefficients.
+//
+// Building the groups (one pass after the pre-pass, deterministic):
+//   k_group_keys    a 32-bit hash of the bits of (sqe, offset) per record;
+//   radix sort      stable, so the records of a group keep their order;
+//   k_group_heads   a group starts wherever the four doubles change (a hash
+//                   collision only splits a group, never merges two);
+//   k_group_header  one thread per group: degrees, bound, cutoff radius;
+//   exclusive scan  of the coefficient counts: the coefficient pool offsets;
+//   k_group_coeffs  one wave per group, a lane per coefficient, the members
+//                   added in their record order.
+// Group header (kGStride doubles), words 0-7 as a prepared record's:
+//   [0] sqe, [1..3] offsets, [4] class Lx | Ly<<8 | Lz<<16 | axes<<24 |
+//   Ls<<32 (Ls the members' largest total degree, Lx = Ly = Lz = Ls),
+//   [5] B: the members' L1 bounds summed, [6] rcut: their largest cutoff
+//   radius, [7] rho (ray step), [8] pool offset, [9] coefficient count M,
+//   [10] member count, [11] FP64 operations per point, 14 + 2 M.
+// Coefficients: every monomial of total degree <= Ls, for a = Ls..0,
+// b = Ls - a..0, c = Ls - a - b..0 (nested Horner order, highest powers
+// first); (Ls + 1)(Ls + 2)(Ls + 3) / 6 of them.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "common.hpp"
+
+namespace inca {
+namespace {
+
+constexpr int kT = 256;
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+
+__global__ __launch_bounds__(kT) void k_group_keys(const double *__restrict__ q, int stride,
+                                                   int64_t n, uint32_t *__restrict__ key,
+                                                   uint32_t *__restrict__ val) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  const double *r = q + i * stride;
+  uint64_t h = 0x9e3779b97f4a7c15ull;
+  for (int w = 0; w < 4; ++w) h = mix64(h ^ (uint64_t)__double_as_longlong(r[w]));
+  key[i] = (uint32_t)(h >> 32);
+  val[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ bool same_gaussian(const double *a, const double *b) {
+  bool s = true;
+  for (int w = 0; w < 4; ++w) s = s && __double_as_longlong(a[w]) == __double_as_longlong(b[w]);
+  return s;
+}
+
+__global__ __launch_bounds__(kT) void k_group_heads(const double *__restrict__ q, int stride,
+                                                    const uint32_t *__restrict__ order, int64_t n,
+                                                    uint8_t *__restrict__ head) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i >= n) return;
+  head[i] = (uint8_t)(i == 0 || !same_gaussian(q + (int64_t)order[i] * stride,
+                                               q + (int64_t)order[i - 1] * stride));
+}
+
+__host__ __device__ __forceinline__ int monomials(int Lx, int Ly, int Lz, int Ls) {
+  int m = 0;
+  for (int a = Lx; a >= 0; --a) {
+    const int bm = Ly < Ls - a ? Ly : Ls - a;
+    for (int b = bm; b >= 0; --b) m += (Lz < Ls - a - b ? Lz : Ls - a - b) + 1;
+  }
+  return m;
+}
+
+__device__ __forceinline__ void record_degrees(const double *r, int &Lx, int &Ly, int &Lz) {
+  const unsigned cls = (unsigned)__double_as_longlong(r[4]);
+  Lx = cls & 0xff;
+  Ly = (cls >> 8) & 0xff;
+  Lz = (cls >> 16) & 0xff;
+}
+
+// Members of group g: order[first[g] .. first[g+1]) (the last group ends at n).
+__global__ __launch_bounds__(kT) void k_group_header(const double *__restrict__ q, int stride,
+                                                     const uint32_t *__restrict__ order,
+                                                     const int64_t *__restrict__ first, int64_t ng,
+                                                     int64_t n, double *__restrict__ gh,
+                                                     int64_t *__restrict__ mcount) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= ng) return;
+  const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
+  int Lx = 0, Ly = 0, Lz = 0, Ls = 0;
+  double B = 0.0, rcut = 0.0;
+  const double *r0 = q + (int64_t)order[m0] * stride;
+  for (int64_t m = m0; m < m1; ++m) {
+    const double *r = q + (int64_t)order[m] * stride;
+    int lx, ly, lz;
+    record_degrees(r, lx, ly, lz);
+    Lx = max(Lx, lx);
+    Ly = max(Ly, ly);
+    Lz = max(Lz, lz);
+    Ls = max(Ls, lx + ly + lz);
+    B += r[5];
+    rcut = fmax(rcut, r[6]);
+  }
+  // every monomial of total degree <= Ls is laid out (groups are built for
+  // total degrees up to 8, and the evaluation unrolls per degree)
+  Lx = Ly = Lz = Ls;
+  const int M = monomials(Lx, Ly, Lz, Ls);
+  const int axes = (Lx > 0) | ((Ly > 0) << 1) | ((Lz > 0) << 2);
+  double *o = gh + g * kGStride;
+  o[0] = r0[0];
+  o[1] = r0[1];
+  o[2] = r0[2];
+  o[3] = r0[3];
+  o[4] = __longlong_as_double((long long)((uint64_t)(Lx | (Ly << 8) | (Lz << 16) | (axes << 24)) |
+                                          ((uint64_t)Ls << 32)));
+  o[5] = B;
+  o[6] = rcut;
+  o[7] = r0[7];
+  o[8] = 0.0;  // pool offset, after the scan
+  o[9] = __longlong_as_double((long long)M);
+  o[10] = __longlong_as_double((long long)(m1 - m0));
+  o[11] = 14.0 + 2.0 * M;
+  mcount[g] = M;
+}
+
+__global__ __launch_bounds__(kT) void k_group_offsets(const int64_t *__restrict__ off, int64_t ng,
+                                                      double *__restrict__ gh) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g < ng) gh[g * kGStride + 8] = __longlong_as_double((long long)off[g]);
+}
+
+// One wave per group; lane j of pass p owns coefficient 64 p + j and adds
+// the members' products into it in member order: (A Cx[a] Cy[b]) Cz[c].
+// The members' records are staged in LDS kStage at a time by the whole
+// wave (their loads in flight together), then read from LDS; `lds` holds
+// kStage * stride doubles per wave.
+constexpr int kStage = 32;
+__global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
+                                                     const uint32_t *__restrict__ order,
+                                                     const int64_t *__restrict__ first, int64_t ng,
+                                                     int64_t n, const double *__restrict__ gh,
+                                                     double *__restrict__ pool) {
+  extern __shared__ double lds_all[];
+  const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (g >= ng) return;  // whole wave (no block barrier below)
+  double *lds = lds_all + (size_t)(threadIdx.x >> 6) * kStage * stride;
+  const int lane = threadIdx.x & 63;
+  const double *h = gh + g * kGStride;
+  const uint64_t cls = (uint64_t)__double_as_longlong(h[4]);
+  const int Lx = cls & 0xff, Ly = (cls >> 8) & 0xff, Lz = (cls >> 16) & 0xff;
+  const int Ls = (int)((cls >> 32) & 0xff);
+  const int M = (int)__double_as_longlong(h[9]);
+  double *K = pool + __double_as_longlong(h[8]);
+  const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
+  const int npass = (M + 63) / 64;
+  // this lane's (a, b, c) in the storage order, per pass (M <= 64 * 4 for
+  // the layouts built here: at most 165 coefficients up to total degree 8)
+  int la[4], lb[4], lc[4];
+  double acc[4];
+  for (int p = 0; p < 4; ++p) {
+    la[p] = -1;
+    lb[p] = lc[p] = 0;
+    acc[p] = 0.0;
+    const int idx = 64 * p + lane;
+    if (p >= npass || idx >= M) continue;
+    int at = 0;
+    for (int aa = Lx; aa >= 0 && la[p] < 0; --aa) {
+      const int bm = min(Ly, Ls - aa);
+      for (int bb = bm; bb >= 0; --bb) {
+        const int cm = min(Lz, Ls - aa - bb);
+        if (idx <= at + cm) {
+          la[p] = aa;
+          lb[p] = bb;
+          lc[p] = cm - (idx - at);
+          break;
+        }
+        at += cm + 1;
+      }
+    }
+  }
+  for (int64_t s0 = m0; s0 < m1; s0 += kStage) {
+    const int ns = (int)((m1 - s0) < kStage ? (m1 - s0) : kStage);
+    for (int t = lane; t < ns * stride; t += 64) {
+      const int r = t / stride;
+      lds[t] = q[(int64_t)order[s0 + r] * stride + (t - r * stride)];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int r = 0; r < ns; ++r) {
+      const double *row = lds + r * stride;
+      int lx, ly, lz;
+      record_degrees(row, lx, ly, lz);
+      const double *cx = row + kQHead;
+      const double *cy = cx + lx + 1;
+      const double *cz = cy + ly + 1;
+      for (int p = 0; p < npass; ++p) {
+        const int a = la[p], b = lb[p], c = lc[p];
+        if (a >= 0 && a <= lx && b <= ly && c <= lz)
+          acc[p] += (cx[lx - a] * cy[ly - b]) * cz[lz - c];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int p = 0; p < npass; ++p) {
+    const int idx = 64 * p + lane;
+    if (idx < M) K[idx] = acc[p];
+  }
+}
+
+// Operations per point summed over the FP64 range [0, ng64) and the rest,
+// in a fixed order (one block).
+__global__ __launch_bounds__(kT) void k_group_flops(const double *__restrict__ gh, int64_t ng,
+                                                    int64_t ng64, double *__restrict__ out) {
+  __shared__ double red[2][kT];
+  double s[2] = {0.0, 0.0};
+  for (int64_t g = threadIdx.x; g < ng; g += kT) s[g < ng64 ? 0 : 1] += gh[g * kGStride + 11];
+  red[0][threadIdx.x] = s[0];
+  red[1][threadIdx.x] = s[1];
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = red[0][0];
+    out[1] = red[1][0];
+  }
+}
+
+constexpr int kMagBins = 2100;  // ilogb in [-1075, 1024]
+constexpr int kMagOff = 1075;
+
+__global__ __launch_bounds__(kT) void k_group_mag_hist(const double *__restrict__ gh, int64_t ng,
+                                                       unsigned long long *__restrict__ hist) {
+  __shared__ unsigned int h[kMagBins];
+  for (int b = threadIdx.x; b < kMagBins; b += kT) h[b] = 0;
+  __syncthreads();
+  for (int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x; g < ng; g += (int64_t)gridDim.x * kT) {
+    const double u = gh[g * kGStride + 5];
+    int b = (u > 0.0) ? ilogb(u) + kMagOff : 0;
+    b = b < 0 ? 0 : (b >= kMagBins ? kMagBins - 1 : b);
+    atomicAdd(&h[b], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kMagBins; b += kT)
+    if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+
+__global__ __launch_bounds__(kT) void k_group_split(const double *__restrict__ gh, int64_t ng,
+                                                    double tau, int heavy,
+                                                    uint8_t *__restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= ng) return;
+  const bool big = gh[g * kGStride + 5] >= tau;
+  out[g] = (uint8_t)(heavy ? big : !big);
+}
+
+__global__ __launch_bounds__(kT) void k_group_gather(const double *__restrict__ src,
+                                                     const int64_t *__restrict__ idx, int64_t n,
+                                                     double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (t >= n * kGStride) return;
+  const int64_t r = t / kGStride;
+  dst[t] = src[idx[r] * kGStride + (t - r * kGStride)];
+}
+
+__global__ __launch_bounds__(kT) void k_to_float(const double *__restrict__ x, int64_t n,
+                                                 float *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+// temporaries of one grouping pass, freed on return
+struct Temps {
+  std::vector<void *> p;
+  ~Temps() {
+    for (void *x : p) (void)hipFree(x);
+  }
+  template <class T>
+  hipError_t get(T *&out, size_t n) {
+    void *x = nullptr;
+    hipError_t e = hipMalloc(&x, (n ? n : 1) * sizeof(T));
+    if (e == hipSuccess) p.push_back(x);
+    out = static_cast<T *>(x);
+    return e;
+  }
+};
+
+}  // namespace
+
+// Group the prepared records (after prepare_quartets).  Mixed precision:
+// the groups are split by their summed L1 bound the way records are (the
+// lightest to FP32 while their bounds stay within mixed_tail * |sum M|), the
+// FP64 groups first.  Leaves ctx->ng = 0 (the evaluation then runs record
+// by record) when there is nothing to group or more than 2^32 - 1 records.
+int build_groups(inca_ctx *ctx, int precision) {
+  hipStream_t st = ctx->stream;
+  ctx->ng = ctx->ng_fp64 = 0;
+  ctx->gpool_n = 0;
+  ctx->gflop_fp64 = ctx->gflop_fp32 = 0.0;
+  const int64_t n = ctx->q2;
+  // total degree of a group at most 8 (s, p and d primitives): at most 165
+  // coefficients, which k_group_coeffs covers in three passes of a wave
+  if (!ctx->grouping || n == 0 || n > (int64_t)UINT32_MAX || 4 * ctx->lmax_prim > 8) return 0;
+  const int stride = ctx->qstride;
+  Temps tmp;
+  uint32_t *d_kin, *d_kout;
+  uint32_t *d_vin, *d_vout;
+  uint8_t *d_head;
+  INCA_HIP(tmp.get(d_kin, n));
+  INCA_HIP(tmp.get(d_kout, n));
+  INCA_HIP(tmp.get(d_vin, n));
+  INCA_HIP(tmp.get(d_vout, n));
+  INCA_HIP(tmp.get(d_head, n));
+  hipLaunchKernelGGL(k_group_keys, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, n, d_kin,
+                     d_vin);
+  INCA_HIP(hipGetLastError());
+  size_t sort_bytes = 0;
+  INCA_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
+                                     0u, 32u, st));
+  char *d_sort;
+  INCA_HIP(tmp.get(d_sort, sort_bytes));
+  INCA_HIP(rocprim::radix_sort_pairs(d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
+                                     0u, 32u, st));
+  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, d_vout, n,
+                     d_head);
+  INCA_HIP(hipGetLastError());
+  int64_t *d_first;
+  INCA_HIP(tmp.get(d_first, n));
+  int64_t ng = 0;
+  int rc;
+  if ((rc = compact_flags(ctx, d_head, n, d_first, &ng))) return rc;
+  if ((rc = ensure(ctx, ctx->d_gh, ctx->gh_cap, (size_t)ng * kGStride))) return rc;
+  int64_t *d_m, *d_off;
+  INCA_HIP(tmp.get(d_m, ng));
+  INCA_HIP(tmp.get(d_off, ng + 1));
+  hipLaunchKernelGGL(k_group_header, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_q, stride, d_vout,
+                     d_first, ng, n, ctx->d_gh, d_m);
+  INCA_HIP(hipGetLastError());
+  size_t scan_bytes = 0;
+  INCA_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_m, d_off, (int64_t)0, (size_t)ng,
+                                   rocprim::plus<int64_t>(), st));
+  char *d_scan;
+  INCA_HIP(tmp.get(d_scan, scan_bytes));
+  INCA_HIP(rocprim::exclusive_scan(d_scan, scan_bytes, d_m, d_off, (int64_t)0, (size_t)ng,
+                                   rocprim::plus<int64_t>(), st));
+  int64_t last[2] = {0, 0};
+  INCA_HIP(hipMemcpyAsync(&last[0], d_off + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipMemcpyAsync(&last[1], d_m + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  const int64_t npool = last[0] + last[1];
+  hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, ctx->d_gh);
+  INCA_HIP(hipGetLastError());
+  if ((rc = ensure(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)npool))) return rc;
+  hipLaunchKernelGGL(k_group_coeffs, dim3((unsigned)((ng + 3) / 4)), dim3(kT),
+                     (kT / 64) * kStage * stride * sizeof(double), st, ctx->d_q, stride, d_vout,
+                     d_first, ng, n, ctx->d_gh, ctx->d_gpool);
+  INCA_HIP(hipGetLastError());
+  int64_t ng64 = ng;
+  if (precision == INCA_MIXED && ctx->mixed_tail > 0.0) {
+    unsigned long long *d_h;
+    INCA_HIP(tmp.get(d_h, kMagBins));
+    std::vector<unsigned long long> h(kMagBins, 0);
+    INCA_HIP(hipMemcpyAsync(d_h, h.data(), kMagBins * sizeof(unsigned long long),
+                            hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_group_mag_hist, dim3(1024), dim3(kT), 0, st, ctx->d_gh, ng, d_h);
+    INCA_HIP(hipGetLastError());
+    INCA_HIP(hipMemcpyAsync(h.data(), d_h, kMagBins * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipStreamSynchronize(st));
+    std::vector<double> mass(kMagBins);
+    double total = 0.0;
+    for (int b = 0; b < kMagBins; ++b) {
+      mass[b] = h[b] ? (double)h[b] * std::ldexp(2.0, std::min(b - kMagOff, 1000)) : 0.0;
+      total += mass[b];
+    }
+    const double budget = ctx->mixed_tail >= 1.0
+                              ? HUGE_VAL
+                              : ctx->mixed_tail * std::min(std::fabs(ctx->term_integral), total);
+    double cum = 0.0;
+    int cut = 0;
+    while (cut < kMagBins && cum + mass[cut] <= budget) cum += mass[cut++];
+    const double tau = std::ldexp(1.0, cut - kMagOff);
+    // stable partition: heavy groups (FP64) first, then the light ones
+    double *d_new;
+    uint8_t *d_fl;
+    int64_t *d_idx;
+    INCA_HIP(tmp.get(d_new, (size_t)ng * kGStride));
+    INCA_HIP(tmp.get(d_fl, ng));
+    INCA_HIP(tmp.get(d_idx, ng));
+    int64_t nh = 0, nl = 0;
+    hipLaunchKernelGGL(k_group_split, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_gh, ng, tau, 1, d_fl);
+    INCA_HIP(hipGetLastError());
+    if ((rc = compact_flags(ctx, d_fl, ng, d_idx, &nh))) return rc;
+    if (nh > 0)
+      hipLaunchKernelGGL(k_group_gather, dim3(nblk(nh * kGStride)), dim3(kT), 0, st, ctx->d_gh,
+                         d_idx, nh, d_new);
+    hipLaunchKernelGGL(k_group_split, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_gh, ng, tau, 0, d_fl);
+    INCA_HIP(hipGetLastError());
+    if ((rc = compact_flags(ctx, d_fl, ng, d_idx, &nl))) return rc;
+    if (nl > 0)
+      hipLaunchKernelGGL(k_group_gather, dim3(nblk(nl * kGStride)), dim3(kT), 0, st, ctx->d_gh,
+                         d_idx, nl, d_new + nh * kGStride);
+    INCA_HIP(hipGetLastError());
+    INCA_HIP(hipMemcpyAsync(ctx->d_gh, d_new, (size_t)ng * kGStride * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
+    ng64 = nh;
+    if ((rc = ensure(ctx, ctx->d_gpool32, ctx->gpool32_cap, (size_t)npool))) return rc;
+    hipLaunchKernelGGL(k_to_float, dim3(nblk(npool)), dim3(kT), 0, st, ctx->d_gpool, npool,
+                       ctx->d_gpool32);
+    INCA_HIP(hipGetLastError());
+  }
+  // operations per point of the FP64 and FP32 ranges (header word 11)
+  double *d_fl2;
+  INCA_HIP(tmp.get(d_fl2, 2));
+  hipLaunchKernelGGL(k_group_flops, dim3(1), dim3(kT), 0, st, ctx->d_gh, ng, ng64, d_fl2);
+  INCA_HIP(hipGetLastError());
+  double fl[2] = {0.0, 0.0};
+  INCA_HIP(hipMemcpyAsync(fl, d_fl2, sizeof(fl), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  ctx->ng = ng;
+  ctx->ng_fp64 = ng64;
+  ctx->gpool_n = npool;
+  ctx->gflop_fp64 = fl[0];
+  ctx->gflop_fp32 = fl[1];
+  return 0;
+}
+
+}  // namespace inca

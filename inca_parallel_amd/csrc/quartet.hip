@@ -914,7 +914,39 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   ctx->q_fp64 = q2;
   ctx->mixed_tau = 0.0;
   ctx->prep_precision = precision;
-  if (precision == INCA_MIXED && q2 > 0 && ctx->mixed_tail > 0.0) {
+  // the records' FP64/FP32 split (split_records) waits until a record-by-
+  // record mixed evaluation needs it: grouped evaluations split the groups
+  ctx->rec_split_pending = precision == INCA_MIXED && q2 > 0 && ctx->mixed_tail > 0.0;
+  INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(hcnt), hipMemcpyDeviceToHost, st));
+  INCA_HIP(hipStreamSynchronize(st));
+  ctx->q2 = q2;
+  counts[0] = nvalid;
+  counts[1] = (int64_t)hcnt[1];
+  counts[2] = q2;
+  counts[3] = (int64_t)hcnt[3];
+  if ((rc = build_groups(ctx, precision))) return rc;
+  // without groups the evaluation runs record by record: split now
+  return ctx->ng > 0 ? 0 : split_records(ctx);
+}
+
+
+// The records' precision split of a mixed preparation.  FP32 roundoff in a
+// term is bounded by a few ulps of its L1 norm, bounded by B; the lightest
+// terms go to the FP32 range while their summed B stays within mixed_tail *
+// |sum M| (sum M being the integral of I over all space), the rest stays
+// FP64 and comes first.  Done once, when a record-by-record mixed
+// evaluation (or a query of the split) first needs it.
+int split_records(inca_ctx *ctx) {
+  if (!ctx->rec_split_pending) return 0;
+  ctx->rec_split_pending = 0;
+  hipStream_t st = ctx->stream;
+  const int64_t q2 = ctx->q2;
+  const int stride = ctx->qstride;
+  int rc = 0;
+  Carve cv;
+  const size_t o_hist = cv.add<unsigned long long>(kMagBins);
+  if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
+  {
     // Precision by magnitude.  FP32 roundoff in a term is bounded by a few
     // ulps of its L1 norm, bounded by B; the lightest terms go to the FP32
     // kernel while their summed B stays within mixed_tail * |sum M| (sum M
@@ -981,13 +1013,6 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       ctx->q_fp64 = nh;
     }
   }
-  INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(hcnt), hipMemcpyDeviceToHost, st));
-  INCA_HIP(hipStreamSynchronize(st));
-  ctx->q2 = q2;
-  counts[0] = nvalid;
-  counts[1] = (int64_t)hcnt[1];
-  counts[2] = q2;
-  counts[3] = (int64_t)hcnt[3];
   return 0;
 }
 

@@ -165,11 +165,15 @@ int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec);
  * inca_evaluate prepared, on the current grid, is enqueued on `stream` (a
  * hipStream_t; NULL: the context's stream), and its result lands in ivec, a
  * DEVICE pointer to G doubles owned by the caller.  Every grid kind is
- * evaluated point by point with the per-point cutoff off (cube grids are not
- * contracted per axis), and the FP32 range of INCA_MIXED always runs the
- * mixed kernel: the result is that of inca_evaluate_prepared after
- * inca_set_point_cutoff(ctx, 0), inca_set_cube_contraction(ctx, 0) and
- * non-zero FP32 tiles, up to summation order.  The tile records and partial
+ * evaluated record by record and point by point with the per-point cutoff
+ * off (cube grids are not contracted per axis, shell-quartet groups are not
+ * used), and the FP32 range of INCA_MIXED always runs the mixed kernel: the
+ * result is that of inca_evaluate_prepared after inca_set_point_cutoff(ctx,
+ * 0), inca_set_cube_contraction(ctx, 0), inca_set_grouping(ctx, 0) and
+ * non-zero FP32 tiles, up to summation order.  A mixed preparation whose
+ * record split has not been made yet (grouped preparations make it at the
+ * first record-by-record mixed evaluation or inca_get_mixed_split) runs
+ * every record in FP64.  The tile records and partial
  * sums live in a static 32 MiB array per device, held until the process
  * ends: the grid must fit one partial row of G doubles per precision range
  * plus the tile records (about 4 million points in FP64, 2 million mixed);
@@ -353,8 +357,38 @@ int inca_set_point_cutoff(inca_ctx *ctx, double log2_rel);
  * (mod_intracule.f90:345-358). */
 int inca_set_cube_contraction(inca_ctx *ctx, int32_t on);
 /* Evaluations of the last grid loop: reference = quartets x points (the
- * reference's count), executed = those the cutoff kept (equal when off). */
+ * reference's count), executed = the (record, point) pairs the cutoff kept
+ * (equal when off), or, when the loop ran by shell-quartet groups, the
+ * (group, point) pairs it kept. */
 int inca_get_eval_counts(inca_ctx *ctx, int64_t *reference, int64_t *executed);
+
+/* Evaluation by shell-quartet groups.  The records of one shell quartet
+ * (and any others with the same sqe and sqe*(P_ik - P_jl), compared bit for
+ * bit) share rp and exp(-|rp|^2) in the grid loop (mod_intracule.f90:
+ * 345-358), so the pre-pass adds their A-scaled axis polynomials into one
+ * three-variable polynomial per group, K_abc = sum_q (A Cx_q)[a] Cy_q[b]
+ * Cz_q[c] in record order, and the grid loop evaluates one exponential and
+ * one polynomial per group and point (at most 165 coefficients for s/p/d
+ * against up to 1296 records).  on = 1 (default) builds the groups in every
+ * inca_prepare / inca_evaluate and evaluates radial scans, multicentre grids
+ * and explicit points by groups; on = 0 evaluates record by record (the
+ * reference's loop order).  Cube grids under the per-axis contraction and
+ * inca_evaluate_prepared_noalloc always run record by record.  The results
+ * differ from the record loop only by summation order.  A mixed preparation
+ * splits the groups by their summed L1 bounds as it splits records (see
+ * inca_set_mixed_tail).  Takes effect at the next preparation (turning it
+ * off also at the next evaluation).  No reference counterpart. */
+int inca_set_grouping(inca_ctx *ctx, int32_t on);
+/* Groups of the last preparation: their count, those evaluated in FP64 in
+ * mixed mode (the FP64 range), and the coefficients in the pool (0 when the
+ * preparation did not group). */
+int inca_get_groups(inca_ctx *ctx, int64_t *ngroups, int64_t *ngroups_fp64,
+                    int64_t *ncoefficients);
+/* FP64-equivalent arithmetic of the last grid loop when it ran by groups:
+ * the sum over the (group, point) pairs it evaluated of 14 + 2 M (M the
+ * group's coefficient count; rp 6, |rp|^2 5, the polynomial 2 M, product and
+ * sum 3, exp not counted), 0 otherwise. */
+int inca_get_eval_flops(inca_ctx *ctx, double *flops);
 
 /* Sum over the quartets prepared by the last inca_evaluate of each term's
  * integral over all space, prod_axis sum_{k even} C_k Gamma((k+1)/2) A /
@@ -380,11 +414,15 @@ int inca_set_polynomials(inca_ctx *ctx, int32_t mode);
  * lanes are rays of that many points).  p32 = 0 runs the FP32 range of a
  * mixed evaluation on the FP64 ray tiles instead, where the grid has them
  * (more accurate; faster wherever the ray kernel is); inca_autotune picks it
- * when it times faster.  inca_autotune times each choice on an evenly strided
+ * when it times faster.  The same values set the points per lane of the
+ * shell-quartet group kernels (10 runs 8 there; a p32 of 0 leaves the
+ * group FP32 choice as it was).  inca_autotune times each choice on an evenly strided
  * sample of the quartets prepared by the last inca_evaluate or inca_prepare
  * (about 2^36 evaluations per launch), on the current grid and with the
  * current cutoff, keeps the
- * fastest and returns it in best[0] (FP64) and best[1] (FP32).  It
+ * fastest and returns it in best[0] (FP64) and best[1] (FP32) -- the group
+ * kernels' choices (1, 2, 4, 6 or 8, from a sample of the groups) when the
+ * current grid is evaluated by groups.  It
  * overwrites the partial sums, so evaluate again before reading Ivec.  No
  * reference counterpart. */
 int inca_set_tiles(inca_ctx *ctx, int32_t points_fp64, int32_t points_fp32);

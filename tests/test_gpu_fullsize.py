@@ -7,14 +7,15 @@ LD0302 (32,502 points).  The oracle cannot finish this size in seconds, so
 the checks are internal:
   * the record count is the closed form npair (npair + 1) / 2 with
     npair = nprim (nprim + 1) / 2, and the screens only remove records;
-  * the ray tiles (8 points per lane, exp(-s) by recurrence) against the
-    point-by-point kernel on the same prepared quartets;
+  * the evaluation by shell-quartet groups (the default) against the record
+    kernels on the same prepared quartets: the ray tiles (8 points per
+    lane, exp(-s) by recurrence) and the point-by-point kernel;
   * the per-point cutoff against the full quartet x point loop;
   * mixed precision against FP64 (1e-8 on the integral, the precision
     target of the mixed path);
   * two evaluations bitwise identical (fixed-order reductions);
   * the 80^3 cube of the same records by the per-axis contraction against
-    the term-by-term ray tiles.
+    the term-by-term evaluation by groups.
 The FP64 bar is the one of the oracle parity tests: 1e-10 relative wherever
 |I| >= 1e-6 max|I|, and 1e-13 of max|I| everywhere.  The oracle-based parity
 of each kernel is in test_gpu_parity.py and test_gpu_ch4.py at sizes the
@@ -78,21 +79,30 @@ def test_record_count_and_screens(full):
 
 
 def test_rays_match_point_by_point(full):
+    """Record by record: the ray tiles (8 points per lane) and the
+    point-by-point kernel, each against the fixture's evaluation by
+    shell-quartet groups."""
     e = full["e"]
+    out = {}
     try:
-        e.set_tiles(1, 8)
-        iv1, cnt1 = e.evaluate(capi.INCA_FP64, THRESH)
-        rad1 = e.radial_values()
+        e.set_grouping(0)
+        for P in (8, 1):
+            e.set_tiles(P, 8)
+            iv, cnt = e.evaluate(capi.INCA_FP64, THRESH)
+            assert list(cnt) == list(full["cnt"])
+            assert e.eval_flops() == 0.0  # record by record
+            out[P] = (iv.copy(), e.radial_values().copy())
     finally:
         e.set_tiles(8, 8)
-    assert list(cnt1) == list(full["cnt"])
-    assert not np.array_equal(iv1, full["iv"])  # two different kernels ran
-    er, ea = _close(full["iv"], iv1)
-    rr, ra = _close(full["rad"], rad1)
-    print("rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e"
-          % (er, ea, rr, ra))
-    assert er < 1e-10 and ea < 1e-13
-    assert rr < 1e-10 and ra < 1e-13
+        e.set_grouping(1)
+    assert not np.array_equal(out[1][0], out[8][0])  # two different kernels ran
+    for P in (8, 1):
+        er, ea = _close(out[P][0], full["iv"])
+        rr, ra = _close(out[P][1], full["rad"])
+        print("records P=%d vs groups: Ivec rel %.2e abs %.2e; I(s) rel %.2e abs %.2e"
+              % (P, er, ea, rr, ra))
+        assert er < 1e-10 and ea < 1e-13
+        assert rr < 1e-10 and ra < 1e-13
 
 
 def test_cutoff_matches_full_loop(full):
@@ -103,7 +113,9 @@ def test_cutoff_matches_full_loop(full):
         ref0, ex0 = e.eval_counts()
     finally:
         e.set_point_cutoff(100)
-    assert ex0 == ref0 == int(full["cnt"][2]) * full["G"]
+    assert ref0 == int(full["cnt"][2]) * full["G"]
+    # the full loop executes every (group, point) pair
+    assert ex0 == e.groups()[0] * full["G"]
     er, ea = _close(full["iv"], iv0)
     print("cutoff vs full loop: rel %.2e abs %.2e" % (er, ea))
     assert er < 1e-10 and ea < 1e-13
@@ -132,8 +144,9 @@ def test_repeat_is_bitwise_identical(full):
 def test_cube_contraction_matches_ray_tiles(full):
     """The 80^3 cube (0.15 bohr steps, centred at the origin) on the same
     records: the per-axis contraction (every quartet at every voxel) on the
-    vector units and on the matrix cores, against the term-by-term ray tiles
-    along z (cutoff on).  Runs last: it replaces the grid."""
+    vector units and on the matrix cores, against the term-by-term
+    evaluation by shell-quartet groups (cutoff on).  Runs last: it replaces
+    the grid."""
     e = full["e"]
     n = e.grid_cube([0.0, 0.0, 0.0], [0.15, 0.15, 0.15], [80, 80, 80])
     assert n == 512000
@@ -154,7 +167,7 @@ def test_cube_contraction_matches_ray_tiles(full):
     assert list(cnt) == list(full["cnt"])
     for mode, iv in got.items():
         er, ea = _close(iv, ray)
-        print("80^3 cube, contraction mode %d vs ray tiles: rel %.2e abs %.2e" % (mode, er, ea))
+        print("80^3 cube, contraction mode %d vs groups: rel %.2e abs %.2e" % (mode, er, ea))
         assert er < 1e-10 and ea < 1e-13
     full["cube_fp64"] = got[1]
 

@@ -313,27 +313,31 @@ def test_point_cutoff_matches_full_loop(engine, small_cluster):
         lambda: engine.grid_cube([0.0, 0.0, 0.0], [0.4, 0.4, 0.4], [15, 15, 15]),
     ]
     try:
-        for make in grids:
-            make()
-            for prec in (capi.INCA_FP64, capi.INCA_MIXED):
-                engine.set_point_cutoff(0)
-                full, _ = engine.evaluate(prec, THRESH)
-                ref_n, ex_n = engine.eval_counts()
-                assert ex_n == ref_n
-                engine.set_point_cutoff(100)
-                cut, _ = engine.evaluate(prec, THRESH)
-                ref_c, ex_c = engine.eval_counts()
-                assert ref_c == ref_n and 0 < ex_c <= ref_c
-                print("executed %d of %d evaluations" % (ex_c, ref_c))
-                er, ea = _close(cut, full)
-                print("grid G=%d prec %d: cutoff vs full rel %.2e abs %.2e" % (
-                    engine.G, prec, er, ea))
-                if prec == capi.INCA_FP64:
-                    assert er < 1e-13 and ea < 1e-15
-                else:  # skipped terms shift the FP32 tile roundoff
-                    assert er < 1e-10 and ea < 1e-12
+        for grouping in (1, 0):  # shell-quartet groups, then record by record
+            engine.set_grouping(grouping)
+            for make in grids:
+                make()
+                for prec in (capi.INCA_FP64, capi.INCA_MIXED):
+                    engine.set_point_cutoff(0)
+                    full, _ = engine.evaluate(prec, THRESH)
+                    ref_n, ex_n = engine.eval_counts()
+                    engine.set_point_cutoff(100)
+                    cut, _ = engine.evaluate(prec, THRESH)
+                    ref_c, ex_c = engine.eval_counts()
+                    assert ref_c == ref_n and 0 < ex_c <= ex_n <= ref_n
+                    # grouped runs count (group, point) pairs
+                    assert ex_n == ref_n if engine.eval_flops() == 0 else ex_n < ref_n
+                    print("executed %d of %d evaluations" % (ex_c, ex_n))
+                    er, ea = _close(cut, full)
+                    print("grouping %d grid G=%d prec %d: cutoff vs full rel %.2e abs %.2e" % (
+                        grouping, engine.G, prec, er, ea))
+                    if prec == capi.INCA_FP64:
+                        assert er < 1e-13 and ea < 1e-15
+                    else:  # skipped terms shift the FP32 tile roundoff
+                        assert er < 1e-10 and ea < 1e-12
     finally:
         engine.set_point_cutoff(100)
+        engine.set_grouping(1)
 
 
 def test_ray_tiles_match_point_by_point(engine):
@@ -356,12 +360,14 @@ def test_ray_tiles_match_point_by_point(engine):
     engine.grid_radial_plot(tart, [0.1, 0.2], [110, 302])
     out = {}
     try:
+        engine.set_grouping(0)  # the record kernels
         for P in (1, 2, 4, 6, 8, 10):
             engine.set_tiles(P, 4)
             iv, _ = engine.evaluate(capi.INCA_FP64, THRESH)
             out[P] = (iv.copy(), engine.radial_values().copy())
     finally:
         engine.set_tiles(4, 4)
+        engine.set_grouping(1)
     for P in (2, 4, 6, 8, 10):
         er, ea = _close(out[P][0], out[1][0])
         rr, ra = _close(out[P][1], out[1][1])
@@ -425,6 +431,7 @@ def test_ray_tiles_on_cube_lines(engine):
     engine.grid_cube([0.1, -0.2, 0.3], [0.3, 0.25, 0.15], [9, 7, 80])
     out = {}
     try:
+        engine.set_grouping(0)  # the record kernels
         engine.set_cube_contraction(0)  # term by term, on the ray tiles
         for cut in (100, 0):
             engine.set_point_cutoff(cut)
@@ -436,6 +443,7 @@ def test_ray_tiles_on_cube_lines(engine):
         engine.set_tiles(4, 4)
         engine.set_point_cutoff(100)
         engine.set_cube_contraction(2)
+        engine.set_grouping(1)
     for cut in (100, 0):
         for P in (4, 6, 8, 10):
             er, ea = _close(out[(cut, P)], out[(cut, 1)])
@@ -519,6 +527,7 @@ def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
         (False, lambda: engine.grid_multicenter(cent, [30] * nq, [110] * nq, sf, Ps, nosym=True)),
     ]
     try:
+        engine.set_grouping(0)  # the record kernels
         engine.set_mixed_tail(0.05)
         engine.set_cube_contraction(0)
         for rays, make in grids:
@@ -536,6 +545,7 @@ def test_mixed_fp32_range_on_ray_tiles(engine, small_cluster):
             else:
                 assert ea < 1e-8
     finally:
+        engine.set_grouping(1)
         engine.set_tiles(4, 4)
         engine.set_mixed_tail(0.02)
         engine.set_cube_contraction(2)

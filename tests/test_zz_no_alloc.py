@@ -1,8 +1,9 @@
 """inca_evaluate_prepared_noalloc against inca_evaluate_prepared.
 
-The no-allocation entry point evaluates point by point with the cutoff
-off, its partial sums in a static per-device array; the allocating entry
-point, run with the cutoff off, is the checker.  Two calls in a row, the
+The no-allocation entry point evaluates record by record, point by point,
+with the cutoff off, its partial sums in a static per-device array; the
+allocating entry point, run with the cutoff off and without shell-quartet
+groups, is the checker.  Two calls in a row, the
 second on a smaller input, so that leftovers of the first in the shared
 array would show; then a grid too large for the array must be refused.
 FP64 bar: 1e-12 of max|I| (the two differ only in summation order and, on
@@ -45,6 +46,7 @@ def test_noalloc_matches_allocating_entry_point(ch4):
     with capi.Engine(0) as big, capi.Engine(0) as small:
         # call 1: d functions on a radial scan (about 16,000 points), FP64
         big.set_point_cutoff(0)
+        big.set_grouping(0)
         big.set_tiles(4, 4)
         big.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
         big.set_quartets(ijkl=ijkl, d=d)
@@ -59,6 +61,7 @@ def test_noalloc_matches_allocating_entry_point(ch4):
         # call 2, a smaller input: CH4 at three points, on another context
         # and on torch's current stream
         small.set_point_cutoff(0)
+        small.set_grouping(0)
         small.set_tiles(4, 4)
         small.set_primitives(ch4["alpha"], ch4["tmn"], ch4["x"], ch4["y"], ch4["z"])
         small.set_quartets(ijkl=ch4["ijkl"], d=ch4["d"])
