@@ -9,10 +9,12 @@ The records are generated directly in HBM before timing.
 
 One step = one pass of the hot path over the resident records:
 inca_evaluate = decode + normalise + screen 1 + prefactors + polycoef +
-screen 2 + compaction + the quartet x grid-point contraction + reduction.
-`value` is reference-equivalent evals/s of the FP64 path (evals = quartets
-passing both screens x grid points); the mixed-precision path is timed the
-same way and reported beside it with its error against FP64.
+screen 2 + compaction + shell-quartet grouping + the quartet x grid-point
+contraction (by groups) + reduction.  `value` is reference-equivalent
+evals/s of the FP64 path (evals = quartets passing both screens x grid
+points, the reference's count); the mixed-precision path is timed the same
+way and reported beside it with its error against FP64, and the same step
+record by record (the reference's loop order) beside both.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -140,14 +142,15 @@ def batch_bench(capi, synth, torch, count=32):
 
 def target_bench(capi, torch, nmol=32):
     """Radial intracule integrals of many small molecules (3-5 atoms, s/p/d,
-    the learned grid's held-out family): the plain FP64 GPU kernel (every
-    term at every point, no cutoff, one inca_evaluate per molecule) against
-    the engine with everything on (learned autogrid, mixed precision,
-    per-point cutoff, one batched hipGraph replay, sum-rule fallback to the
-    plain autogrid).  The plain kernel is timed on the plain autogrid (nrad
-    50 / nang 590 per centre, Bragg-Slater radii) and on the learned grid;
-    the engine's integrals are checked against the FP64 full loop on the
-    same grid."""
+    the learned grid's held-out family), from records and grids already
+    resident on the device: the plain FP64 GPU kernel (the reference's loop:
+    record by record, every term at every point, one inca_evaluate per
+    molecule) against the engine with everything on (shell-quartet groups,
+    mixed precision, per-point cutoff, learned autogrid, one batched
+    hipGraph replay, the sum-rule fallback).  The plain kernel is timed on
+    the plain autogrid (nrad 50 / nang 590 per centre, Bragg-Slater radii)
+    and on the learned grid; the engine's integrals are checked against the
+    FP64 full loop on the grid it used."""
     from inca_parallel_amd import learned_grid as LG
     net = capi.MLP(seed=1)
     net.set_params(np.load(LG.WEIGHTS, allow_pickle=False))
@@ -179,31 +182,33 @@ def target_bench(capi, torch, nmol=32):
                                                        m["s"].ra, nohydro=True)
         return cent, Ps, sf, list(nr), list(na)
 
-    # the plain FP64 kernel: every term at every point, one molecule at a time
+    # the plain FP64 kernel: record by record, every term at every point, one
+    # molecule at a time (contexts, grids and records set up beforehand)
     t_plain = {"plain_grid": 0.0, "learned_grid": 0.0}
     evals = {"plain_grid": 0.0, "learned_grid": 0.0}
-    for m in mols:
-        for key, grid in (("plain_grid", plain_grid(m)), ("learned_grid", learned(m))):
-            e = engine(m, *grid)
+    for key, make in (("plain_grid", plain_grid), ("learned_grid", learned)):
+        engines = [engine(m, *make(m)) for m in mols]
+        for e in engines:
+            e.set_grouping(0)
             e.set_point_cutoff(0)
-            if m is mols[0]:
-                e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)  # warm-up
-            torch.cuda.synchronize()
+        engines[0].evaluate(capi.INCA_FP64, THRESH, want_ivec=False)  # warm-up
+        torch.cuda.synchronize()
+        for m, e in zip(mols, engines):
             t0 = time.perf_counter()
             _, cnt = e.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
             tot = e.integrate()[0]
             t_plain[key] += time.perf_counter() - t0
             evals[key] += float(cnt[2]) * e.G
             m[key] = dict(fp64=tot, analytic=e.analytic_integral(), points=e.G)
+        for e in engines:
             e.close()
-    # the engine: learned grid, mixed, cutoff, batched
+    # the engine: learned grid, groups, mixed, cutoff, batched (contexts,
+    # grids and records set up beforehand)
+    engines = [engine(m, *learned(m)) for m in mols]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    engines = []
-    for m in mols:
-        e = engine(m, *learned(m))
+    for e in engines:
         e.prepare(capi.INCA_MIXED, THRESH)
-        engines.append(e)
     t_prep = time.perf_counter() - t0
     b = capi.Batch(engines)
     t_create = time.perf_counter() - t0 - t_prep
@@ -224,8 +229,6 @@ def target_bench(capi, torch, nmol=32):
     check = [i for i in range(nmol) if err_l[i] > LG.FALLBACK_TOL]
     tp = {}
     if check:
-        # the plain autogrids of the molecules to check, on their prepared
-        # (mixed) quartets, in one batch
         for i in check:
             cent, Ps, sf, nr, na = plain_grid(mols[i])
             engines[i].grid_multicenter(cent, nr, na, sf, Ps, nosym=True)
@@ -270,13 +273,14 @@ def target_bench(capi, torch, nmol=32):
         "max_rel_err_engine_vs_fp64_same_grid": float(max(err_fast)),
         "fallbacks": used.count("plain_grid"),
         "median_sum_rule_err": {k: float(np.median(v)) for k, v in sum_rule.items()},
-        "note": "plain: inca_evaluate FP64 with the cutoff off (the reference's full loop) plus "
-                "the integral, one molecule at a time, on the learned grid (same grid) and on the "
-                "plain autogrid; engine: inca_prepare(INCA_MIXED) per molecule, batch create and "
-                "the first replay on the learned grid (replay: later replays); fallback check: "
-                "the plain autogrids of the molecules whose learned-grid integral misses the "
-                "analytic integral by more than 1e-8, built and evaluated (mixed, on the prepared "
-                "quartets) in one batch",
+        "note": "from records and grids resident on the device (context set-up excluded on both "
+                "sides); plain: inca_evaluate FP64 record by record with the cutoff off (the "
+                "reference's full loop) plus the integral, one molecule at a time, on the learned "
+                "grid (same grid) and on the plain autogrid; engine: inca_prepare(INCA_MIXED) per "
+                "molecule (pre-pass and shell-quartet groups), batch create and the first replay "
+                "on the learned grid (replay: later replays); fallback check: the plain autogrids "
+                "of the molecules whose learned-grid integral misses the analytic integral by more "
+                "than 1e-8, built and evaluated (mixed, on the prepared quartets) in one batch",
     }
 
 
@@ -406,25 +410,42 @@ def main():
         dt = (time.perf_counter() - t0) / a.steps
         kern = np.array(kern)
         results[name] = dict(step_s=dt, counts=cnt, kern_ms=kern.mean(axis=0),
-                             rintra=eng.radial_values(), split=eng.mixed_split(),
-                             evals=eng.eval_counts())
+                             rintra=eng.radial_values(), groups=eng.groups(),
+                             evals=eng.eval_counts(), flops=eng.eval_flops())
         log("%s: %.3f s/step, kernels (prep, eval, reduce) ms = %s, counts %s" % (
             name, dt, np.round(kern.mean(axis=0), 2), cnt))
 
-    hist = eng.class_histogram(64)
     cnt = results["fp64"]["counts"]
     q2 = int(cnt[2])
     evals = float(q2) * G
-    W = flops_per_eval(hist)
     r64 = results["fp64"]
     rm = results["mixed"]
     eval_s = r64["kern_ms"][1] * 1e-3
     executed = float(r64["evals"][1])
-    # the roofline counts the work the kernel executed (the cutoff skips
-    # (quartet, tile) pairs whose terms are below 2^-100 of their peak)
-    achieved = executed * W / eval_s
+    ngroups, _, ncoef = eng.groups()
+    # the roofline counts the work the group kernel executed: 14 + 2 M FP64
+    # operations per (group, point) pair it evaluated (M coefficients; the
+    # cutoff skips (group, tile) pairs whose terms are below 2^-100 of their
+    # peak)
+    achieved = r64["flops"] / eval_s
+    W = r64["flops"] / max(executed, 1.0)
     ref = r64["rintra"]
     mixed_err = float(np.max(np.abs(rm["rintra"] - ref) / np.maximum(np.abs(ref), 1e-300)))
+    # the record loop (the reference's order: one exponential and three axis
+    # polynomials per record and point; ray tiles on this scan) for comparison
+    eng.set_grouping(0)
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    rec_tiles = eng.autotune()
+    t0 = time.perf_counter()
+    _, cnt_r = eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    rec_step = time.perf_counter() - t0
+    rec_kern = eng.timings()
+    rec_rintra = eng.radial_values()
+    hist = eng.class_histogram(64)
+    eng.set_grouping(1)
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    rec_err = float(np.max(np.abs(rec_rintra - ref)[np.abs(ref) >= 1e-6 * np.abs(ref).max()] /
+                           np.abs(ref)[np.abs(ref) >= 1e-6 * np.abs(ref).max()]))
 
     out = {
         "metric": "quartet x gridpoint evals/s; time to 1e-8 rel integral, mixed vs FP64 GPU",
@@ -444,33 +465,43 @@ def main():
                         "%d symmetry-unique DM2 records, s=0..10 step 0.05 x LD0302 "
                         "(%d points), thresh 1e-12" % (sysm.nprim, Q, G),
             "records": Q, "quartets_after_screen1": int(cnt[1]),
-            "quartets_computed": q2, "grid_points": G, "flop_per_eval": W,
-            "reference_evals": evals, "executed_evals": executed,
-            "point_cutoff": "2^-100 of each term's peak (executed/reference = %.3f)"
-                            % (executed / evals),
+            "quartets_computed": q2, "grid_points": G,
+            "shell_quartet_groups": ngroups, "group_coefficients": ncoef,
+            "flop_per_executed_group_eval": W,
+            "reference_evals": evals, "executed_group_evals": executed,
+            "point_cutoff": "2^-100 of each term's peak",
             "points_per_lane": {"fp64": tiles[0], "fp32": tiles[1]},
         },
         "roofline": {
-            "bound": "valu", "kernel": "k_eval_rays (ray tiles of the radial scan)",
+            "bound": "valu", "kernel": "k_geval (shell-quartet groups, FP64)",
             "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_VALU, "traffic": profiled_traffic(),
-            "note": "FLOP = 16 + 2*Lsum per executed eval (FMA = 2), exp not counted; grid loop "
-                    "%.1f ms per step (HIP events on the engine stream around the skip mask, rho "
-                    "and the ray-tile kernel); traffic = HBM bytes of the ray kernel per step from "
-                    "rocprofv3 FETCH_SIZE + WRITE_SIZE at 8 points per lane (%s); the ray kernel "
-                    "is bound by FP64 VALU issue (0.83 busy at occupancy 5)"
+            "note": "FLOP = 14 + 2 M per executed (group, point) pair (M coefficients of the "
+                    "group's polynomial; FMA = 2; exp not counted), summed by the skip-mask pass "
+                    "(inca_get_eval_flops); grid loop %.1f ms per step (HIP events on the engine "
+                    "stream around the tile bounds, skip mask and group kernel); traffic = HBM "
+                    "bytes of the group kernel per step from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s)"
                     % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
+        },
+        "record_loop": {
+            "note": "the same step record by record (the reference's loop order; ray tiles)",
+            "step_s": rec_step, "evals_per_s": evals / rec_step,
+            "kernel_ms": {"prep": float(rec_kern[0]), "eval": float(rec_kern[1])},
+            "points_per_lane": rec_tiles[0], "flop_per_eval": flops_per_eval(hist),
+            "max_rel_err_rintra_vs_groups": rec_err,
         },
         "mixed": {
             "value": evals / rm["step_s"], "unit": "evals/s",
             "speedup_vs_fp64": r64["step_s"] / rm["step_s"],
             "max_rel_err_rintra_vs_fp64": mixed_err,
             "eval_kernel_ms": float(rm["kern_ms"][1]),
-            "quartets_fp64": int(rm["split"][0]), "quartets_fp32": int(rm["split"][1]),
+            "groups_fp64": int(rm["groups"][1]),
+            "groups_fp32": int(rm["groups"][0] - rm["groups"][1]),
         },
         "dm2_stream": {"bytes": 24 * Q, "prep_ms": float(r64["kern_ms"][0]),
                        "GB_per_s": 24 * Q / (float(r64["kern_ms"][0]) * 1e-3) / 1e9,
-                       "note": "24-byte .dm2p records decoded, screened and prepared per step"},
+                       "note": "24-byte .dm2p records decoded, screened, prepared and grouped "
+                               "per step"},
         "kernel_ms": {"prep": float(r64["kern_ms"][0]), "eval": float(r64["kern_ms"][1]),
                       "reduce": float(r64["kern_ms"][2])},
     }

@@ -83,7 +83,7 @@ int inca_destroy(inca_ctx *c) {
                   c->d_idx, c->d_bc, c->d_cnt, c->d_partial, c->d_ivec, c->d_tb,
                   c->d_gcent, c->d_mask, c->d_evals, c->d_wcount, c->d_ridx, c->d_dup,
                   c->d_red, c->d_tmp, c->d_cf, c->d_gh, c->d_gpool, c->d_gpool32,
-                  c->d_wflop, c->d_flops};
+                  c->d_wflop, c->d_flops, c->d_gscr, c->d_ghtmp};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
@@ -471,6 +471,9 @@ int inca_release_scratch(inca_ctx *ctx) {
   drop(ctx->d_wcount, ctx->wcount_cap);
   drop(ctx->d_cf, ctx->cf_cap);
   drop(ctx->d_qf, ctx->qf_cap);
+  drop(ctx->d_gscr, ctx->gscr_cap);
+  drop(ctx->d_ghtmp, ctx->ghtmp_cap);
+  drop(ctx->d_wflop, ctx->wflop_cap);
   return 0;
 }
 

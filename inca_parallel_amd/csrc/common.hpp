@@ -134,6 +134,10 @@ struct inca_ctx {
   double flops_executed = 0.0;                // of the last grid loop (group evaluation)
   int grouped_last = 0;                       // the last grid loop ran by groups
   int gtile_p[2] = {4, 4};                    // points per lane of the group kernels
+  char *d_gscr = nullptr;                     // grouping scratch (sort, scan, maps)
+  size_t gscr_cap = 0;
+  double *d_ghtmp = nullptr;                  // headers during the mixed split
+  size_t ghtmp_cap = 0;
   double *d_wflop = nullptr;                  // per 64-group word: operations kept
   size_t wflop_cap = 0;
   double *d_flops = nullptr;                  // their sum over the last grid loop

@@ -294,22 +294,6 @@ __global__ __launch_bounds__(kT) void k_to_float(const double *__restrict__ x, i
   if (i < n) y[i] = (float)x[i];
 }
 
-// temporaries of one grouping pass, freed on return
-struct Temps {
-  std::vector<void *> p;
-  ~Temps() {
-    for (void *x : p) (void)hipFree(x);
-  }
-  template <class T>
-  hipError_t get(T *&out, size_t n) {
-    void *x = nullptr;
-    hipError_t e = hipMalloc(&x, (n ? n : 1) * sizeof(T));
-    if (e == hipSuccess) p.push_back(x);
-    out = static_cast<T *>(x);
-    return e;
-  }
-};
-
 }  // namespace
 
 // Group the prepared records (after prepare_quartets).  Mixed precision:
@@ -327,45 +311,47 @@ int build_groups(inca_ctx *ctx, int precision) {
   // coefficients, which k_group_coeffs covers in three passes of a wave
   if (!ctx->grouping || n == 0 || n > (int64_t)UINT32_MAX || 4 * ctx->lmax_prim > 8) return 0;
   const int stride = ctx->qstride;
-  Temps tmp;
-  uint32_t *d_kin, *d_kout;
-  uint32_t *d_vin, *d_vout;
-  uint8_t *d_head;
-  INCA_HIP(tmp.get(d_kin, n));
-  INCA_HIP(tmp.get(d_kout, n));
-  INCA_HIP(tmp.get(d_vin, n));
-  INCA_HIP(tmp.get(d_vout, n));
-  INCA_HIP(tmp.get(d_head, n));
+  // temporaries carved from the context's grouping scratch (kept between
+  // preparations, so that small molecules pay no allocation)
+  size_t sort_bytes = 0, scan_bytes = 0;
+  {
+    uint32_t *k = nullptr, *v = nullptr;
+    int64_t *a = nullptr, *b = nullptr;
+    INCA_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, k, k, v, v, (size_t)n, 0u, 32u, st));
+    INCA_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, a, b, (int64_t)0, (size_t)n,
+                                     rocprim::plus<int64_t>(), st));
+  }
+  Carve cv;
+  const size_t o_kin = cv.add<uint32_t>(n), o_kout = cv.add<uint32_t>(n),
+               o_vin = cv.add<uint32_t>(n), o_vout = cv.add<uint32_t>(n),
+               o_head = cv.add<uint8_t>(n), o_first = cv.add<int64_t>(n),
+               o_m = cv.add<int64_t>(n), o_off = cv.add<int64_t>(n + 1),
+               o_sort = cv.add<char>((int64_t)sort_bytes), o_scan = cv.add<char>((int64_t)scan_bytes),
+               o_hist = cv.add<unsigned long long>(kMagBins), o_fl2 = cv.add<double>(2),
+               o_fl = cv.add<uint8_t>(n), o_idx = cv.add<int64_t>(n);
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_gscr, ctx->gscr_cap, cv.size))) return rc;
+  char *base = ctx->d_gscr;
+  uint32_t *d_kin = Carve::at<uint32_t>(base, o_kin), *d_kout = Carve::at<uint32_t>(base, o_kout);
+  uint32_t *d_vin = Carve::at<uint32_t>(base, o_vin), *d_vout = Carve::at<uint32_t>(base, o_vout);
+  uint8_t *d_head = Carve::at<uint8_t>(base, o_head);
+  int64_t *d_first = Carve::at<int64_t>(base, o_first);
+  int64_t *d_m = Carve::at<int64_t>(base, o_m), *d_off = Carve::at<int64_t>(base, o_off);
+  char *d_sort = Carve::at<char>(base, o_sort), *d_scan = Carve::at<char>(base, o_scan);
   hipLaunchKernelGGL(k_group_keys, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, n, d_kin,
                      d_vin);
   INCA_HIP(hipGetLastError());
-  size_t sort_bytes = 0;
-  INCA_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
-                                     0u, 32u, st));
-  char *d_sort;
-  INCA_HIP(tmp.get(d_sort, sort_bytes));
   INCA_HIP(rocprim::radix_sort_pairs(d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
                                      0u, 32u, st));
   hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, d_vout, n,
                      d_head);
   INCA_HIP(hipGetLastError());
-  int64_t *d_first;
-  INCA_HIP(tmp.get(d_first, n));
   int64_t ng = 0;
-  int rc;
   if ((rc = compact_flags(ctx, d_head, n, d_first, &ng))) return rc;
   if ((rc = ensure(ctx, ctx->d_gh, ctx->gh_cap, (size_t)ng * kGStride))) return rc;
-  int64_t *d_m, *d_off;
-  INCA_HIP(tmp.get(d_m, ng));
-  INCA_HIP(tmp.get(d_off, ng + 1));
   hipLaunchKernelGGL(k_group_header, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_q, stride, d_vout,
                      d_first, ng, n, ctx->d_gh, d_m);
   INCA_HIP(hipGetLastError());
-  size_t scan_bytes = 0;
-  INCA_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_m, d_off, (int64_t)0, (size_t)ng,
-                                   rocprim::plus<int64_t>(), st));
-  char *d_scan;
-  INCA_HIP(tmp.get(d_scan, scan_bytes));
   INCA_HIP(rocprim::exclusive_scan(d_scan, scan_bytes, d_m, d_off, (int64_t)0, (size_t)ng,
                                    rocprim::plus<int64_t>(), st));
   int64_t last[2] = {0, 0};
@@ -382,8 +368,7 @@ int build_groups(inca_ctx *ctx, int precision) {
   INCA_HIP(hipGetLastError());
   int64_t ng64 = ng;
   if (precision == INCA_MIXED && ctx->mixed_tail > 0.0) {
-    unsigned long long *d_h;
-    INCA_HIP(tmp.get(d_h, kMagBins));
+    unsigned long long *d_h = Carve::at<unsigned long long>(base, o_hist);
     std::vector<unsigned long long> h(kMagBins, 0);
     INCA_HIP(hipMemcpyAsync(d_h, h.data(), kMagBins * sizeof(unsigned long long),
                             hipMemcpyHostToDevice, st));
@@ -406,12 +391,10 @@ int build_groups(inca_ctx *ctx, int precision) {
     while (cut < kMagBins && cum + mass[cut] <= budget) cum += mass[cut++];
     const double tau = std::ldexp(1.0, cut - kMagOff);
     // stable partition: heavy groups (FP64) first, then the light ones
-    double *d_new;
-    uint8_t *d_fl;
-    int64_t *d_idx;
-    INCA_HIP(tmp.get(d_new, (size_t)ng * kGStride));
-    INCA_HIP(tmp.get(d_fl, ng));
-    INCA_HIP(tmp.get(d_idx, ng));
+    if ((rc = ensure(ctx, ctx->d_ghtmp, ctx->ghtmp_cap, (size_t)ng * kGStride))) return rc;
+    double *d_new = ctx->d_ghtmp;
+    uint8_t *d_fl = Carve::at<uint8_t>(base, o_fl);
+    int64_t *d_idx = Carve::at<int64_t>(base, o_idx);
     int64_t nh = 0, nl = 0;
     hipLaunchKernelGGL(k_group_split, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_gh, ng, tau, 1, d_fl);
     INCA_HIP(hipGetLastError());
@@ -435,8 +418,7 @@ int build_groups(inca_ctx *ctx, int precision) {
     INCA_HIP(hipGetLastError());
   }
   // operations per point of the FP64 and FP32 ranges (header word 11)
-  double *d_fl2;
-  INCA_HIP(tmp.get(d_fl2, 2));
+  double *d_fl2 = Carve::at<double>(base, o_fl2);
   hipLaunchKernelGGL(k_group_flops, dim3(1), dim3(kT), 0, st, ctx->d_gh, ng, ng64, d_fl2);
   INCA_HIP(hipGetLastError());
   double fl[2] = {0.0, 0.0};
