@@ -155,6 +155,8 @@ __global__ __launch_bounds__(kT) void k_group_offsets(const int64_t *__restrict_
 // wave (their loads in flight together), then read from LDS; `lds` holds
 // kStage * stride doubles per wave.
 constexpr int kStage = 32;
+constexpr int kMaxStride = 36;  // s/p/d records: 8 + 3 (8 + 1) doubles, rounded even
+constexpr int kLoads = kStage * kMaxStride / 64;
 __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
                                                      const uint32_t *__restrict__ order,
                                                      const int64_t *__restrict__ first, int64_t ng,
@@ -200,9 +202,22 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
   }
   for (int64_t s0 = m0; s0 < m1; s0 += kStage) {
     const int ns = (int)((m1 - s0) < kStage ? (m1 - s0) : kStage);
-    for (int t = lane; t < ns * stride; t += 64) {
+    // lane j < ns holds the offset of member s0 + j's record; the stage's
+    // ns * stride doubles are then fetched kLoads per lane, all in flight
+    // together, and written to LDS
+    const int64_t myrow = lane < ns ? (int64_t)order[s0 + lane] * stride : 0;
+    double v[kLoads];
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+      const int t = lane + 64 * k;
       const int r = t / stride;
-      lds[t] = q[(int64_t)order[s0 + r] * stride + (t - r * stride)];
+      const int64_t row = __shfl(myrow, r < 64 ? r : 0, 64);
+      v[k] = (r < ns) ? q[row + (t - r * stride)] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+      const int t = lane + 64 * k;
+      if (t < ns * stride) lds[t] = v[k];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -229,12 +244,16 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
 }
 
 // Operations per point summed over the FP64 range [0, ng64) and the rest,
-// in a fixed order (one block).
+// in a fixed order: block b sums its contiguous slice (part[2 b], part[2 b +
+// 1]), then one block adds the slices in order.
+constexpr int kFlopBlocks = 1024;
 __global__ __launch_bounds__(kT) void k_group_flops(const double *__restrict__ gh, int64_t ng,
-                                                    int64_t ng64, double *__restrict__ out) {
+                                                    int64_t ng64, double *__restrict__ part) {
   __shared__ double red[2][kT];
+  const int64_t per = (ng + kFlopBlocks - 1) / kFlopBlocks;
+  const int64_t g0 = (int64_t)blockIdx.x * per, g1 = g0 + per < ng ? g0 + per : ng;
   double s[2] = {0.0, 0.0};
-  for (int64_t g = threadIdx.x; g < ng; g += kT) s[g < ng64 ? 0 : 1] += gh[g * kGStride + 11];
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += kT) s[g < ng64 ? 0 : 1] += gh[g * kGStride + 11];
   red[0][threadIdx.x] = s[0];
   red[1][threadIdx.x] = s[1];
   __syncthreads();
@@ -246,9 +265,20 @@ __global__ __launch_bounds__(kT) void k_group_flops(const double *__restrict__ g
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = red[0][0];
-    out[1] = red[1][0];
+    part[2 * blockIdx.x] = red[0][0];
+    part[2 * blockIdx.x + 1] = red[1][0];
   }
+}
+
+__global__ void k_group_flops_total(const double *__restrict__ part, double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < kFlopBlocks; ++i) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  out[0] = a;
+  out[1] = b;
 }
 
 constexpr int kMagBins = 2100;  // ilogb in [-1075, 1024]
@@ -309,7 +339,9 @@ int build_groups(inca_ctx *ctx, int precision) {
   const int64_t n = ctx->q2;
   // total degree of a group at most 8 (s, p and d primitives): at most 165
   // coefficients, which k_group_coeffs covers in three passes of a wave
-  if (!ctx->grouping || n == 0 || n > (int64_t)UINT32_MAX || 4 * ctx->lmax_prim > 8) return 0;
+  if (!ctx->grouping || n == 0 || n > (int64_t)UINT32_MAX || 4 * ctx->lmax_prim > 8 ||
+      ctx->qstride > kMaxStride)
+    return 0;
   const int stride = ctx->qstride;
   // temporaries carved from the context's grouping scratch (kept between
   // preparations, so that small molecules pay no allocation)
@@ -327,7 +359,8 @@ int build_groups(inca_ctx *ctx, int precision) {
                o_head = cv.add<uint8_t>(n), o_first = cv.add<int64_t>(n),
                o_m = cv.add<int64_t>(n), o_off = cv.add<int64_t>(n + 1),
                o_sort = cv.add<char>((int64_t)sort_bytes), o_scan = cv.add<char>((int64_t)scan_bytes),
-               o_hist = cv.add<unsigned long long>(kMagBins), o_fl2 = cv.add<double>(2),
+               o_hist = cv.add<unsigned long long>(kMagBins),
+               o_fl2 = cv.add<double>(2 + 2 * kFlopBlocks),
                o_fl = cv.add<uint8_t>(n), o_idx = cv.add<int64_t>(n);
   int rc;
   if ((rc = ensure(ctx, ctx->d_gscr, ctx->gscr_cap, cv.size))) return rc;
@@ -419,7 +452,9 @@ int build_groups(inca_ctx *ctx, int precision) {
   }
   // operations per point of the FP64 and FP32 ranges (header word 11)
   double *d_fl2 = Carve::at<double>(base, o_fl2);
-  hipLaunchKernelGGL(k_group_flops, dim3(1), dim3(kT), 0, st, ctx->d_gh, ng, ng64, d_fl2);
+  hipLaunchKernelGGL(k_group_flops, dim3(kFlopBlocks), dim3(kT), 0, st, ctx->d_gh, ng, ng64,
+                     d_fl2 + 2);
+  hipLaunchKernelGGL(k_group_flops_total, dim3(1), dim3(64), 0, st, d_fl2 + 2, d_fl2);
   INCA_HIP(hipGetLastError());
   double fl[2] = {0.0, 0.0};
   INCA_HIP(hipMemcpyAsync(fl, d_fl2, sizeof(fl), hipMemcpyDeviceToHost, st));
