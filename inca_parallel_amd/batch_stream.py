@@ -30,15 +30,21 @@ from inca_parallel_amd import capi
 
 
 def prepared_bytes(engine, counts):
-    """Device bytes a molecule adds to a batch arena: its prepared quartets,
-    its grid (points, weights and maps), and the batch's per-molecule
-    scratch.  The scratch grows with quartets x points: one partial row of G
-    doubles per chunk of at most 4096 quartets, and one 64-bit skip word per
-    64 quartets and 64-point wave tile at one point per lane (the autotune's
-    smallest tile), together 8 / 2048 bytes per evaluation."""
-    stride = engine.prepared(0, 0).shape[1] if counts[2] else 0
-    evals = float(counts[2]) * engine.G
-    return int(counts[2]) * stride * 8 + engine.G * 8 * 8 + int(evals * 8 / 2048)
+    """Device bytes a molecule adds to a batch arena: its prepared quartets
+    (shell-quartet group headers of 96 bytes and their coefficients, FP64
+    and FP32, when the context grouped them; else the records), its grid
+    (points, weights and maps), and the batch's per-molecule scratch.  The
+    scratch grows with items x points: one partial row of G doubles per
+    chunk of at most 4096 items, and one 64-bit skip word per 64 items and
+    64-point wave tile at one point per lane (the autotune's smallest tile),
+    together 8 / 2048 bytes per (item, point)."""
+    ng, _, ncoef = engine.groups()
+    if ng > 0:
+        items, stored = ng, ng * 96 + ncoef * 12
+    else:
+        stride = engine.prepared(0, 0).shape[1] if counts[2] else 0
+        items, stored = int(counts[2]), int(counts[2]) * stride * 8
+    return stored + engine.G * 8 * 8 + int(float(items) * engine.G * 8 / 2048)
 
 
 def prepare_molecule(s, nrad, nang, precision, thresh=1e-12, exact=False):
@@ -188,3 +194,66 @@ def config4_by_cost(count):
         npair = len(alpha) * (len(alpha) + 1) // 2
         cost.append((npair * (npair + 1) // 2 * len(cent) * nrad * nang, seed))
     return sorted(sd for _, sd in sorted(cost)[:count])
+
+
+def config4_by_rank(lo, hi):
+    """Seeds of the batched workload ranked lo..hi-1 by records x grid
+    points (cheapest first), in rank order."""
+    from inca_parallel_amd import synth
+    cost = []
+    for seed in synth.CONFIG4_SEEDS:
+        _, el, coords, alpha, _, _, _, _, nrad, nang = synth._config4_layout(seed)
+        an = np.array([synth.ZNUM[x] for x in el], np.int32)
+        cent, _, _ = capi.autogrid(an, coords, nohydro=True)
+        npair = len(alpha) * (len(alpha) + 1) // 2
+        cost.append((npair * (npair + 1) // 2 * len(cent) * nrad * nang, seed))
+    return [sd for _, sd in sorted(cost)[lo:hi]]
+
+
+def main(argv=None):
+    """Run molecules of the batched workload (ranks lo:hi by cost) through
+    the stream in FP64 and mixed precision, every `single`-th of them also
+    one context at a time, and write the per-molecule and per-group results
+    as JSON."""
+    import argparse
+    import json
+    import sys
+    from inca_parallel_amd import synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rank", default="0:32")
+    ap.add_argument("--single-every", type=int, default=8)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    import torch
+    torch.cuda.init()  # torch's HIP runtime before the library's first context
+    lo, hi = (int(v) for v in a.rank.split(":"))
+    seeds = config4_by_rank(lo, hi)
+    mols = []
+    for sd in seeds:
+        s, nrad, nang = synth.config4_molecule(sd)
+        mols.append(("seed%d" % sd, s, nrad, nang))
+    single = {m[0] for k, m in enumerate(mols) if a.single_every and k % a.single_every == 0}
+
+    def log(*x):
+        print(*x, file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    r64, g64 = evaluate_stream(mols, capi.INCA_FP64, single=single, log=log)
+    t1 = time.time()
+    rmx, gmx = evaluate_stream(mols, capi.INCA_MIXED, replays=1, log=log)
+    t2 = time.time()
+    keep = ("name", "atoms", "nprim", "nelec", "nrad", "nang", "points", "counts",
+            "reference_evals", "npairs", "analytic", "integral", "vee", "sum_rule_err", "group",
+            "single_fp64", "single_kernel_ms")
+    out = dict(rank=[lo, hi], seconds_fp64=t1 - t0, seconds_mixed=t2 - t1,
+               fp64=[{k: m[k] for k in keep if k in m} for m in r64],
+               mixed=[{k: m[k] for k in keep if k in m} for m in rmx],
+               groups_fp64=g64, groups_mixed=gmx)
+    with open(a.out, "w") as f:
+        json.dump(out, f)
+    log("done: %d molecules, FP64 %.0f s, mixed %.0f s" % (len(mols), t1 - t0, t2 - t1))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -894,9 +894,8 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
 // ctx->d_gh) in `nch` chunks of `gch` groups, P points per lane; partial
 // sums to partial[chunk][g].  With the cutoff, the skip mask also counts the
 // FP64 operations the launch executes (header word 11) into ctx->d_flops.
-int launch_geval(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, int64_t gch,
-                 int64_t nch, double *partial) {
-  if (P != 1 && P != 2 && P != 6 && P != 8) P = 4;
+int launch_geval_part(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, int64_t gch,
+                      int64_t nch, double *partial) {
   const int64_t G = ctx->G;
   const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
   const int xcd = nch >= kXcds;
@@ -956,6 +955,31 @@ int launch_geval(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, 
 #undef INCA_GEVAL
 #undef INCA_GEVAL2
   INCA_HIP(hipGetLastError());
+  return 0;
+}
+
+// The skip mask of a launch holds one word per 64 groups and wave tile.
+// Where that passes kMaskBudget bytes (large molecules on large grids), the
+// chunks run in consecutive parts, each with the mask of its own groups
+// and writing its own rows of the partial sums.
+constexpr size_t kMaskBudget = (size_t)2 << 30;
+int launch_geval(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, int64_t gch,
+                 int64_t nch, double *partial) {
+  if (P != 1 && P != 2 && P != 6 && P != 8) P = 4;
+  const int64_t G = ctx->G;
+  const int64_t ntile = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P) * (kT / 64);
+  const size_t per_chunk = (size_t)((gch + 63) / 64) * (size_t)ntile * sizeof(unsigned long long);
+  int64_t cpp = nch;  // chunks per part
+  if (ctx->cut_log2 > 0.0 && per_chunk * (size_t)nch > kMaskBudget)
+    cpp = std::max<int64_t>(1, (int64_t)(kMaskBudget / std::max<size_t>(per_chunk, 1)));
+  for (int64_t c0 = 0; c0 < nch; c0 += cpp) {
+    const int64_t c1 = std::min(nch, c0 + cpp);
+    const int64_t g0 = c0 * gch, g1 = std::min(n, c1 * gch);
+    if (g1 <= g0) break;
+    int rc = launch_geval_part(ctx, mixed, P, gh + g0 * kGStride, g1 - g0, gch, c1 - c0,
+                               partial + c0 * G);
+    if (rc) return rc;
+  }
   return 0;
 }
 

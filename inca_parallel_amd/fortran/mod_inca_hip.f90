@@ -102,6 +102,17 @@ module inca_hip
          type(c_ptr), value :: ctx
          real(c_double), value :: log2_rel
       end function
+      integer(c_int) function inca_set_grouping(ctx, on) bind(C, name="inca_set_grouping")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: on
+      end function
+      integer(c_int) function inca_get_groups(ctx, ngroups, ngroups_fp64, ncoef) &
+            bind(C, name="inca_get_groups")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), intent(out) :: ngroups, ngroups_fp64, ncoef
+      end function
       integer(c_int) function inca_get_eval_counts(ctx, reference, executed) &
             bind(C, name="inca_get_eval_counts")
          import :: c_int, c_ptr, c_int64_t

@@ -2,11 +2,12 @@
 !! src/mod_input_reader.f90: the blocks &files, &intracule_job and &cube_job
 !! (each optional; a syntax error stops the run), the same defaults, the
 !! calc_type dispatch and the default output names derived from the input
-!! file name.  &intracule_job also accepts four engine options of this
+!! file name.  &intracule_job also accepts engine options of this
 !! build: engine_precision ('fp64' or 'mixed'), mixed_tail,
 !! engine_polynomials ('reference' fits or 'exact' expansion),
-!! point_cutoff (log2 of the per-point cutoff, 0 = evaluate every term) and
-!! learned_grid_weights (a .npy of the learned autogrid network: predicted
+!! point_cutoff (log2 of the per-point cutoff, 0 = evaluate every term),
+!! engine_grouping (.false.: evaluate record by record instead of by
+!! shell-quartet groups) and learned_grid_weights (a .npy of the learned autogrid network: predicted
 !! radii and per-centre orders for the automatic multicentre grid, checked
 !! against the sum rule with the plain autogrid as fallback).
 module input_reader
@@ -47,7 +48,8 @@ contains
                                nang, nquad_in, cent_in, nradc_in, nangc_in, sfalpha_in, Ps_in, &
                                nblock, scan_start, scan_end, scan_step, scan_nang, center_i, &
                                step_i, np_i, engine_precision, mixed_tail, &
-                               engine_polynomials, point_cutoff, learned_grid_weights
+                               engine_polynomials, point_cutoff, engine_grouping, &
+                               learned_grid_weights
       namelist /cube_job/ cube, denscube, primcube, aocube, mocube, gradient, laplacian, npr, &
                           cao, mo, center, step, np
 

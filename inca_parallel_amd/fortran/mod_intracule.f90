@@ -17,6 +17,7 @@ module intracule
    integer(c_int64_t), private :: counts(4) = 0
    double precision, private :: kern_ms(3) = 0.d0
    integer(c_int64_t), private :: evals_ref = 0, evals_exec = 0
+   integer(c_int64_t), private :: ngroups = 0, ngroups64 = 0, ncoef = 0
    character(len=200), private :: grid_note = ''
 
 contains
@@ -94,6 +95,8 @@ contains
       call inca_check(inca_set_polynomials(inca_ctx, merge(1_c_int32_t, 0_c_int32_t, &
                       exact_polynomials)), "inca_set_polynomials")
       call inca_check(inca_set_point_cutoff(inca_ctx, point_cutoff), "inca_set_point_cutoff")
+      call inca_check(inca_set_grouping(inca_ctx, merge(1_c_int32_t, 0_c_int32_t, &
+                      engine_grouping)), "inca_set_grouping")
       write(*,*) "Starting GPU evaluation over primitive quartets..."
 
       t3 = wall()
@@ -229,13 +232,25 @@ contains
       else
          write(iout,*) "Axis polynomials: Gauss-Hermite + Vandermonde fits (reference)"
       end if
-      if (point_cutoff > 0.d0) then
+      call inca_check(inca_get_groups(inca_ctx, ngroups, ngroups64, ncoef), "inca_get_groups")
+      if (ngroups > 0) then
+         write(iout,'(A, I0, A, I0, A)') " Evaluation: by shell-quartet groups (", ngroups, &
+              " groups, ", ncoef, " coefficients)"
+      else
+         write(iout,*) "Evaluation: record by record"
+      end if
+      if (point_cutoff > 0.d0 .and. ngroups > 0) then
+         write(iout,'(A, F6.1, A, I0, A, I0)') " Per-point cutoff: 2^-", point_cutoff, &
+              " of each term's peak; group-point pairs executed ", evals_exec, &
+              " for record-point evaluations ", evals_ref
+      else if (point_cutoff > 0.d0) then
          write(iout,'(A, F6.1, A, I0, A, I0)') " Per-point cutoff: 2^-", point_cutoff, &
               " of each term's peak; evaluations executed ", evals_exec, " of ", evals_ref
       else
          write(iout,*) "Per-point cutoff: off (every term at every point)"
       end if
-      if (len_trim(grid_note) > 0) write(iout,*) trim(grid_note)
+      ! an explicit format: list-directed output would wrap the long line
+      if (len_trim(grid_note) > 0) write(iout,'(1X,A)') trim(grid_note)
       write(iout,*) "=========================================================="
       close(iout)
 

@@ -68,6 +68,7 @@ module intrainfo
    double precision :: mixed_tail = 0.02d0
    logical :: exact_polynomials = .false.
    double precision :: point_cutoff = 100.0d0   ! log2 of the per-point cutoff; 0 = off
+   logical :: engine_grouping = .true.          ! evaluate by shell-quartet groups
    character(len=256) :: learned_grid_weights = ''  ! .npy of the learned autogrid network
    logical :: learned_grid_used = .false.
 end module intrainfo
