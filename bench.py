@@ -89,6 +89,32 @@ def cpu_baseline(sysm, rec_host, grid_r, seconds_target=15.0):
                            n2, n_total, int(cnt[2]), grid_r.shape[0], t))
 
 
+def record_loop(capi, eng, ref, evals):
+    """The same FP64 step record by record (the reference's loop order: one
+    exponential and three axis polynomials per record and point; ray tiles
+    on a radial scan), for comparison with the grouped step."""
+    eng.set_grouping(0)
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    rec_tiles = eng.autotune()
+    t0 = time.perf_counter()
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    rec_step = time.perf_counter() - t0
+    rec_kern = eng.timings()
+    rec_rintra = eng.radial_values()
+    hist = eng.class_histogram(64)
+    eng.set_grouping(1)
+    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+    big = np.abs(ref) >= 1e-6 * np.abs(ref).max()
+    return {
+        "note": "the same step record by record (the reference's loop order; ray tiles)",
+        "step_s": rec_step, "evals_per_s": evals / rec_step,
+        "kernel_ms": {"prep": float(rec_kern[0]), "eval": float(rec_kern[1])},
+        "points_per_lane": rec_tiles[0], "flop_per_eval": flops_per_eval(hist),
+        "max_rel_err_rintra_vs_groups": float(np.max(np.abs(rec_rintra - ref)[big] /
+                                                     np.abs(ref)[big])),
+    }
+
+
 def batch_bench(capi, synth, torch, count=32):
     """Batched molecules (the batched workload: seeded molecules of 4-20
     atoms, 30-300 s/p/d primitives, each on its own autogrid with nrad
@@ -347,6 +373,8 @@ def main():
                     help="molecules of the batched workload in the batch leg (cheapest first; "
                          "256 is the whole workload, several hours)")
     ap.add_argument("--no-cube", action="store_true")
+    ap.add_argument("--no-record-loop", action="store_true",
+                    help="skip the record-by-record comparison step (profiling runs)")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the learned-grid + mixed + batched leg against the plain FP64 kernel")
     ap.add_argument("--tiles", default=None,
@@ -433,19 +461,9 @@ def main():
     mixed_err = float(np.max(np.abs(rm["rintra"] - ref) / np.maximum(np.abs(ref), 1e-300)))
     # the record loop (the reference's order: one exponential and three axis
     # polynomials per record and point; ray tiles on this scan) for comparison
-    eng.set_grouping(0)
-    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
-    rec_tiles = eng.autotune()
-    t0 = time.perf_counter()
-    _, cnt_r = eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
-    rec_step = time.perf_counter() - t0
-    rec_kern = eng.timings()
-    rec_rintra = eng.radial_values()
-    hist = eng.class_histogram(64)
-    eng.set_grouping(1)
-    eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
-    rec_err = float(np.max(np.abs(rec_rintra - ref)[np.abs(ref) >= 1e-6 * np.abs(ref).max()] /
-                           np.abs(ref)[np.abs(ref) >= 1e-6 * np.abs(ref).max()]))
+    rec = None
+    if not a.no_record_loop:
+        rec = record_loop(capi, eng, ref, evals)
 
     out = {
         "metric": "quartet x gridpoint evals/s; time to 1e-8 rel integral, mixed vs FP64 GPU",
@@ -483,13 +501,7 @@ def main():
                     "bytes of the group kernel per step from rocprofv3 FETCH_SIZE + WRITE_SIZE (%s)"
                     % (eval_s * 1e3, os.path.relpath(PROFILE, ROOT)),
         },
-        "record_loop": {
-            "note": "the same step record by record (the reference's loop order; ray tiles)",
-            "step_s": rec_step, "evals_per_s": evals / rec_step,
-            "kernel_ms": {"prep": float(rec_kern[0]), "eval": float(rec_kern[1])},
-            "points_per_lane": rec_tiles[0], "flop_per_eval": flops_per_eval(hist),
-            "max_rel_err_rintra_vs_groups": rec_err,
-        },
+        "record_loop": rec,
         "mixed": {
             "value": evals / rm["step_s"], "unit": "evals/s",
             "speedup_vs_fp64": r64["step_s"] / rm["step_s"],

@@ -57,6 +57,9 @@ def prepare_molecule(s, nrad, nang, precision, thresh=1e-12, exact=False):
     e.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
     if exact:
         e.set_polynomials(True)
+    # the groups alone, built chunk by chunk: no molecule holds all its
+    # prepared records at once (the batch and the solo path take groups)
+    e.set_keep_records(0)
     cent, Ps, sf = capi.autogrid(p["an"], p["cartes"], nohydro=True)
     nq = len(Ps)
     e.grid_multicenter(cent, [nrad] * nq, [nang] * nq, sf, Ps, nosym=True)
@@ -164,7 +167,8 @@ def evaluate_stream(molecules, precision=capi.INCA_FP64, arena_bytes=64 << 30, t
         if log:
             log("prepared %s: %d atoms, %d primitives, %d quartets, %d points"
                 % (name, m["atoms"], m["nprim"], cnt[2], m["points"]))
-        # only the grid and the prepared quartets stay on the device
+        # only the grid and the prepared quartets (their groups, when the
+        # context made them) stay on the device
         e.release_scratch()
         if need > (arena_bytes if solo_bytes is None else solo_bytes):
             # larger than a whole arena: the context's own grid loop (rows

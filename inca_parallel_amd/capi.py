@@ -29,6 +29,7 @@ SYMBOLS = ["inca_create", "inca_destroy", "inca_last_error", "inca_set_primitive
            "inca_set_polynomials", "inca_get_analytic_integral", "inca_set_point_cutoff",
            "inca_set_cube_contraction",
            "inca_get_eval_counts", "inca_set_grouping", "inca_get_groups", "inca_get_eval_flops",
+           "inca_release_records", "inca_set_keep_records",
            "inca_prepare", "inca_evaluate_prepared", "inca_evaluate_prepared_noalloc", "inca_autogrid", "inca_bragg_slater", "inca_onee", "inca_batch_create", "inca_batch_destroy",
            "inca_batch_last_error", "inca_batch_evaluate", "inca_batch_get_ivec",
            "inca_batch_autotune", "inca_batch_set_points", "inca_batch_info",
@@ -91,6 +92,8 @@ def load():
     L.inca_set_grouping.argtypes = [vp, C.c_int32]
     L.inca_get_groups.argtypes = [vp, vp, vp, vp]
     L.inca_get_eval_flops.argtypes = [vp, vp]
+    L.inca_release_records.argtypes = [vp]
+    L.inca_set_keep_records.argtypes = [vp, C.c_int32]
     L.inca_set_cube_contraction.argtypes = [vp, C.c_int32]
     L.inca_get_eval_counts.argtypes = [vp, vp, vp]
     L.inca_prepare.argtypes = [vp, C.c_int32, C.c_double, vp]
@@ -548,6 +551,16 @@ class Engine:
     def release_scratch(self):
         """Free the records and scratch, keeping grid and prepared quartets."""
         self._ck(self._L.inca_release_scratch(self._h), "inca_release_scratch")
+
+    def release_records(self):
+        """Free the prepared records of a grouped preparation (only the group
+        evaluation remains; inca_release_records)."""
+        self._ck(self._L.inca_release_records(self._h), "inca_release_records")
+
+    def set_keep_records(self, keep):
+        """0: build the groups chunk by chunk and keep no prepared records
+        (inca_set_keep_records)."""
+        self._ck(self._L.inca_set_keep_records(self._h, int(bool(keep))), "inca_set_keep_records")
 
     def set_prepare_chunk(self, records):
         """Records per pre-pass chunk (inca_set_prepare_chunk)."""

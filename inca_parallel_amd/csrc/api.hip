@@ -477,6 +477,26 @@ int inca_release_scratch(inca_ctx *ctx) {
   return 0;
 }
 
+int inca_set_keep_records(inca_ctx *ctx, int32_t keep) {
+  CHECK_CTX(ctx);
+  if (keep != 0 && keep != 1) return set_err(ctx, INCA_EINVAL, "keep_records must be 0 or 1");
+  ctx->keep_records = keep;
+  return 0;
+}
+
+int inca_release_records(inca_ctx *ctx) {
+  CHECK_CTX(ctx);
+  if (ctx->ng == 0 || !ctx->grouping)
+    return set_err(ctx, INCA_EINVAL, "no shell-quartet groups: the prepared records are needed");
+  INCA_HIP(hipSetDevice(ctx->device));
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->d_q) (void)hipFree(ctx->d_q);
+  ctx->d_q = nullptr;
+  ctx->q_cap = 0;
+  ctx->rec_split_pending = 0;
+  return 0;
+}
+
 int inca_set_prepare_chunk(inca_ctx *ctx, int64_t records) {
   CHECK_CTX(ctx);
   if (records < 1024) return set_err(ctx, INCA_EINVAL, "pre-pass chunk below 1024 records");
@@ -587,6 +607,7 @@ int inca_get_prepared(inca_ctx *ctx, int64_t first, int64_t n, double *out, int3
     return set_err(ctx, INCA_EINVAL, "prepared-record range out of bounds");
   if (n > 0 && out) {
     INCA_HIP(hipSetDevice(ctx->device));
+    if (int rc = need_records(ctx)) return rc;
     if (int rc = split_records(ctx)) return rc;  // the FP64 range first
   }
   if (n > 0 && out)

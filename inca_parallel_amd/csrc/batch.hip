@@ -568,8 +568,9 @@ int setup(inca_batch *b, int nmol, inca_ctx *const *mols) {
     // shell-quartet groups where the context has them, else records (a
     // mixed preparation's record split is made now)
     b->grouped[m] = c->grouping && c->ng > 0;
-    if (!b->grouped[m] && c->device == b->device && split_records(mols[m]))
-      return bfail(b, INCA_EHIP, "molecule " + std::to_string(m) + ": " + c->err);
+    if (!b->grouped[m] && c->device == b->device &&
+        (need_records(mols[m]) || split_records(mols[m])))
+      return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + ": " + c->err);
     if (c->device != b->device)
       return bfail(b, INCA_EINVAL, "molecule " + std::to_string(m) + " is on another device");
     if (c->grid_kind == GRID_NONE)

@@ -122,6 +122,7 @@ struct inca_ctx {
   // shell-quartet groups of the prepared records (group.hip): headers of
   // kGStride doubles (FP64 range first), one coefficient pool, its FP32 copy
   int grouping = 1;            // evaluate by groups where the grid loop allows
+  int keep_records = 1;        // 0: a grouped preparation keeps only its groups
   int64_t ng = 0, ng_fp64 = 0; // groups, of them evaluated in FP64
   double *d_gh = nullptr;
   size_t gh_cap = 0;
@@ -134,6 +135,7 @@ struct inca_ctx {
   double flops_executed = 0.0;                // of the last grid loop (group evaluation)
   int grouped_last = 0;                       // the last grid loop ran by groups
   int gtile_p[2] = {4, 4};                    // points per lane of the group kernels
+  double grho_h = 0.0;                        // ray step whose rho the group headers hold
   char *d_gscr = nullptr;                     // grouping scratch (sort, scan, maps)
   size_t gscr_cap = 0;
   double *d_ghtmp = nullptr;                  // headers during the mixed split
@@ -255,8 +257,15 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision,
                      int64_t *counts);
 // the records' FP64/FP32 split of a mixed preparation, made on first need
 int split_records(inca_ctx *ctx);
-// group.hip
+// INCA_EINVAL when the prepared records were released (inca_release_records)
+int need_records(inca_ctx *ctx);
+// group.hip: groups of all prepared records (build_groups), or built chunk
+// by chunk (reset_groups, group_rows per chunk, finish_groups)
 int build_groups(inca_ctx *ctx, int precision);
+bool groupable(const inca_ctx *ctx);
+void reset_groups(inca_ctx *ctx);
+int group_rows(inca_ctx *ctx, const double *q, int64_t n);
+int finish_groups(inca_ctx *ctx, int precision);
 
 // autogrid.hip
 double bragg_slater_bohr(int z);

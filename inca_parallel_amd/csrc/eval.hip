@@ -118,6 +118,56 @@ __global__ __launch_bounds__(kT) void k_geval(const double *__restrict__ gh, int
   }
 }
 
+// Shell-quartet groups on the ray tiles of a radial scan (FP64): the lanes
+// of k_eval_rays (one direction at P consecutive radii), each group by
+// group_ray.
+template <int P, bool CUT>
+__global__ __launch_bounds__(kT) void k_geval_rays(const double *__restrict__ gh, int64_t ng,
+                                                   int64_t gchunk, const double *__restrict__ pool,
+                                                   const double *__restrict__ gx,
+                                                   const double *__restrict__ gy,
+                                                   const double *__restrict__ gz, int64_t G,
+                                                   const double *__restrict__ tb,
+                                                   const unsigned long long *__restrict__ mask,
+                                                   int64_t ntile, int64_t tiles, int64_t nch,
+                                                   int xcd, const int32_t *__restrict__ idx,
+                                                   double h, double *__restrict__ partial) {
+  const BlockTask task = block_task(tiles, xcd);
+  if (task.chunk >= nch) return;  // padding block of the XCD order (whole block)
+  const int64_t tile = task.tile_block * (kT / 64) +
+                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t base = tile * (64 * P) + (threadIdx.x & 63);
+  const bool empty = !(tb[tile * kTB + 5] > 0.0);
+  __shared__ double s_exp2[kExpTable];
+  stage_exp_table(s_exp2);
+  const int32_t g0 = idx[base];
+  double o[3] = {0.0, 0.0, 0.0}, hw[3] = {0.0, 0.0, 0.0};
+  if (g0 >= 0) {
+    o[0] = gx[g0];
+    o[1] = gy[g0];
+    o[2] = gz[g0];
+    const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
+    const double f = r0 > 0.0 ? h / r0 : 0.0;
+    hw[0] = f * o[0];
+    hw[1] = f * o[1];
+    hw[2] = f * o[2];
+  }
+  const double span = (P - 1) * h;
+  double acc[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) acc[k] = 0.0;
+  const int64_t q0 = task.chunk * gchunk;
+  const int64_t q1 = empty ? q0 : (q0 + gchunk < ng ? q0 + gchunk : ng);
+  for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t gi) {
+    group_ray<P>(gh + gi * kGStride, pool, o, hw, span, h * h, acc, s_exp2);
+  });
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int32_t g = idx[base + k * 64];
+    if (g >= 0) partial[task.chunk * G + g] = acc[k];
+  }
+}
+
 // Ray tiles: lane `lane` of tile t holds the points idx[t][k][lane], k < P
 // (-1: none), equally spaced by h w from its first point o (or o alone):
 // one direction w = o / |o| at consecutive radii of a radial scan (kind 1),
@@ -575,6 +625,7 @@ __global__ void k_zero(double *p, int64_t n) {
 }  // namespace
 
 int class_histogram(inca_ctx *ctx, int nbins, int64_t *hist) {
+  if (int rc = need_records(ctx)) return rc;
   if (nbins > 64) nbins = 64;
   unsigned long long *d_h = nullptr;
   INCA_HIP(hipMalloc((void **)&d_h, nbins * sizeof(unsigned long long)));
@@ -890,6 +941,34 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
   return 0;
 }
 
+// Wave tiles of the group kernels: the ray tiles of a radial scan for the
+// FP64 kernel at P >= 2 points per lane (the record kernel's plan), plain
+// tiles of 64 P consecutive points otherwise.
+int group_tiles(inca_ctx *ctx, bool mixed, int P, int64_t *tiles, bool *rays) {
+  *rays = false;
+  *tiles = (ctx->G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  if (mixed || P < 2 || ctx->grid_kind != GRID_RADIAL) return 0;
+  int rc;
+  if ((rc = ray_plan(ctx, P))) return rc;
+  if (ctx->ray_P != P || ctx->ray_kind != 1) return 0;
+  *rays = true;
+  *tiles = ctx->ray_ntile / 4;
+  return 0;
+}
+
+// rho = exp(-2 sqe^2 h^2) of every group header (word 7) for the scan's
+// ray step h, once per step.
+int ensure_grho(inca_ctx *ctx) {
+  int rc;
+  if ((rc = ray_analyse(ctx))) return rc;
+  if (ctx->ray_h <= 0.0 || ctx->ng == 0 || ctx->grho_h == ctx->ray_h) return 0;
+  hipLaunchKernelGGL(k_ray_rho, dim3((unsigned)((ctx->ng + kT - 1) / kT)), dim3(kT), 0,
+                     ctx->stream, ctx->d_gh, kGStride, ctx->ng, ctx->ray_h * ctx->ray_h);
+  INCA_HIP(hipGetLastError());
+  ctx->grho_h = ctx->ray_h;
+  return 0;
+}
+
 // Launch the group kernel over groups [g0, g0 + n) (the headers at
 // ctx->d_gh) in `nch` chunks of `gch` groups, P points per lane; partial
 // sums to partial[chunk][g].  With the cutoff, the skip mask also counts the
@@ -897,14 +976,17 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
 int launch_geval_part(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_t n, int64_t gch,
                       int64_t nch, double *partial) {
   const int64_t G = ctx->G;
-  const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+  int64_t tiles;
+  bool rays;
+  int rc;
+  if ((rc = group_tiles(ctx, mixed, P, &tiles, &rays))) return rc;
+  const int32_t *ridx = rays ? ctx->d_ridx : nullptr;
   const int xcd = nch >= kXcds;
   const int64_t nblocks = tiles * (xcd ? (nch + kXcds - 1) / kXcds * kXcds : nch);
   dim3 grid((unsigned)nblocks);
   hipStream_t st = ctx->stream;
   const bool cut = ctx->cut_log2 > 0.0;
   const int64_t ntile = tiles * (kT / 64);
-  int rc;
   if ((rc = ensure(ctx, ctx->d_tb, ctx->tb_cap, (size_t)ntile * kTB))) return rc;
   if ((rc = ensure(ctx, ctx->d_evals, ctx->evals_cap, 1))) return rc;
   if ((rc = ensure(ctx, ctx->d_flops, ctx->flops_cap, 1))) return rc;
@@ -912,7 +994,7 @@ int launch_geval_part(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_
     const int mode = ctx->grid_kind == GRID_RADIAL ? 0 : (ctx->grid_kind == GRID_MULTI ? 1 : 2);
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)tiles), dim3(kT), 0, st, ctx->d_gx,
                        ctx->d_gy, ctx->d_gz, G, P, mode, ctx->d_map, ctx->d_gcent, ntile,
-                       ctx->d_tb, (const int32_t *)nullptr);
+                       ctx->d_tb, ridx);
     const int64_t nwords = (n + 63) / 64;
     const size_t nw = (size_t)(nwords > 0 ? nwords : 1);
     if ((rc = ensure(ctx, ctx->d_mask, ctx->mask_cap, nw * ntile))) return rc;
@@ -931,7 +1013,12 @@ int launch_geval_part(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_
   }
   INCA_HIP(hipGetLastError());
 #define INCA_GEVAL2(PP, CC)                                                                    \
-  if (mixed)                                                                                   \
+  if (rays) {                                                                                  \
+    if constexpr (PP >= 2)                                                                     \
+      hipLaunchKernelGGL((k_geval_rays<PP, CC>), grid, dim3(kT), 0, st, gh, n, gch,            \
+                         ctx->d_gpool, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,          \
+                         ctx->d_mask, ntile, tiles, nch, xcd, ridx, ctx->ray_h, partial);      \
+  } else if (mixed)                                                                            \
     hipLaunchKernelGGL((k_geval<PP, CC, true>), grid, dim3(kT), 0, st, gh, n, gch,             \
                        ctx->d_gpool, ctx->d_gpool32, ctx->d_gx, ctx->d_gy, ctx->d_gz, G,       \
                        ctx->d_tb, ctx->d_mask, ntile, tiles, nch, xcd, partial);               \
@@ -1150,6 +1237,7 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
   // not take); FP64 groups first, then the FP32 range of a mixed preparation
   const bool grouped = !contracted && ctx->grouping && ctx->ng > 0;
   ctx->grouped_last = grouped;
+  if (!grouped && (rc = need_records(ctx))) return rc;
   if (precision == INCA_MIXED && !grouped && (rc = split_records(ctx))) return rc;
   int64_t nchunks = 0;
   bool rays_used = false;  // a range ran on the ray lists (duplicate points skipped)
@@ -1162,8 +1250,11 @@ int evaluate_grid(inca_ctx *ctx, int precision) {
     int64_t nch[2] = {0, 0}, gch[2] = {1, 1};
     for (int r = 0; r < 2; ++r) {
       if (nr[r] == 0) continue;
-      const int P = ctx->gtile_p[r];
-      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      int64_t tiles;
+      bool rays;
+      if ((rc = group_tiles(ctx, r == 1, ctx->gtile_p[r], &tiles, &rays))) return rc;
+      if (rays && (rc = ensure_grho(ctx))) return rc;
+      rays_used = rays_used || rays;
       chunking(nr[r], tiles, (double)nr[r] / (double)ng, &gch[r], &nch[r]);
     }
     nchunks = nch[0] + nch[1];
@@ -1283,6 +1374,7 @@ int autotune_groups(inca_ctx *ctx, int32_t *best) {
     n = n < 1024 ? 1024 : (n > (1 << 20) ? (1 << 20) : n);
     if (n > nr) n = nr;
     if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)n * kGStride))) break;
+    if (ctx->grid_kind == GRID_RADIAL && (rc = ensure_grho(ctx))) break;  // the sample's word 7
     hipLaunchKernelGGL(k_gather_strided, dim3((unsigned)((n * kGStride + kT - 1) / kT)), dim3(kT),
                        0, ctx->stream, ctx->d_gh + g0 * kGStride, nr, n, kGStride, ctx->d_qtmp);
     INCA_HIP(hipGetLastError());
@@ -1290,7 +1382,9 @@ int autotune_groups(inca_ctx *ctx, int32_t *best) {
     int best_p = 4;
     for (int c = 0; c < (int)(sizeof(cand) / sizeof(cand[0])) && !rc; ++c) {
       const int P = cand[c];
-      const int64_t tiles = (G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
+      int64_t tiles;
+      bool rays;
+      if ((rc = group_tiles(ctx, kind == 1, P, &tiles, &rays))) break;
       int64_t qch, nch;
       chunking(nr, tiles, (double)nr / (double)ctx->ng, &qch, &nch);
       nch = (n + qch - 1) / qch;
@@ -1324,6 +1418,7 @@ int autotune(inca_ctx *ctx, int32_t *best) {
   if (G == 0 || nq == 0) return set_err(ctx, INCA_EINVAL, "autotune needs a grid and quartets");
   if (ctx->grouping && ctx->ng > 0 && !(ctx->grid_kind == GRID_CUBE && ctx->cube_contract))
     return autotune_groups(ctx, best);
+  if (int rc = need_records(ctx)) return rc;
   const int cand[] = {1, 2, 4, 6, 8, 10};
   // an evenly strided sample of the prepared quartets (file order groups
   // quartets by primitive, and with the cutoff their cost depends on where
@@ -1460,6 +1555,7 @@ hipError_t noalloc_scratch_init(int device) {
 // tile records and partial rows in `scratch`, reduced in a fixed order into
 // the caller's device array ivec.  Enqueued on `st`; nothing waits.
 int evaluate_noalloc(inca_ctx *ctx, int precision, double *ivec, hipStream_t st) {
+  if (int rc = need_records(ctx)) return rc;
   double *base = noalloc_scratch(ctx->device);
   if (!base) return set_err(ctx, INCA_EINVAL, "no-allocation scratch not set up on device %d",
                             ctx->device);

@@ -647,6 +647,52 @@ __device__ __forceinline__ void group_fp64(const double *__restrict__ gh,
   }
 }
 
+// One group on a ray lane of P points o + k h w (k = 0 .. P-1): rp_k =
+// rp_0 + k d with d = sqe h w, the polynomial at each point, and exp(-s_k)
+// by the two-product recurrence of the record ray kernel (e_{k+1} = e_k q_k,
+// q_{k+1} = q_k rho, rho = exp(-2 sqe^2 h^2) in header word 7), under the
+// same range condition (sqe (P-1) h + rcut <= sqrt(690)); groups beyond it
+// take exp(-s_k) point by point at the same ideal points.
+template <int P>
+__device__ __forceinline__ void group_ray(const double *__restrict__ gh,
+                                          const double *__restrict__ pool, const double (&o)[3],
+                                          const double (&hw)[3], double span, double h2,
+                                          double (&acc)[P], const double *__restrict__ tbl) {
+  const GHeader h = load_gheader(gh);
+  const double rcut = gh[6];
+  const double p0x = fma(h.sqe, o[0], h.ax), p0y = fma(h.sqe, o[1], h.ay),
+               p0z = fma(h.sqe, o[2], h.az);
+  const double dx = h.sqe * hw[0], dy = h.sqe * hw[1], dz = h.sqe * hw[2];
+  double px[P], py[P], pz[P], v[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    px[k] = k == 0 ? p0x : fma((double)k, dx, p0x);
+    py[k] = k == 0 ? p0y : fma((double)k, dy, p0y);
+    pz[k] = k == 0 ? p0z : fma((double)k, dz, p0z);
+  }
+  group_poly_any<P, double, double>(pool + h.off, h, px, py, pz, v);
+  if (!(fma(h.sqe, span, rcut) <= kRayMaxRp)) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double s = fma(px[k], px[k], fma(py[k], py[k], pz[k] * pz[k]));
+      acc[k] = fma(exp_neg(s, tbl), v[k], acc[k]);
+    }
+    return;
+  }
+  const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
+  const double dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
+  const double dd = h.sqe * h.sqe * h2;  // |d|^2 = (sqe h)^2
+  const double rho = gh[7];
+  double e = exp_neg(s0, tbl);
+  double qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    acc[k] = fma(e, v[k], acc[k]);
+    if (k + 1 < P) e *= qk;
+    if (k + 2 < P) qk *= rho;
+  }
+}
+
 // The same in mixed precision: rp in FP64, then FP32 (hardware exp2, the
 // FP32 copy of the coefficients), each term added to the FP64 accumulator.
 template <int P>

@@ -326,22 +326,47 @@ __global__ __launch_bounds__(kT) void k_to_float(const double *__restrict__ x, i
 
 }  // namespace
 
-// Group the prepared records (after prepare_quartets).  Mixed precision:
-// the groups are split by their summed L1 bound the way records are (the
-// lightest to FP32 while their bounds stay within mixed_tail * |sum M|), the
-// FP64 groups first.  Leaves ctx->ng = 0 (the evaluation then runs record
-// by record) when there is nothing to group or more than 2^32 - 1 records.
-int build_groups(inca_ctx *ctx, int precision) {
-  hipStream_t st = ctx->stream;
-  ctx->ng = ctx->ng_fp64 = 0;
-  ctx->gpool_n = 0;
-  ctx->gflop_fp64 = ctx->gflop_fp32 = 0.0;
-  const int64_t n = ctx->q2;
+// Device buffer growth keeping the first `used` elements (device to
+// device), to at least `need` (1.5x the old capacity when larger).
+template <class T>
+int grow_dev(inca_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t used) {
+  if (need <= cap && ptr) return 0;
+  const size_t n = std::max(need, cap + cap / 2);
+  T *p = nullptr;
+  hipError_t e = hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_err(ctx, INCA_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+  }
+  if (used && ptr) INCA_HIP(hipMemcpyAsync(p, ptr, used * sizeof(T), hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+  INCA_HIP(hipStreamSynchronize(ctx->stream));
+  if (ptr) (void)hipFree(ptr);
+  ptr = p;
+  cap = n;
+  return 0;
+}
+
+bool groupable(const inca_ctx *ctx) {
   // total degree of a group at most 8 (s, p and d primitives): at most 165
   // coefficients, which k_group_coeffs covers in three passes of a wave
-  if (!ctx->grouping || n == 0 || n > (int64_t)UINT32_MAX || 4 * ctx->lmax_prim > 8 ||
-      ctx->qstride > kMaxStride)
-    return 0;
+  return ctx->grouping && 4 * ctx->lmax_prim <= 8 && ctx->qstride <= kMaxStride;
+}
+
+void reset_groups(inca_ctx *ctx) {
+  ctx->ng = ctx->ng_fp64 = 0;
+  ctx->gpool_n = 0;
+  ctx->grho_h = 0.0;  // word 7 of the new headers holds no rho yet
+  ctx->gflop_fp64 = ctx->gflop_fp32 = 0.0;
+}
+
+// Groups of the prepared rows q[0..n) (n < 2^32), appended to the
+// context's groups: their headers after the existing ones, their
+// coefficients after the existing pool.
+int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
+  if (n == 0) return 0;
+  if (n > (int64_t)UINT32_MAX) return set_err(ctx, INCA_ERANGE, "grouping pass of 2^32 records or more");
+  hipStream_t st = ctx->stream;
   const int stride = ctx->qstride;
   // temporaries carved from the context's grouping scratch (kept between
   // preparations, so that small molecules pay no allocation)
@@ -358,10 +383,7 @@ int build_groups(inca_ctx *ctx, int precision) {
                o_vin = cv.add<uint32_t>(n), o_vout = cv.add<uint32_t>(n),
                o_head = cv.add<uint8_t>(n), o_first = cv.add<int64_t>(n),
                o_m = cv.add<int64_t>(n), o_off = cv.add<int64_t>(n + 1),
-               o_sort = cv.add<char>((int64_t)sort_bytes), o_scan = cv.add<char>((int64_t)scan_bytes),
-               o_hist = cv.add<unsigned long long>(kMagBins),
-               o_fl2 = cv.add<double>(2 + 2 * kFlopBlocks),
-               o_fl = cv.add<uint8_t>(n), o_idx = cv.add<int64_t>(n);
+               o_sort = cv.add<char>((int64_t)sort_bytes), o_scan = cv.add<char>((int64_t)scan_bytes);
   int rc;
   if ((rc = ensure(ctx, ctx->d_gscr, ctx->gscr_cap, cv.size))) return rc;
   char *base = ctx->d_gscr;
@@ -371,34 +393,57 @@ int build_groups(inca_ctx *ctx, int precision) {
   int64_t *d_first = Carve::at<int64_t>(base, o_first);
   int64_t *d_m = Carve::at<int64_t>(base, o_m), *d_off = Carve::at<int64_t>(base, o_off);
   char *d_sort = Carve::at<char>(base, o_sort), *d_scan = Carve::at<char>(base, o_scan);
-  hipLaunchKernelGGL(k_group_keys, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, n, d_kin,
-                     d_vin);
+  hipLaunchKernelGGL(k_group_keys, dim3(nblk(n)), dim3(kT), 0, st, q, stride, n, d_kin, d_vin);
   INCA_HIP(hipGetLastError());
   INCA_HIP(rocprim::radix_sort_pairs(d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
                                      0u, 32u, st));
-  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, d_vout, n,
-                     d_head);
+  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, q, stride, d_vout, n, d_head);
   INCA_HIP(hipGetLastError());
   int64_t ng = 0;
   if ((rc = compact_flags(ctx, d_head, n, d_first, &ng))) return rc;
-  if ((rc = ensure(ctx, ctx->d_gh, ctx->gh_cap, (size_t)ng * kGStride))) return rc;
-  hipLaunchKernelGGL(k_group_header, dim3(nblk(ng)), dim3(kT), 0, st, ctx->d_q, stride, d_vout,
-                     d_first, ng, n, ctx->d_gh, d_m);
+  const int64_t ng0 = ctx->ng, pool0 = ctx->gpool_n;
+  if ((rc = grow_dev(ctx, ctx->d_gh, ctx->gh_cap, (size_t)(ng0 + ng) * kGStride,
+                     (size_t)ng0 * kGStride)))
+    return rc;
+  double *gh = ctx->d_gh + ng0 * kGStride;
+  hipLaunchKernelGGL(k_group_header, dim3(nblk(ng)), dim3(kT), 0, st, q, stride, d_vout, d_first,
+                     ng, n, gh, d_m);
   INCA_HIP(hipGetLastError());
-  INCA_HIP(rocprim::exclusive_scan(d_scan, scan_bytes, d_m, d_off, (int64_t)0, (size_t)ng,
+  INCA_HIP(rocprim::exclusive_scan(d_scan, scan_bytes, d_m, d_off, pool0, (size_t)ng,
                                    rocprim::plus<int64_t>(), st));
   int64_t last[2] = {0, 0};
   INCA_HIP(hipMemcpyAsync(&last[0], d_off + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   INCA_HIP(hipMemcpyAsync(&last[1], d_m + ng - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   INCA_HIP(hipStreamSynchronize(st));
-  const int64_t npool = last[0] + last[1];
-  hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, ctx->d_gh);
+  const int64_t pool_end = last[0] + last[1];
+  hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, gh);
   INCA_HIP(hipGetLastError());
-  if ((rc = ensure(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)npool))) return rc;
+  if ((rc = grow_dev(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)pool_end, (size_t)pool0))) return rc;
   hipLaunchKernelGGL(k_group_coeffs, dim3((unsigned)((ng + 3) / 4)), dim3(kT),
-                     (kT / 64) * kStage * stride * sizeof(double), st, ctx->d_q, stride, d_vout,
-                     d_first, ng, n, ctx->d_gh, ctx->d_gpool);
+                     (kT / 64) * kStage * stride * sizeof(double), st, q, stride, d_vout,
+                     d_first, ng, n, gh, ctx->d_gpool);
   INCA_HIP(hipGetLastError());
+  ctx->ng = ng0 + ng;
+  ctx->gpool_n = pool_end;
+  return 0;
+}
+
+// After the last group_rows: the precision split of a mixed preparation
+// (the groups by their summed L1 bounds, the way records are split: the
+// lightest to FP32 while their bounds stay within mixed_tail * |sum M|; the
+// FP64 groups first) and the operation counts of the two ranges.
+int finish_groups(inca_ctx *ctx, int precision) {
+  hipStream_t st = ctx->stream;
+  const int64_t ng = ctx->ng, npool = ctx->gpool_n;
+  ctx->ng_fp64 = ng;
+  if (ng == 0) return 0;
+  Carve cv;
+  const size_t o_hist = cv.add<unsigned long long>(kMagBins),
+               o_fl2 = cv.add<double>(2 + 2 * kFlopBlocks), o_fl = cv.add<uint8_t>(ng),
+               o_idx = cv.add<int64_t>(ng);
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_gscr, ctx->gscr_cap, cv.size))) return rc;
+  char *base = ctx->d_gscr;
   int64_t ng64 = ng;
   if (precision == INCA_MIXED && ctx->mixed_tail > 0.0) {
     unsigned long long *d_h = Carve::at<unsigned long long>(base, o_hist);
@@ -459,12 +504,22 @@ int build_groups(inca_ctx *ctx, int precision) {
   double fl[2] = {0.0, 0.0};
   INCA_HIP(hipMemcpyAsync(fl, d_fl2, sizeof(fl), hipMemcpyDeviceToHost, st));
   INCA_HIP(hipStreamSynchronize(st));
-  ctx->ng = ng;
   ctx->ng_fp64 = ng64;
-  ctx->gpool_n = npool;
   ctx->gflop_fp64 = fl[0];
   ctx->gflop_fp32 = fl[1];
   return 0;
+}
+
+
+// Group the prepared records (after prepare_quartets keeps them all).
+// Leaves ctx->ng = 0 (the evaluation then runs record by record) when there
+// is nothing to group, 2^32 records or more, or primitives beyond d.
+int build_groups(inca_ctx *ctx, int precision) {
+  reset_groups(ctx);
+  if (!groupable(ctx) || ctx->q2 == 0 || ctx->q2 > (int64_t)UINT32_MAX) return 0;
+  int rc;
+  if ((rc = group_rows(ctx, ctx->d_q, ctx->q2))) return rc;
+  return finish_groups(ctx, precision);
 }
 
 }  // namespace inca

@@ -842,6 +842,10 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   uint8_t *d_defer = (split && la > 4) ? Carve::at<uint8_t>(ctx->d_tmp, o_def) : nullptr;
   double mom_total = 0.0;  // chunk totals added in file order
   int64_t q2 = 0;
+  // groups without the records (inca_set_keep_records(ctx, 0)): grouped
+  // chunk by chunk, so the prepared rows never exceed one chunk
+  const bool groups_only = !ctx->keep_records && groupable(ctx);
+  if (groups_only) reset_groups(ctx);
   for (int64_t r0 = 0; r0 < nvalid; r0 += chunk) {
     const int64_t n = std::min(chunk, nvalid - r0);
     const int32_t *recs = (const int32_t *)ctx->d_rec + 6 * r0;
@@ -892,22 +896,49 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e, "quartet preparation", __LINE__);
     mom_total += chunk_mom;
-    // survivors of the chunk, appended in file order
+    // survivors of the chunk, appended in file order; groups only: the
+    // chunk's survivors are grouped and their rows dropped (groups are
+    // independent terms, so a shell quartet split between chunks is two
+    // groups)
     int64_t qc = 0;
     if ((rc = compact_flags(ctx, ctx->d_flag, n, ctx->d_idx, &qc))) return rc;
-    if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 + qc) * stride,
-                        (size_t)q2 * stride)))
+    const int64_t at = groups_only ? 0 : q2;
+    if (groups_only) {
+      if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(qc > 0 ? qc : 1) * stride))) return rc;
+    } else if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 + qc) * stride,
+                               (size_t)q2 * stride))) {
       return rc;
+    }
     if (qc > 0) {
       hipLaunchKernelGGL(k_gather_records, dim3(nblk(qc * stride)), dim3(kT), 0, st,
-                         ctx->d_qtmp, ctx->d_idx, qc, stride, ctx->d_q + q2 * stride);
+                         ctx->d_qtmp, ctx->d_idx, qc, stride, ctx->d_q + at * stride);
       INCA_HIP(hipGetLastError());
+      if (groups_only && (rc = group_rows(ctx, ctx->d_q, qc))) return rc;
     }
     q2 += qc;
   }
   // in the grids' convention: their weights carry 2 pi where the volume
   // element has 4 pi (mod_build_grid.f90:132)
   ctx->term_integral = 0.5 * mom_total;
+  if (groups_only) {
+    // only the groups stay: the rows of the last chunk go, and every
+    // record-by-record use reports the release (need_records)
+    if (ctx->d_q) (void)hipFree(ctx->d_q);
+    ctx->d_q = nullptr;
+    ctx->q_cap = 0;
+    ctx->q2 = q2;
+    ctx->q_fp64 = q2;
+    ctx->mixed_tau = 0.0;
+    ctx->prep_precision = precision;
+    ctx->rec_split_pending = 0;
+    INCA_HIP(hipMemcpyAsync(hcnt, ctx->d_cnt, sizeof(hcnt), hipMemcpyDeviceToHost, st));
+    INCA_HIP(hipStreamSynchronize(st));
+    counts[0] = nvalid;
+    counts[1] = (int64_t)hcnt[1];
+    counts[2] = q2;
+    counts[3] = (int64_t)hcnt[3];
+    return finish_groups(ctx, precision);
+  }
   if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride,
                       (size_t)q2 * stride)))
     return rc;
@@ -936,8 +967,17 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
 // |sum M| (sum M being the integral of I over all space), the rest stays
 // FP64 and comes first.  Done once, when a record-by-record mixed
 // evaluation (or a query of the split) first needs it.
+int need_records(inca_ctx *ctx) {
+  if (ctx->q2 > 0 && !ctx->d_q)
+    return set_err(ctx, INCA_EINVAL,
+                   "the prepared records were released (inca_release_records): only the "
+                   "evaluation by shell-quartet groups remains; prepare again");
+  return 0;
+}
+
 int split_records(inca_ctx *ctx) {
   if (!ctx->rec_split_pending) return 0;
+  if (int rc = need_records(ctx)) return rc;
   ctx->rec_split_pending = 0;
   hipStream_t st = ctx->stream;
   const int64_t q2 = ctx->q2;

@@ -384,6 +384,22 @@ int inca_set_grouping(inca_ctx *ctx, int32_t on);
  * preparation did not group). */
 int inca_get_groups(inca_ctx *ctx, int64_t *ngroups, int64_t *ngroups_fp64,
                     int64_t *ncoefficients);
+/* Free the prepared records of a grouped preparation, keeping its groups,
+ * grid and counters: for callers that keep many prepared molecules (the
+ * batch takes groups).  Afterwards only evaluations by groups work; any
+ * record-by-record use (grouping off, cube contraction, the no-allocation
+ * entry point, inca_get_prepared, inca_get_mixed_split, autotune of the
+ * record kernels) fails with INCA_EINVAL until the next preparation.  Fails
+ * when the preparation made no groups.  No reference counterpart. */
+int inca_release_records(inca_ctx *ctx);
+/* keep = 0: the next preparations build the shell-quartet groups chunk by
+ * chunk of the pre-pass (inca_set_prepare_chunk) and keep no prepared
+ * records, so a molecule whose records would not fit the device is still
+ * prepared (the state inca_release_records leaves, reached without ever
+ * holding every record).  A shell quartet split between two chunks becomes
+ * two groups.  Bases beyond d functions, or grouping off, keep the records
+ * whatever the setting.  Default 1.  No reference counterpart. */
+int inca_set_keep_records(inca_ctx *ctx, int32_t keep);
 /* FP64-equivalent arithmetic of the last grid loop when it ran by groups:
  * the sum over the (group, point) pairs it evaluated of 14 + 2 M (M the
  * group's coefficient count; rp 6, |rp|^2 5, the polynomial 2 M, product and

@@ -155,3 +155,37 @@ def test_groups_repeat_bitwise_and_tiles(dsys):
                 assert er < 1e-13 and ea < 1e-15
         finally:
             e.set_tiles(4, 4)
+
+
+def test_groups_built_chunk_by_chunk_without_records(dsys):
+    """inca_set_keep_records(ctx, 0): the groups are built per pre-pass chunk
+    (here 20,000 records) and no prepared record is kept; a shell quartet
+    split between chunks becomes two groups.  Same counters, same values up
+    to summation order; every record-by-record use reports the release."""
+    s, ijkl, d = dsys
+    with capi.Engine(0) as a, capi.Engine(0) as b:
+        for e in (a, b):
+            p = _setup(e, s, ijkl, d)
+            cent, Ps, sf = O.centercalc(p["an"], p["cartes"], True, False)
+            nq = len(Ps)
+            e.grid_multicenter(cent, [40] * nq, [110] * nq, sf, Ps, nosym=True)
+        b.set_keep_records(0)
+        b.set_prepare_chunk(20000)
+        for prec in (capi.INCA_FP64, capi.INCA_MIXED):
+            ref, cnt = a.evaluate(prec, THRESH)
+            iv, cnt2 = b.evaluate(prec, THRESH)
+            assert list(cnt2) == list(cnt)
+            assert b.groups()[0] >= a.groups()[0]
+            assert b.analytic_integral() == pytest.approx(a.analytic_integral(), rel=1e-13)
+            er, ea = _close(iv, ref)
+            print("chunked groups (%d vs %d) prec %d: rel %.2e abs %.2e" % (
+                b.groups()[0], a.groups()[0], prec, er, ea))
+            if prec == capi.INCA_FP64:
+                assert er < 1e-12 and ea < 1e-14
+            else:
+                assert ea < 1e-8
+        with pytest.raises(capi.IncaError):
+            b.prepared(0, 1)
+        b.set_grouping(0)
+        with pytest.raises(capi.IncaError):
+            b.evaluate_prepared(capi.INCA_FP64)
