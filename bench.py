@@ -254,14 +254,11 @@ def target_bench(capi, torch, nmol=32):
              for i, m in enumerate(mols)]
     check = [i for i in range(nmol) if err_l[i] > LG.FALLBACK_TOL]
     tp = {}
-    if check:
-        for i in check:
-            cent, Ps, sf, nr, na = plain_grid(mols[i])
-            engines[i].grid_multicenter(cent, nr, na, sf, Ps, nosym=True)
-        b2 = capi.Batch([engines[i] for i in check])
-        tpl, _ = b2.evaluate()
-        b2.close()
-        tp = {i: float(v) for i, v in zip(check, tpl)}
+    for i in check:  # each context's own grid loop (groups, mixed) on its prepared quartets
+        cent, Ps, sf, nr, na = plain_grid(mols[i])
+        engines[i].grid_multicenter(cent, nr, na, sf, Ps, nosym=True)
+        engines[i].evaluate_prepared(capi.INCA_MIXED, want_ivec=False)
+        tp[i] = float(engines[i].integrate()[0])
     final, used = [], []
     for i, m in enumerate(mols):
         ia = m["learned_grid"]["analytic"]
@@ -306,7 +303,8 @@ def target_bench(capi, torch, nmol=32):
                 "molecule (pre-pass and shell-quartet groups), batch create and the first replay "
                 "on the learned grid (replay: later replays); fallback check: the plain autogrids "
                 "of the molecules whose learned-grid integral misses the analytic integral by more "
-                "than 1e-8, built and evaluated (mixed, on the prepared quartets) in one batch",
+                "than 1e-8, built and evaluated (mixed, by groups, on each context's prepared "
+                "quartets)",
     }
 
 

@@ -227,6 +227,9 @@ def main(argv=None):
     ap.add_argument("--rank", default="0:32")
     ap.add_argument("--single-every", type=int, default=8)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--precision", choices=("fp64", "mixed", "both"), default="both")
+    ap.add_argument("--replays", type=int, default=2,
+                    help="FP64 replays per group (2: replays compared bit for bit)")
     a = ap.parse_args(argv)
     import torch
     torch.cuda.init()  # torch's HIP runtime before the library's first context
@@ -241,21 +244,30 @@ def main(argv=None):
     def log(*x):
         print(*x, file=sys.stderr, flush=True)
 
-    t0 = time.time()
-    r64, g64 = evaluate_stream(mols, capi.INCA_FP64, single=single, log=log)
-    t1 = time.time()
-    rmx, gmx = evaluate_stream(mols, capi.INCA_MIXED, replays=1, log=log)
-    t2 = time.time()
     keep = ("name", "atoms", "nprim", "nelec", "nrad", "nang", "points", "counts",
             "reference_evals", "npairs", "analytic", "integral", "vee", "sum_rule_err", "group",
             "single_fp64", "single_kernel_ms")
-    out = dict(rank=[lo, hi], seconds_fp64=t1 - t0, seconds_mixed=t2 - t1,
-               fp64=[{k: m[k] for k in keep if k in m} for m in r64],
-               mixed=[{k: m[k] for k in keep if k in m} for m in rmx],
-               groups_fp64=g64, groups_mixed=gmx)
-    with open(a.out, "w") as f:
-        json.dump(out, f)
-    log("done: %d molecules, FP64 %.0f s, mixed %.0f s" % (len(mols), t1 - t0, t2 - t1))
+    out = dict(rank=[lo, hi])
+
+    def save():
+        with open(a.out, "w") as f:
+            json.dump(out, f)
+
+    # results are written after each precision's pass
+    t0 = time.time()
+    if a.precision in ("fp64", "both"):
+        r64, g64 = evaluate_stream(mols, capi.INCA_FP64, single=single, log=log,
+                                   replays=a.replays)
+        out.update(seconds_fp64=time.time() - t0, groups_fp64=g64,
+                   fp64=[{k: m[k] for k in keep if k in m} for m in r64])
+        save()
+    t1 = time.time()
+    if a.precision in ("mixed", "both"):
+        rmx, gmx = evaluate_stream(mols, capi.INCA_MIXED, replays=1, log=log)
+        out.update(seconds_mixed=time.time() - t1, groups_mixed=gmx,
+                   mixed=[{k: m[k] for k in keep if k in m} for m in rmx])
+        save()
+    log("done: %d molecules in %.0f s" % (len(mols), time.time() - t0))
     return 0
 
 

@@ -189,3 +189,31 @@ def test_groups_built_chunk_by_chunk_without_records(dsys):
         b.set_grouping(0)
         with pytest.raises(capi.IncaError):
             b.evaluate_prepared(capi.INCA_FP64)
+
+
+def test_group_rays_match_point_by_point():
+    """Radial scans: the groups on ray tiles (P >= 2 points per lane on one
+    direction, exp(-s) by the two-product recurrence) against the groups
+    point by point (P = 1), two scan blocks with different steps and orders,
+    exponents up to 2e4 so that tight groups take the point-by-point branch
+    inside ray tiles."""
+    s = synth.config_cluster(seed=31, natoms=4, nheavy=3, nocc=4, exp_lo=0.05, exp_hi=2e4,
+                             shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
+    ijkl, d = s.dm2_unique()
+    with capi.Engine(0) as e:
+        _setup(e, s, ijkl, d)
+        e.grid_radial_plot([[0.0, 3.0], [3.2, 6.0]], [0.1, 0.2], [110, 302])
+        out = {}
+        for P in (1, 2, 4, 6, 8):
+            e.set_tiles(P, 4)
+            iv, _ = e.evaluate(capi.INCA_FP64, THRESH)
+            assert e.groups()[0] > 0
+            out[P] = (iv.copy(), e.radial_values().copy())
+        for P in (2, 4, 6, 8):
+            er, ea = _close(out[P][0], out[1][0])
+            rr, ra = _close(out[P][1], out[1][1])
+            print("groups, P=%d rays vs point by point: Ivec rel %.2e abs %.2e; I(s) rel %.2e "
+                  "abs %.2e" % (P, er, ea, rr, ra))
+            assert not np.array_equal(out[P][0], out[1][0])  # the ray arithmetic ran
+            assert er < 1e-11 and ea < 1e-13
+            assert rr < 1e-11 and ra < 1e-13
