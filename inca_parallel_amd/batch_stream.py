@@ -182,7 +182,11 @@ def evaluate_stream(molecules, precision=capi.INCA_FP64, arena_bytes=64 << 30, t
         used += need
     flush()
     for m in mols:
+        # against the pair count N(N-1)/2 (the reference's check), and
+        # against the exact integral of the prepared terms (the grid's own
+        # quadrature error, free of the fits' roundoff)
         m["sum_rule_err"] = (m["integral"] - m["npairs"]) / m["npairs"]
+        m["grid_err"] = (m["integral"] - m["analytic"]) / abs(m["analytic"])
     return mols, groups
 
 
@@ -245,7 +249,8 @@ def main(argv=None):
         print(*x, file=sys.stderr, flush=True)
 
     keep = ("name", "atoms", "nprim", "nelec", "nrad", "nang", "points", "counts",
-            "reference_evals", "npairs", "analytic", "integral", "vee", "sum_rule_err", "group",
+            "reference_evals", "npairs", "analytic", "integral", "vee", "sum_rule_err", "grid_err",
+            "group",
             "single_fp64", "single_kernel_ms")
     out = dict(rank=[lo, hi])
 

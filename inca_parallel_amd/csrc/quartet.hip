@@ -8,11 +8,13 @@
 //                  Screen 1 then costs two table reads per record instead of
 //                  two JA8 evaluations.
 //   k_first_bad    the reader's stop rule (first record with a bad index).
-//   k_prep<NPM,S>  one thread per record: screens + prefactors + polycoef;
-//                  writes the prepared record into slot n and a pass flag.
+//   k_screen1      one thread per record: screen 1 alone, a pass flag.
+//   k_prep<NPM,S>  one thread per screen-1 survivor (compacted, file order):
+//                  prefactors + polycoef + screen 2; writes the prepared
+//                  record into the survivor's slot and a pass flag.
 //                  S: records with every axis of degree <= 4, fits in
 //                  registers; the others are deferred to a second launch
-//                  over their compacted indices with the fit in scratch.
+//                  over their compacted slots with the fit in scratch.
 //   compaction     deterministic scan of the flags, then a gather of the
 //                  surviving records in file order.
 // Records are read straight from HBM as three 8-byte words per 24-byte
@@ -459,17 +461,41 @@ __device__ double cutoff_radius(const double *c, const int Ls[3], double s0) {
   return (r >= 0.0) ? r : HUGE_VAL;
 }
 
+// Screen 1 alone (the expression of k_prep, mod_intracule.f90:297-305):
+// pass[n] = 1 for the records that pass, so that k_prep can run over the
+// compacted survivors (about half of the records) with no lane idle on a
+// failed one.
+__global__ __launch_bounds__(kT) void k_screen1(const int32_t *__restrict__ rec, int64_t n,
+                                                int nprim, const double *__restrict__ prim,
+                                                const PairEntry *__restrict__ pair, double lim,
+                                                int normalize, uint8_t *__restrict__ pass) {
+  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= n) return;
+  const int32_t *p = rec + 6 * g;
+  const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+  double D;
+  __builtin_memcpy(&D, p + 4, sizeof(double));
+  const double *nrm = prim + 4 * nprim;
+  double DMval = D;
+  if (normalize) DMval = nrm[i] * nrm[j] * nrm[k] * nrm[l] * D;
+  const double Jik = pair[(int64_t)i * nprim + k].J, Jjl = pair[(int64_t)j * nprim + l].J;
+  pass[g] = (uint8_t)(fabs(DMval) * sqrt(Jik * Jjl) >= lim);
+}
+
 // One thread per record; the output record layout is described at kQHead.
 //
-// SMALL: the records whose three axes all have degree below NPM, each axis
-// by the padded compile-time-size fit (axis_pad), so nothing is indexed at
-// run time and the matrices stay in registers (fitted polynomials only); every other record is marked in
-// `defer` and left alone.  Otherwise: the records idx[0..nrun) (all records
-// when idx is null), each axis by the run-time-size fit, whose matrix lives
-// in scratch.
+// Slots: the lanes take the slots idx[0..nrun) (slots 0..nrun when idx is
+// null); slot s holds record rmap[s] (record s when rmap is null), and its
+// row, flag and deferral flag are written at s.  SMALL: the records
+// whose three axes all have degree below NPM, each axis by the padded
+// compile-time-size fit (axis_pad), so nothing is indexed at run time and
+// the matrices stay in registers (fitted polynomials only); every other
+// record is marked in `defer` and left alone.  Otherwise each axis by the
+// run-time-size fit, whose matrix lives in scratch.
 template <int NPM, bool SMALL>
 __global__ __launch_bounds__(kT) void k_prep(
-    const int32_t *__restrict__ rec, int64_t nrun, const int64_t *__restrict__ idx, int nprim,
+    const int32_t *__restrict__ rec, int64_t nrun, const int64_t *__restrict__ idx,
+    const int64_t *__restrict__ rmap, int nprim,
     const double *__restrict__ prim, const int32_t *__restrict__ tmn,
     const PairEntry *__restrict__ pair, double lim, int normalize, int qstride, int exact,
     double cut_s0, double *__restrict__ qout, uint8_t *__restrict__ flag,
@@ -481,8 +507,10 @@ __global__ __launch_bounds__(kT) void k_prep(
   double mom = 0.0;  // signed whole-space integral of a surviving term
   bool run = g < nrun;
   int64_t n = g;
-  if (run && !SMALL && idx) n = idx[g];
-  const int32_t *p = rec + 6 * n;
+  if (run && idx) n = idx[g];
+  int64_t r = n;
+  if (run && rmap) r = rmap[n];
+  const int32_t *p = rec + 6 * r;
   int i = 0, j = 0, k = 0, l = 0;
   if (run) {
     i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
@@ -791,11 +819,11 @@ int grow_keep(inca_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t used) {
 }
 
 // The pre-pass runs over chunks of ctx->prep_chunk records (default 2^26).
-// A chunk's uncompacted prepared slots (one record of `stride` doubles per
-// DM2 record) live in d_qtmp, and the survivors of every chunk are appended
-// to d_q in file order, so a stream of any length is prepared in bounded
-// scratch (2^26 records: 19 GB of slots for s/p/d bases; the radial-plot
-// benchmark's 65.9M records are one chunk).
+// A chunk's prepared slots (one record of `stride` doubles per screen-1
+// survivor) live in d_qtmp, and the screen-2 survivors of every chunk are
+// appended to d_q in file order, so a stream of any length is prepared in
+// bounded scratch (at most 2^26 slots: 19 GB for s/p/d bases; the
+// radial-plot benchmark's 65.9M records are one chunk with 31.9M slots).
 
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
   hipStream_t st = ctx->stream;
@@ -819,7 +847,6 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   INCA_HIP(hipStreamSynchronize(st));
   const int64_t nvalid = (int64_t)hcnt[0];
   const int64_t chunk = std::max<int64_t>(1, std::min(nvalid, ctx->prep_chunk));
-  if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)chunk * stride))) return rc;
   if ((rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)chunk))) return rc;
   if ((rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)chunk))) return rc;
   // mod_intracule.f90:163
@@ -830,16 +857,20 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // temporaries from the context's scratch block: the term-integral
   // partials, the deferral flags and the mixed split's histogram
   Carve cv;
+  const bool split = !ctx->poly_exact;
+  const bool deferred = split && la > 4;
   const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(chunk),
-               o_hist = cv.add<unsigned long long>(kMagBins);
+               o_hist = cv.add<unsigned long long>(kMagBins), o_pass = cv.add<uint8_t>(chunk),
+               o_dslot = cv.add<int64_t>(deferred ? chunk : 1);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
   double *d_mom = Carve::at<double>(ctx->d_tmp, o_mom);
   // Fitted polynomials: records whose axes are all of degree <= 4 (most
   // of them) go through the register-resident fits; the rest are compacted
   // in file order and handled by the scratch kernel sized for the basis.
   // Exact polynomials: the scratch kernel takes every record.
-  const bool split = !ctx->poly_exact;
-  uint8_t *d_defer = (split && la > 4) ? Carve::at<uint8_t>(ctx->d_tmp, o_def) : nullptr;
+  uint8_t *d_defer = deferred ? Carve::at<uint8_t>(ctx->d_tmp, o_def) : nullptr;
+  uint8_t *d_pass = Carve::at<uint8_t>(ctx->d_tmp, o_pass);
+  int64_t *d_dslot = Carve::at<int64_t>(ctx->d_tmp, o_dslot);
   double mom_total = 0.0;  // chunk totals added in file order
   int64_t q2 = 0;
   // groups without the records (inca_set_keep_records(ctx, 0)): grouped
@@ -850,27 +881,38 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     const int64_t n = std::min(chunk, nvalid - r0);
     const int32_t *recs = (const int32_t *)ctx->d_rec + 6 * r0;
     hipError_t e = hipSuccess;
-    // the small-degree pass's nblk(n) partials end at d_mom + nb1, where the
+    // screen 1 first: the fits run over its compacted survivors only.  Slot
+    // s (0..n1) holds survivor d_idx[s] (file order); rows, flags and
+    // deferral flags are indexed by slot, so d_qtmp holds n1 rows.
+    hipLaunchKernelGGL(k_screen1, dim3(nblk(n)), dim3(kT), 0, st, recs, n, nprim, ctx->d_prim,
+                       ctx->d_pair, lim, ctx->normalize, d_pass);
+    INCA_HIP(hipGetLastError());
+    int64_t n1 = 0;
+    if ((rc = compact_flags(ctx, d_pass, n, ctx->d_idx, &n1))) return rc;
+    if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)(n1 > 0 ? n1 : 1) * stride)))
+      return rc;
+    // the small-degree pass's nblk(n1) partials end at d_mom + nb1, where the
     // deferred pass's begin, so that one fixed-order sum covers both
-    double *small_mom = d_mom + (nb1 - (int64_t)nblk(n));
-    if (split) {
-      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n)), dim3(kT), 0, st, recs, n,
-                         (const int64_t *)nullptr, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair,
+    double *small_mom = d_mom + (nb1 - (int64_t)nblk(n1));
+    if (split && n1 > 0) {
+      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, recs, n1,
+                         (const int64_t *)nullptr, (const int64_t *)ctx->d_idx, nprim,
+                         ctx->d_prim, ctx->d_tmn, ctx->d_pair,
                          lim, ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
                          ctx->d_qtmp, ctx->d_flag, d_defer, ctx->d_cnt, small_mom);
       e = hipGetLastError();
     }
-    int64_t nrun = split ? 0 : n;  // records for the scratch kernel
-    if (e == hipSuccess && d_defer) {
-      rc = compact_flags(ctx, d_defer, n, ctx->d_idx, &nrun);
+    int64_t nrun = split ? 0 : n1;  // records for the scratch kernel
+    if (e == hipSuccess && d_defer && n1 > 0) {
+      rc = compact_flags(ctx, d_defer, n1, d_dslot, &nrun);
       if (rc) return rc;
     }
-    const int64_t *run_idx = split ? ctx->d_idx : nullptr;
+    const int64_t *run_idx = split ? d_dslot : nullptr;
     double *run_mom = split ? d_mom + nb1 : d_mom;
     if (e == hipSuccess && nrun > 0) {
 #define INCA_PREP(NPM)                                                                        \
   hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st, recs, nrun,    \
-                     run_idx, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,               \
+                     run_idx, ctx->d_idx, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,   \
                      ctx->normalize, stride, ctx->poly_exact,                                 \
                      ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
                      (uint8_t *)nullptr, ctx->d_cnt, run_mom)
@@ -882,8 +924,9 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
 #undef INCA_PREP
       e = hipGetLastError();
     }
-    const int64_t nbt = split ? nblk(n) + (nrun > 0 ? (int64_t)nblk(nrun) : 0)
-                              : (int64_t)nblk(nrun);
+    const int64_t nbt = split ? (n1 > 0 ? (int64_t)nblk(n1) : 0) +
+                                    (nrun > 0 ? (int64_t)nblk(nrun) : 0)
+                              : (nrun > 0 ? (int64_t)nblk(nrun) : 0);
     const double *sum_from = split ? small_mom : d_mom;
     double chunk_mom = 0.0;
     if (e == hipSuccess && nbt > 0) {
@@ -901,7 +944,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     // independent terms, so a shell quartet split between chunks is two
     // groups)
     int64_t qc = 0;
-    if ((rc = compact_flags(ctx, ctx->d_flag, n, ctx->d_idx, &qc))) return rc;
+    if ((rc = compact_flags(ctx, ctx->d_flag, n1, ctx->d_idx, &qc))) return rc;
     const int64_t at = groups_only ? 0 : q2;
     if (groups_only) {
       if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(qc > 0 ? qc : 1) * stride))) return rc;
