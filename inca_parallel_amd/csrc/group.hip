@@ -13,12 +13,13 @@
 // record and point.  A shell quartet of d functions holds up to 1296 records
 // and needs at most 165 coefficients.
 //
-// Building the groups (one pass after the pre-pass, deterministic):
-//   k_group_keys    a 32-bit hash of the bits of (sqe, offset) per record;
-//   radix sort      stable, so the records of a group keep their order;
+// Building the groups (after the pre-pass, deterministic).  The pre-pass
+// has already ordered its slots by a 32-bit hash of the bits of (sqe,
+// offset) with a stable radix sort (sort_slots below), so the rows of a
+// group are adjacent and keep their file order:
 //   k_group_heads   a group starts wherever the four doubles change (a hash
 //                   collision only splits a group, never merges two);
-//   k_group_header  one thread per group: degrees, bound, cutoff radius;
+//   k_group_header  one wave per group: degrees, bound, cutoff radius;
 //   exclusive scan  of the coefficient counts: the coefficient pool offsets;
 //   k_group_coeffs  one wave per group, a lane per coefficient, the members
 //                   added in their record order.
@@ -46,40 +47,19 @@ namespace {
 constexpr int kT = 256;
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x ^= x >> 30;
-  x *= 0xbf58476d1ce4e5b9ull;
-  x ^= x >> 27;
-  x *= 0x94d049bb133111ebull;
-  x ^= x >> 31;
-  return x;
-}
-
-__global__ __launch_bounds__(kT) void k_group_keys(const double *__restrict__ q, int stride,
-                                                   int64_t n, uint32_t *__restrict__ key,
-                                                   uint32_t *__restrict__ val) {
-  const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (i >= n) return;
-  const double *r = q + i * stride;
-  uint64_t h = 0x9e3779b97f4a7c15ull;
-  for (int w = 0; w < 4; ++w) h = mix64(h ^ (uint64_t)__double_as_longlong(r[w]));
-  key[i] = (uint32_t)(h >> 32);
-  val[i] = (uint32_t)i;
-}
-
 __device__ __forceinline__ bool same_gaussian(const double *a, const double *b) {
   bool s = true;
   for (int w = 0; w < 4; ++w) s = s && __double_as_longlong(a[w]) == __double_as_longlong(b[w]);
   return s;
 }
 
+// Rows arrive sorted by group (the pre-pass orders its slots by a hash of
+// the four words): a group starts wherever the words change.
 __global__ __launch_bounds__(kT) void k_group_heads(const double *__restrict__ q, int stride,
-                                                    const uint32_t *__restrict__ order, int64_t n,
-                                                    uint8_t *__restrict__ head) {
+                                                    int64_t n, uint8_t *__restrict__ head) {
   const int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (i >= n) return;
-  head[i] = (uint8_t)(i == 0 || !same_gaussian(q + (int64_t)order[i] * stride,
-                                               q + (int64_t)order[i - 1] * stride));
+  head[i] = (uint8_t)(i == 0 || !same_gaussian(q + i * stride, q + (i - 1) * stride));
 }
 
 __host__ __device__ __forceinline__ int monomials(int Lx, int Ly, int Lz, int Ls) {
@@ -98,32 +78,38 @@ __device__ __forceinline__ void record_degrees(const double *r, int &Lx, int &Ly
   Lz = (cls >> 16) & 0xff;
 }
 
-// Members of group g: order[first[g] .. first[g+1]) (the last group ends at n).
+// Members of group g: rows first[g] .. first[g+1] (the last group ends at n).
+// One wave per group: lane j takes members j, j + 64, ...; the degrees and
+// the cutoff radius reduce by max, B by a fixed-order butterfly sum.
 __global__ __launch_bounds__(kT) void k_group_header(const double *__restrict__ q, int stride,
-                                                     const uint32_t *__restrict__ order,
                                                      const int64_t *__restrict__ first, int64_t ng,
                                                      int64_t n, double *__restrict__ gh,
                                                      int64_t *__restrict__ mcount) {
-  const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (g >= ng) return;
+  const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (g >= ng) return;  // whole wave
+  const int lane = threadIdx.x & 63;
   const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
-  int Lx = 0, Ly = 0, Lz = 0, Ls = 0;
+  int Ls = 0;
   double B = 0.0, rcut = 0.0;
-  const double *r0 = q + (int64_t)order[m0] * stride;
-  for (int64_t m = m0; m < m1; ++m) {
-    const double *r = q + (int64_t)order[m] * stride;
+  for (int64_t m = m0 + lane; m < m1; m += 64) {
+    const double *r = q + m * stride;
     int lx, ly, lz;
     record_degrees(r, lx, ly, lz);
-    Lx = max(Lx, lx);
-    Ly = max(Ly, ly);
-    Lz = max(Lz, lz);
     Ls = max(Ls, lx + ly + lz);
     B += r[5];
     rcut = fmax(rcut, r[6]);
   }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    Ls = max(Ls, __shfl_xor(Ls, d, 64));
+    B += __shfl_xor(B, d, 64);
+    rcut = fmax(rcut, __shfl_xor(rcut, d, 64));
+  }
+  if (lane != 0) return;
+  const double *r0 = q + m0 * stride;
   // every monomial of total degree <= Ls is laid out (groups are built for
   // total degrees up to 8, and the evaluation unrolls per degree)
-  Lx = Ly = Lz = Ls;
+  const int Lx = Ls, Ly = Ls, Lz = Ls;
   const int M = monomials(Lx, Ly, Lz, Ls);
   const int axes = (Lx > 0) | ((Ly > 0) << 1) | ((Lz > 0) << 2);
   double *o = gh + g * kGStride;
@@ -151,45 +137,34 @@ __global__ __launch_bounds__(kT) void k_group_offsets(const int64_t *__restrict_
 
 // One wave per group; lane j of pass p owns coefficient 64 p + j and adds
 // the members' products into it in member order: (A Cx[a] Cy[b]) Cz[c].
-// The members' records are staged in LDS kStage at a time by the whole
-// wave (their loads in flight together), then read from LDS; `lds` holds
-// kStage * stride doubles per wave.
+// The members' rows are contiguous: the wave stages kStage of them at a time
+// in LDS (all loads in flight together); lane r of a stage decodes member
+// r's degrees once, and the member loop reads them back as wave-uniform
+// values (readlane), so each member costs a lane three compares, three LDS
+// reads and the product.  `lds` holds kStage * stride doubles per wave.
 constexpr int kStage = 32;
 constexpr int kMaxStride = 36;  // s/p/d records: 8 + 3 (8 + 1) doubles, rounded even
 constexpr int kLoads = kStage * kMaxStride / 64;
-__global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
-                                                     const uint32_t *__restrict__ order,
-                                                     const int64_t *__restrict__ first, int64_t ng,
-                                                     int64_t n, const double *__restrict__ gh,
-                                                     double *__restrict__ pool) {
-  extern __shared__ double lds_all[];
-  const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (g >= ng) return;  // whole wave (no block barrier below)
-  double *lds = lds_all + (size_t)(threadIdx.x >> 6) * kStage * stride;
-  const int lane = threadIdx.x & 63;
-  const double *h = gh + g * kGStride;
-  const uint64_t cls = (uint64_t)__double_as_longlong(h[4]);
-  const int Lx = cls & 0xff, Ly = (cls >> 8) & 0xff, Lz = (cls >> 16) & 0xff;
-  const int Ls = (int)((cls >> 32) & 0xff);
-  const int M = (int)__double_as_longlong(h[9]);
-  double *K = pool + __double_as_longlong(h[8]);
-  const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
-  const int npass = (M + 63) / 64;
-  // this lane's (a, b, c) in the storage order, per pass (M <= 64 * 4 for
-  // the layouts built here: at most 165 coefficients up to total degree 8)
-  int la[4], lb[4], lc[4];
-  double acc[4];
-  for (int p = 0; p < 4; ++p) {
+
+template <int NP>
+__device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int stride, int64_t m0,
+                                             int64_t m1, int Ls, int M, double *__restrict__ K,
+                                             double *lds, int lane) {
+  // this lane's (a, b, c) per pass, in the storage order (every monomial of
+  // total degree <= Ls: a = Ls..0, b = Ls - a..0, c = Ls - a - b..0)
+  int la[NP], lb[NP], lc[NP];
+  double acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
     la[p] = -1;
     lb[p] = lc[p] = 0;
     acc[p] = 0.0;
     const int idx = 64 * p + lane;
-    if (p >= npass || idx >= M) continue;
+    if (idx >= M) continue;
     int at = 0;
-    for (int aa = Lx; aa >= 0 && la[p] < 0; --aa) {
-      const int bm = min(Ly, Ls - aa);
-      for (int bb = bm; bb >= 0; --bb) {
-        const int cm = min(Lz, Ls - aa - bb);
+    for (int aa = Ls; aa >= 0 && la[p] < 0; --aa) {
+      for (int bb = Ls - aa; bb >= 0; --bb) {
+        const int cm = Ls - aa - bb;
         if (idx <= at + cm) {
           la[p] = aa;
           lb[p] = bb;
@@ -202,33 +177,33 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
   }
   for (int64_t s0 = m0; s0 < m1; s0 += kStage) {
     const int ns = (int)((m1 - s0) < kStage ? (m1 - s0) : kStage);
-    // lane j < ns holds the offset of member s0 + j's record; the stage's
-    // ns * stride doubles are then fetched kLoads per lane, all in flight
-    // together, and written to LDS
-    const int64_t myrow = lane < ns ? (int64_t)order[s0 + lane] * stride : 0;
+    const double *src = q + s0 * stride;
+    const int nw = ns * stride;
     double v[kLoads];
 #pragma unroll
     for (int k = 0; k < kLoads; ++k) {
       const int t = lane + 64 * k;
-      const int r = t / stride;
-      const int64_t row = __shfl(myrow, r < 64 ? r : 0, 64);
-      v[k] = (r < ns) ? q[row + (t - r * stride)] : 0.0;
+      v[k] = (t < nw) ? src[t] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < kLoads; ++k) {
       const int t = lane + 64 * k;
-      if (t < ns * stride) lds[t] = v[k];
+      if (t < nw) lds[t] = v[k];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // lane r: member r's degrees Lx | Ly << 8 | Lz << 16
+    int info = 0;
+    if (lane < ns) info = (int)((unsigned)__double_as_longlong(lds[lane * stride + 4]) & 0xffffffu);
+#pragma unroll 2
     for (int r = 0; r < ns; ++r) {
-      const double *row = lds + r * stride;
-      int lx, ly, lz;
-      record_degrees(row, lx, ly, lz);
-      const double *cx = row + kQHead;
+      const int inf = __builtin_amdgcn_readlane(info, r);
+      const int lx = inf & 0xff, ly = (inf >> 8) & 0xff, lz = (inf >> 16) & 0xff;
+      const double *cx = lds + r * stride + kQHead;
       const double *cy = cx + lx + 1;
       const double *cz = cy + ly + 1;
-      for (int p = 0; p < npass; ++p) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
         const int a = la[p], b = lb[p], c = lc[p];
         if (a >= 0 && a <= lx && b <= ly && c <= lz)
           acc[p] += (cx[lx - a] * cy[ly - b]) * cz[lz - c];
@@ -237,10 +212,32 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  for (int p = 0; p < npass; ++p) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
     const int idx = 64 * p + lane;
     if (idx < M) K[idx] = acc[p];
   }
+}
+
+__global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
+                                                     const int64_t *__restrict__ first, int64_t ng,
+                                                     int64_t n, const double *__restrict__ gh,
+                                                     double *__restrict__ pool) {
+  extern __shared__ double lds_all[];
+  const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (g >= ng) return;  // whole wave (no block barrier below)
+  double *lds = lds_all + (size_t)(threadIdx.x >> 6) * kStage * stride;
+  const int lane = threadIdx.x & 63;
+  const double *h = gh + g * kGStride;
+  const uint64_t cls = (uint64_t)__double_as_longlong(h[4]);
+  const int Ls = (int)((cls >> 32) & 0xff);
+  const int M = (int)__double_as_longlong(h[9]);
+  double *K = pool + __double_as_longlong(h[8]);
+  const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
+  // M <= 165 (total degree 8): at most three passes of the wave
+  if (M <= 64) group_coeffs<1>(q, stride, m0, m1, Ls, M, K, lds, lane);
+  else if (M <= 128) group_coeffs<2>(q, stride, m0, m1, Ls, M, K, lds, lane);
+  else group_coeffs<3>(q, stride, m0, m1, Ls, M, K, lds, lane);
 }
 
 // Operations per point summed over the FP64 range [0, ng64) and the rest,
@@ -360,44 +357,34 @@ void reset_groups(inca_ctx *ctx) {
   ctx->gflop_fp64 = ctx->gflop_fp32 = 0.0;
 }
 
-// Groups of the prepared rows q[0..n) (n < 2^32), appended to the
-// context's groups: their headers after the existing ones, their
-// coefficients after the existing pool.
+// Groups of the prepared rows q[0..n), which arrive sorted by group
+// (adjacent, file order within a group), appended to the context's groups:
+// their headers after the existing ones, their coefficients after the
+// existing pool.
 int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   if (n == 0) return 0;
-  if (n > (int64_t)UINT32_MAX) return set_err(ctx, INCA_ERANGE, "grouping pass of 2^32 records or more");
   hipStream_t st = ctx->stream;
   const int stride = ctx->qstride;
   // temporaries carved from the context's grouping scratch (kept between
   // preparations, so that small molecules pay no allocation)
-  size_t sort_bytes = 0, scan_bytes = 0;
+  size_t scan_bytes = 0;
   {
-    uint32_t *k = nullptr, *v = nullptr;
     int64_t *a = nullptr, *b = nullptr;
-    INCA_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, k, k, v, v, (size_t)n, 0u, 32u, st));
     INCA_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, a, b, (int64_t)0, (size_t)n,
                                      rocprim::plus<int64_t>(), st));
   }
   Carve cv;
-  const size_t o_kin = cv.add<uint32_t>(n), o_kout = cv.add<uint32_t>(n),
-               o_vin = cv.add<uint32_t>(n), o_vout = cv.add<uint32_t>(n),
-               o_head = cv.add<uint8_t>(n), o_first = cv.add<int64_t>(n),
+  const size_t o_head = cv.add<uint8_t>(n), o_first = cv.add<int64_t>(n),
                o_m = cv.add<int64_t>(n), o_off = cv.add<int64_t>(n + 1),
-               o_sort = cv.add<char>((int64_t)sort_bytes), o_scan = cv.add<char>((int64_t)scan_bytes);
+               o_scan = cv.add<char>((int64_t)scan_bytes);
   int rc;
   if ((rc = ensure(ctx, ctx->d_gscr, ctx->gscr_cap, cv.size))) return rc;
   char *base = ctx->d_gscr;
-  uint32_t *d_kin = Carve::at<uint32_t>(base, o_kin), *d_kout = Carve::at<uint32_t>(base, o_kout);
-  uint32_t *d_vin = Carve::at<uint32_t>(base, o_vin), *d_vout = Carve::at<uint32_t>(base, o_vout);
   uint8_t *d_head = Carve::at<uint8_t>(base, o_head);
   int64_t *d_first = Carve::at<int64_t>(base, o_first);
   int64_t *d_m = Carve::at<int64_t>(base, o_m), *d_off = Carve::at<int64_t>(base, o_off);
-  char *d_sort = Carve::at<char>(base, o_sort), *d_scan = Carve::at<char>(base, o_scan);
-  hipLaunchKernelGGL(k_group_keys, dim3(nblk(n)), dim3(kT), 0, st, q, stride, n, d_kin, d_vin);
-  INCA_HIP(hipGetLastError());
-  INCA_HIP(rocprim::radix_sort_pairs(d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, (size_t)n,
-                                     0u, 32u, st));
-  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, q, stride, d_vout, n, d_head);
+  char *d_scan = Carve::at<char>(base, o_scan);
+  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, q, stride, n, d_head);
   INCA_HIP(hipGetLastError());
   int64_t ng = 0;
   if ((rc = compact_flags(ctx, d_head, n, d_first, &ng))) return rc;
@@ -406,8 +393,9 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
                      (size_t)ng0 * kGStride)))
     return rc;
   double *gh = ctx->d_gh + ng0 * kGStride;
-  hipLaunchKernelGGL(k_group_header, dim3(nblk(ng)), dim3(kT), 0, st, q, stride, d_vout, d_first,
-                     ng, n, gh, d_m);
+  const unsigned gblk = (unsigned)((ng + kT / 64 - 1) / (kT / 64));  // a wave per group
+  hipLaunchKernelGGL(k_group_header, dim3(gblk), dim3(kT), 0, st, q, stride, d_first, ng, n, gh,
+                     d_m);
   INCA_HIP(hipGetLastError());
   INCA_HIP(rocprim::exclusive_scan(d_scan, scan_bytes, d_m, d_off, pool0, (size_t)ng,
                                    rocprim::plus<int64_t>(), st));
@@ -419,12 +407,30 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, gh);
   INCA_HIP(hipGetLastError());
   if ((rc = grow_dev(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)pool_end, (size_t)pool0))) return rc;
-  hipLaunchKernelGGL(k_group_coeffs, dim3((unsigned)((ng + 3) / 4)), dim3(kT),
-                     (kT / 64) * kStage * stride * sizeof(double), st, q, stride, d_vout,
-                     d_first, ng, n, gh, ctx->d_gpool);
+  hipLaunchKernelGGL(k_group_coeffs, dim3(gblk), dim3(kT),
+                     (kT / 64) * kStage * stride * sizeof(double), st, q, stride, d_first, ng, n,
+                     gh, ctx->d_gpool);
   INCA_HIP(hipGetLastError());
   ctx->ng = ng0 + ng;
   ctx->gpool_n = pool_end;
+  return 0;
+}
+
+// Stable sort of n (key, value) pairs by the 32-bit key (rocPRIM radix
+// sort); temp of sort_slots_bytes(n) bytes.
+size_t sort_slots_bytes(int64_t n) {
+  size_t bytes = 0;
+  uint32_t *k = nullptr, *v = nullptr;
+  if (rocprim::radix_sort_pairs(nullptr, bytes, k, k, v, v, (size_t)n, 0u, 32u) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+int sort_slots(inca_ctx *ctx, void *temp, size_t bytes, uint32_t *kin, uint32_t *kout,
+               uint32_t *vin, uint32_t *vout, int64_t n) {
+  if (n == 0) return 0;
+  INCA_HIP(rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, 32u,
+                                     ctx->stream));
   return 0;
 }
 
@@ -511,12 +517,13 @@ int finish_groups(inca_ctx *ctx, int precision) {
 }
 
 
-// Group the prepared records (after prepare_quartets keeps them all).
-// Leaves ctx->ng = 0 (the evaluation then runs record by record) when there
-// is nothing to group, 2^32 records or more, or primitives beyond d.
+// Group the prepared records (after prepare_quartets keeps them all, sorted
+// by group within each pre-pass chunk; a group split between chunks becomes
+// two).  Leaves ctx->ng = 0 (the evaluation then runs record by record) when
+// there is nothing to group or primitives beyond d.
 int build_groups(inca_ctx *ctx, int precision) {
   reset_groups(ctx);
-  if (!groupable(ctx) || ctx->q2 == 0 || ctx->q2 > (int64_t)UINT32_MAX) return 0;
+  if (!groupable(ctx) || ctx->q2 == 0) return 0;
   int rc;
   if ((rc = group_rows(ctx, ctx->d_q, ctx->q2))) return rc;
   return finish_groups(ctx, precision);

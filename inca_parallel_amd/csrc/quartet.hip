@@ -482,6 +482,59 @@ __global__ __launch_bounds__(kT) void k_screen1(const int32_t *__restrict__ rec,
   pass[g] = (uint8_t)(fabs(DMval) * sqrt(Jik * Jjl) >= lim);
 }
 
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+
+// Group keys of the screen-1 survivors s < n1 (record idx[s]): a 32-bit
+// hash of the bits of sqe and sqe (P_ik - P_jl), computed by k_prep's own
+// operations, so that the records of a shell-quartet group (equal words)
+// share a key; val[s] = s for the stable sort.
+__global__ __launch_bounds__(kT) void k_slot_keys(const int32_t *__restrict__ rec, int64_t n1,
+                                                  const int64_t *__restrict__ idx, int nprim,
+                                                  const PairEntry *__restrict__ pair,
+                                                  uint32_t *__restrict__ key,
+                                                  uint32_t *__restrict__ val) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= n1) return;
+  const int32_t *p = rec + 6 * idx[s];
+  const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+  const PairEntry pik = pair[(int64_t)i * nprim + k];
+  const PairEntry pjl = pair[(int64_t)j * nprim + l];
+  const double aik = pik.aik, ajl = pjl.aik;
+  const double zeta = aik + ajl;
+  const double eijkl = (aik * ajl) * (1.0 / zeta);
+  const double sqe = sqrt(eijkl);
+  const double w[4] = {sqe, sqe * (pik.P[0] - pjl.P[0]), sqe * (pik.P[1] - pjl.P[1]),
+                       sqe * (pik.P[2] - pjl.P[2])};
+  uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) h = mix64(h ^ (uint64_t)__double_as_longlong(w[q]));
+  key[s] = (uint32_t)(h >> 32);
+  val[s] = (uint32_t)s;
+}
+
+// Slot j after the sort holds survivor order[j], record idx[order[j]]: its
+// 24 bytes are copied to out[j], so that the fits read their records in
+// slot order.
+__global__ __launch_bounds__(kT) void k_slot_records(const uint32_t *__restrict__ order,
+                                                     const int64_t *__restrict__ idx, int64_t n1,
+                                                     const int32_t *__restrict__ rec,
+                                                     int32_t *__restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (j >= n1) return;
+  const int64_t *src = reinterpret_cast<const int64_t *>(rec + 6 * idx[order[j]]);
+  int64_t *dst = reinterpret_cast<int64_t *>(out + 6 * j);
+  dst[0] = src[0];
+  dst[1] = src[1];
+  dst[2] = src[2];
+}
+
 // One thread per record; the output record layout is described at kQHead.
 //
 // Slots: the lanes take the slots idx[0..nrun) (slots 0..nrun when idx is
@@ -646,12 +699,20 @@ __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict_
                                                        const int64_t *__restrict__ idx,
                                                        int64_t n, int stride,
                                                        double *__restrict__ dst) {
-  // one thread per (record, word)
+  // one thread per (record, word); only the header and the record's
+  // coefficient words are copied (the slots past its degrees are never
+  // written, so they carry nothing)
   const int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (t >= n * stride) return;
   const int64_t r = t / stride;
   const int w = (int)(t - r * stride);
-  dst[t] = src[idx[r] * stride + w];
+  const double *row = src + idx[r] * stride;
+  if (w >= kQHead) {
+    const unsigned cls = (unsigned)__double_as_longlong(row[4]);
+    if (w >= kQHead + 3 + (int)(cls & 0xff) + (int)((cls >> 8) & 0xff) + (int)((cls >> 16) & 0xff))
+      return;
+  }
+  dst[t] = row[w];
 }
 
 // Histogram of the binary exponent of B (header word 5) over the records
@@ -821,9 +882,12 @@ int grow_keep(inca_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t used) {
 // The pre-pass runs over chunks of ctx->prep_chunk records (default 2^26).
 // A chunk's prepared slots (one record of `stride` doubles per screen-1
 // survivor) live in d_qtmp, and the screen-2 survivors of every chunk are
-// appended to d_q in file order, so a stream of any length is prepared in
+// appended to d_q in slot order, so a stream of any length is prepared in
 // bounded scratch (at most 2^26 slots: 19 GB for s/p/d bases; the
 // radial-plot benchmark's 65.9M records are one chunk with 31.9M slots).
+// Slot order is file order, or with grouping on, the order of a stable
+// sort by group key: the records of a shell-quartet group then sit
+// together (in file order), ready for group_rows.
 
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
   hipStream_t st = ctx->stream;
@@ -859,18 +923,32 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   Carve cv;
   const bool split = !ctx->poly_exact;
   const bool deferred = split && la > 4;
+  // grouping: the slots are ordered by group key, so the rows of a group
+  // come out adjacent (file order within the group)
+  const bool sort_groups = groupable(ctx) && chunk <= (int64_t)UINT32_MAX;
+  const int64_t ns = sort_groups ? chunk : 1;
+  const size_t sort_bytes = sort_groups ? sort_slots_bytes(chunk) : 0;
   const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(chunk),
                o_hist = cv.add<unsigned long long>(kMagBins), o_pass = cv.add<uint8_t>(chunk),
-               o_dslot = cv.add<int64_t>(deferred ? chunk : 1);
+               o_dslot = cv.add<int64_t>(deferred ? chunk : 1), o_kin = cv.add<uint32_t>(ns),
+               o_kout = cv.add<uint32_t>(ns), o_vin = cv.add<uint32_t>(ns),
+               o_vout = cv.add<uint32_t>(ns), o_srec = cv.add<int32_t>(6 * ns),
+               o_sort = cv.add<char>((int64_t)sort_bytes);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
   double *d_mom = Carve::at<double>(ctx->d_tmp, o_mom);
   // Fitted polynomials: records whose axes are all of degree <= 4 (most
   // of them) go through the register-resident fits; the rest are compacted
-  // in file order and handled by the scratch kernel sized for the basis.
+  // in slot order and handled by the scratch kernel sized for the basis.
   // Exact polynomials: the scratch kernel takes every record.
   uint8_t *d_defer = deferred ? Carve::at<uint8_t>(ctx->d_tmp, o_def) : nullptr;
   uint8_t *d_pass = Carve::at<uint8_t>(ctx->d_tmp, o_pass);
   int64_t *d_dslot = Carve::at<int64_t>(ctx->d_tmp, o_dslot);
+  uint32_t *d_kin = Carve::at<uint32_t>(ctx->d_tmp, o_kin);
+  uint32_t *d_kout = Carve::at<uint32_t>(ctx->d_tmp, o_kout);
+  uint32_t *d_vin = Carve::at<uint32_t>(ctx->d_tmp, o_vin);
+  uint32_t *d_vout = Carve::at<uint32_t>(ctx->d_tmp, o_vout);
+  int32_t *d_srec = Carve::at<int32_t>(ctx->d_tmp, o_srec);
+  char *d_sort = Carve::at<char>(ctx->d_tmp, o_sort);
   double mom_total = 0.0;  // chunk totals added in file order
   int64_t q2 = 0;
   // groups without the records (inca_set_keep_records(ctx, 0)): grouped
@@ -891,12 +969,26 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     if ((rc = compact_flags(ctx, d_pass, n, ctx->d_idx, &n1))) return rc;
     if ((rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)(n1 > 0 ? n1 : 1) * stride)))
       return rc;
+    // the fits read slot s's record at srec + 6 rmap[s] (rmap null: s)
+    const int32_t *srec = recs;
+    const int64_t *slot_rec = ctx->d_idx;
+    if (sort_groups && n1 > 0) {
+      hipLaunchKernelGGL(k_slot_keys, dim3(nblk(n1)), dim3(kT), 0, st, recs, n1,
+                         (const int64_t *)ctx->d_idx, nprim, ctx->d_pair, d_kin, d_vin);
+      INCA_HIP(hipGetLastError());
+      if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1))) return rc;
+      hipLaunchKernelGGL(k_slot_records, dim3(nblk(n1)), dim3(kT), 0, st, d_vout,
+                         (const int64_t *)ctx->d_idx, n1, recs, d_srec);
+      INCA_HIP(hipGetLastError());
+      srec = d_srec;
+      slot_rec = nullptr;
+    }
     // the small-degree pass's nblk(n1) partials end at d_mom + nb1, where the
     // deferred pass's begin, so that one fixed-order sum covers both
     double *small_mom = d_mom + (nb1 - (int64_t)nblk(n1));
     if (split && n1 > 0) {
-      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, recs, n1,
-                         (const int64_t *)nullptr, (const int64_t *)ctx->d_idx, nprim,
+      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, srec, n1,
+                         (const int64_t *)nullptr, slot_rec, nprim,
                          ctx->d_prim, ctx->d_tmn, ctx->d_pair,
                          lim, ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
                          ctx->d_qtmp, ctx->d_flag, d_defer, ctx->d_cnt, small_mom);
@@ -911,8 +1003,8 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     double *run_mom = split ? d_mom + nb1 : d_mom;
     if (e == hipSuccess && nrun > 0) {
 #define INCA_PREP(NPM)                                                                        \
-  hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st, recs, nrun,    \
-                     run_idx, ctx->d_idx, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,   \
+  hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st, srec, nrun,    \
+                     run_idx, slot_rec, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,     \
                      ctx->normalize, stride, ctx->poly_exact,                                 \
                      ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
                      (uint8_t *)nullptr, ctx->d_cnt, run_mom)

@@ -613,9 +613,13 @@ def test_prepared_records_match_oracle_fits(engine, small_cluster):
     engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
     engine.set_quartets(ijkl=sub, d=dsub)
     engine.grid_points(np.zeros((1, 3)))
-    _, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
-    n = int(cnt[2])
-    recs = engine.prepared(0, n)
+    engine.set_grouping(0)  # records in file order (grouping sorts them by group)
+    try:
+        _, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
+        n = int(cnt[2])
+        recs = engine.prepared(0, n)
+    finally:
+        engine.set_grouping(1)
     ref = [O.prepare_quartet(p, sub[r], dsub[r], THRESH) for r in range(len(sel))]
     keep = [q for q in ref if q.pass1 and q.pass2]
     assert n == len(keep)
@@ -667,25 +671,41 @@ def test_fp64_exp_accuracy_through_one_s_primitive(engine):
 
 
 def test_prepare_in_chunks_matches_one_pass(engine, small_cluster):
-    """The pre-pass over chunks of records (survivors appended in file order)
-    gives the one-pass result: equal counters, bitwise-equal prepared
-    records in FP64 and in the mixed split, and the term integral within
-    1e-14 (its partial sums are grouped by chunk)."""
+    """The pre-pass over chunks of records (survivors appended in file order,
+    grouping off) gives the one-pass result: equal counters, bitwise-equal
+    prepared records in FP64 and in the mixed split, and the term integral
+    within 1e-14 (its partial sums are grouped by chunk).  With grouping on
+    (records sorted by group within each chunk, a group split between
+    chunks becoming two), the same counters and the same values up to
+    summation order."""
     s, ijkl, d = small_cluster
     p = s.prims()
     engine.set_primitives(p["alpha"], p["tmn"], p["x"], p["y"], p["z"])
     engine.set_quartets(ijkl=ijkl, d=d)
-    engine.grid_points(np.zeros((1, 3)))
-    out = {}
+    cent, Ps, sf = O.centercalc(p["an"], p["cartes"], True, False)
+    engine.grid_multicenter(cent, [20] * len(Ps), [50] * len(Ps), sf, Ps, nosym=True)
+    out, grouped = {}, {}
     try:
         for chunk in (1 << 26, 4096, 7777):
             engine.set_prepare_chunk(chunk)
+            engine.set_grouping(0)
             for prec in (capi.INCA_FP64, capi.INCA_MIXED):
                 cnt = engine.prepare(prec, THRESH)
                 out[(chunk, prec)] = (list(cnt), engine.prepared(), engine.analytic_integral(),
                                       engine.mixed_split()[:2])
+            engine.set_grouping(1)
+            iv, cnt = engine.evaluate(capi.INCA_FP64, THRESH)
+            grouped[chunk] = (list(cnt), iv.copy(), engine.groups()[0])
     finally:
         engine.set_prepare_chunk(1 << 26)
+        engine.set_grouping(1)
+    for chunk in (4096, 7777):
+        a, b = grouped[chunk], grouped[1 << 26]
+        assert a[0] == b[0] and a[2] >= b[2] > 0
+        er, ea = _close(a[1], b[1])
+        print("grouped, chunks of %d (%d groups against %d): rel %.2e abs %.2e" % (
+            chunk, a[2], b[2], er, ea))
+        assert er < 1e-12 and ea < 1e-14
     assert len(d) > 8 * 7777
 
     def used(rec):
