@@ -493,6 +493,7 @@ int inca_release_records(inca_ctx *ctx) {
   if (ctx->d_q) (void)hipFree(ctx->d_q);
   ctx->d_q = nullptr;
   ctx->q_cap = 0;
+  ctx->rows_sparse = 0;
   ctx->rec_split_pending = 0;
   return 0;
 }

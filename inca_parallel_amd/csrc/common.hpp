@@ -24,6 +24,11 @@ constexpr int kNpMax = kLmaxAxis + 1;
 // k_ray_rho, 0 before), then the coefficients A*Cx(0..Lx), Cy(0..Ly),
 // Cz(0..Lz) (C(0) = highest power).
 constexpr int kQHead = 8;
+// Word 4 bit 30 of a pre-pass slot whose record failed screen 2: its words
+// 0-3 are set (so it sorts with its shell quartet) and nothing else counts.
+// Such rows exist only while a grouped preparation's rows are uncompacted
+// (rows_sparse); every record-by-record use compacts them first.
+constexpr unsigned kQDropped = 1u << 30;
 // Shell-quartet group header (group.hip): words 0-7 as a prepared record's,
 // [8] coefficient pool offset, [9] coefficient count, [10] member count,
 // [11] FP64 operations per grid point.
@@ -105,6 +110,8 @@ struct inca_ctx {
   double mixed_tail = 0.02;   // FP32 tail budget: sum of its L1 bounds / |integral|
   double mixed_tau = 0.0;     // L1-bound threshold chosen for the split
   int rec_split_pending = 0;  // mixed preparation whose record split is not made yet
+  int64_t rows_sparse = 0;    // > 0: d_q holds that many pre-pass slots, screen-2
+                              // failures marked (kQDropped), not yet compacted
   double term_integral = 0.0; // sum of the prepared terms' integrals over all space
   int tile_p[2] = {4, 4};     // grid points per lane: FP64 kernel, mixed kernel
   int poly_exact = 0;         // 1: exact axis polynomials instead of the fits

@@ -1555,6 +1555,11 @@ hipError_t noalloc_scratch_init(int device) {
 // tile records and partial rows in `scratch`, reduced in a fixed order into
 // the caller's device array ivec.  Enqueued on `st`; nothing waits.
 int evaluate_noalloc(inca_ctx *ctx, int precision, double *ivec, hipStream_t st) {
+  if (ctx->rows_sparse)  // their compaction would allocate and wait
+    return set_err(ctx, INCA_EINVAL,
+                   "the records of this grouped preparation are not compacted yet: make one "
+                   "record-by-record use first (e.g. inca_get_prepared), or prepare with "
+                   "grouping off");
   if (int rc = need_records(ctx)) return rc;
   double *base = noalloc_scratch(ctx->device);
   if (!base) return set_err(ctx, INCA_EINVAL, "no-allocation scratch not set up on device %d",

@@ -89,19 +89,22 @@ __global__ __launch_bounds__(kT) void k_group_header(const double *__restrict__ 
   if (g >= ng) return;  // whole wave
   const int lane = threadIdx.x & 63;
   const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
-  int Ls = 0;
+  int Ls = 0, live = 0;
   double B = 0.0, rcut = 0.0;
   for (int64_t m = m0 + lane; m < m1; m += 64) {
     const double *r = q + m * stride;
+    if ((unsigned)__double_as_longlong(r[4]) & kQDropped) continue;  // failed screen 2
     int lx, ly, lz;
     record_degrees(r, lx, ly, lz);
     Ls = max(Ls, lx + ly + lz);
     B += r[5];
     rcut = fmax(rcut, r[6]);
+    ++live;
   }
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     Ls = max(Ls, __shfl_xor(Ls, d, 64));
+    live += __shfl_xor(live, d, 64);
     B += __shfl_xor(B, d, 64);
     rcut = fmax(rcut, __shfl_xor(rcut, d, 64));
   }
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kT) void k_group_header(const double *__restrict__ 
   o[7] = r0[7];
   o[8] = 0.0;  // pool offset, after the scan
   o[9] = __longlong_as_double((long long)M);
-  o[10] = __longlong_as_double((long long)(m1 - m0));
+  o[10] = __longlong_as_double((long long)live);
   o[11] = 14.0 + 2.0 * M;
   mcount[g] = M;
 }
@@ -192,12 +195,16 @@ __device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int s
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // lane r: member r's degrees Lx | Ly << 8 | Lz << 16
+    // lane r: member r's degrees Lx | Ly << 8 | Lz << 16 (-1: failed screen 2)
     int info = 0;
-    if (lane < ns) info = (int)((unsigned)__double_as_longlong(lds[lane * stride + 4]) & 0xffffffu);
+    if (lane < ns) {
+      const unsigned c = (unsigned)__double_as_longlong(lds[lane * stride + 4]);
+      info = (c & kQDropped) ? -1 : (int)(c & 0xffffffu);
+    }
 #pragma unroll 2
     for (int r = 0; r < ns; ++r) {
       const int inf = __builtin_amdgcn_readlane(info, r);
+      if (inf < 0) continue;
       const int lx = inf & 0xff, ly = (inf >> 8) & 0xff, lz = (inf >> 16) & 0xff;
       const double *cx = lds + r * stride + kQHead;
       const double *cy = cx + lx + 1;
@@ -358,7 +365,8 @@ void reset_groups(inca_ctx *ctx) {
 }
 
 // Groups of the prepared rows q[0..n), which arrive sorted by group
-// (adjacent, file order within a group), appended to the context's groups:
+// (adjacent, file order within a group; rows marked kQDropped failed screen
+// 2 and count for nothing), appended to the context's groups:
 // their headers after the existing ones, their coefficients after the
 // existing pool.
 int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
@@ -525,7 +533,7 @@ int build_groups(inca_ctx *ctx, int precision) {
   reset_groups(ctx);
   if (!groupable(ctx) || ctx->q2 == 0) return 0;
   int rc;
-  if ((rc = group_rows(ctx, ctx->d_q, ctx->q2))) return rc;
+  if ((rc = group_rows(ctx, ctx->d_q, ctx->rows_sparse ? ctx->rows_sparse : ctx->q2))) return rc;
   return finish_groups(ctx, precision);
 }
 

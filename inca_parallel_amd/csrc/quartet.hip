@@ -653,6 +653,16 @@ __global__ __launch_bounds__(kT) void k_prep(
         out[6] = cutoff_radius(out + kQHead, Ls, cut_s0);
         out[7] = 0.0;
         mom = M;
+      } else {
+        // dropped by screen 2: the group words and the mark only
+        out[0] = sqe;
+        out[1] = sqe * (pik.P[0] - pjl.P[0]);
+        out[2] = sqe * (pik.P[1] - pjl.P[1]);
+        out[3] = sqe * (pik.P[2] - pjl.P[2]);
+        out[4] = __longlong_as_double((long long)kQDropped);
+        out[5] = 0.0;
+        out[6] = 0.0;
+        out[7] = 0.0;
       }
     }
     flag[n] = (uint8_t)pass2;
@@ -713,6 +723,14 @@ __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict_
       return;
   }
   dst[t] = row[w];
+}
+
+// flag[r] = 1 for the slots that passed screen 2 (no kQDropped mark).
+__global__ __launch_bounds__(kT) void k_live_flags(const double *__restrict__ q, int stride,
+                                                   int64_t n, uint8_t *__restrict__ flag) {
+  const int64_t r = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (r >= n) return;
+  flag[r] = (uint8_t)!((unsigned)__double_as_longlong(q[r * stride + 4]) & kQDropped);
 }
 
 // Histogram of the binary exponent of B (header word 5) over the records
@@ -892,6 +910,7 @@ int grow_keep(inca_ctx *ctx, T *&ptr, size_t &cap, size_t need, size_t used) {
 int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *counts) {
   hipStream_t st = ctx->stream;
   ctx->rho_h = 0.0;  // word 7 of the new records holds no rho yet
+  ctx->rows_sparse = 0;
   const int nprim = ctx->nprim;
   const int64_t nrec = ctx->nrec;
   int rc;
@@ -955,6 +974,10 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // chunk by chunk, so the prepared rows never exceed one chunk
   const bool groups_only = !ctx->keep_records && groupable(ctx);
   if (groups_only) reset_groups(ctx);
+  // one chunk, sorted by group and kept: its slots become the records as
+  // they are (screen-2 failures marked), compacted only when a record-by-
+  // record use asks (need_records); the groups are built from them directly
+  const bool sparse = sort_groups && !groups_only && nvalid <= chunk;
   for (int64_t r0 = 0; r0 < nvalid; r0 += chunk) {
     const int64_t n = std::min(chunk, nvalid - r0);
     const int32_t *recs = (const int32_t *)ctx->d_rec + 6 * r0;
@@ -1037,18 +1060,22 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     // groups)
     int64_t qc = 0;
     if ((rc = compact_flags(ctx, ctx->d_flag, n1, ctx->d_idx, &qc))) return rc;
-    const int64_t at = groups_only ? 0 : q2;
     if (groups_only) {
-      if ((rc = ensure(ctx, ctx->d_q, ctx->q_cap, (size_t)(qc > 0 ? qc : 1) * stride))) return rc;
-    } else if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 + qc) * stride,
-                               (size_t)q2 * stride))) {
-      return rc;
-    }
-    if (qc > 0) {
-      hipLaunchKernelGGL(k_gather_records, dim3(nblk(qc * stride)), dim3(kT), 0, st,
-                         ctx->d_qtmp, ctx->d_idx, qc, stride, ctx->d_q + at * stride);
-      INCA_HIP(hipGetLastError());
-      if (groups_only && (rc = group_rows(ctx, ctx->d_q, qc))) return rc;
+      // the chunk's slots grouped as they are (screen-2 failures skipped)
+      if (qc > 0 && (rc = group_rows(ctx, ctx->d_qtmp, n1))) return rc;
+    } else if (sparse) {
+      std::swap(ctx->d_q, ctx->d_qtmp);
+      std::swap(ctx->q_cap, ctx->qtmp_cap);
+      ctx->rows_sparse = n1;
+    } else {
+      if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 + qc) * stride,
+                          (size_t)q2 * stride)))
+        return rc;
+      if (qc > 0) {
+        hipLaunchKernelGGL(k_gather_records, dim3(nblk(qc * stride)), dim3(kT), 0, st,
+                           ctx->d_qtmp, ctx->d_idx, qc, stride, ctx->d_q + q2 * stride);
+        INCA_HIP(hipGetLastError());
+      }
     }
     q2 += qc;
   }
@@ -1074,8 +1101,9 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     counts[3] = (int64_t)hcnt[3];
     return finish_groups(ctx, precision);
   }
-  if ((rc = grow_keep(ctx, ctx->d_q, ctx->q_cap, (size_t)(q2 > 0 ? q2 : 1) * stride,
-                      (size_t)q2 * stride)))
+  if (!ctx->rows_sparse && (rc = grow_keep(ctx, ctx->d_q, ctx->q_cap,
+                                           (size_t)(q2 > 0 ? q2 : 1) * stride,
+                                           (size_t)q2 * stride)))
     return rc;
   ctx->q_fp64 = q2;
   ctx->mixed_tau = 0.0;
@@ -1107,6 +1135,32 @@ int need_records(inca_ctx *ctx) {
     return set_err(ctx, INCA_EINVAL,
                    "the prepared records were released (inca_release_records): only the "
                    "evaluation by shell-quartet groups remains; prepare again");
+  if (ctx->rows_sparse == 0) return 0;
+  // compact the kept slots: the screen-2 survivors in slot order
+  hipStream_t st = ctx->stream;
+  const int64_t n = ctx->rows_sparse, q2 = ctx->q2;
+  const int stride = ctx->qstride;
+  int rc;
+  if ((rc = ensure(ctx, ctx->d_flag, ctx->flag_cap, (size_t)n)) ||
+      (rc = ensure(ctx, ctx->d_idx, ctx->idx_cap, (size_t)n)) ||
+      (rc = ensure(ctx, ctx->d_qtmp, ctx->qtmp_cap, (size_t)(q2 > 0 ? q2 : 1) * stride)))
+    return rc;
+  hipLaunchKernelGGL(k_live_flags, dim3(nblk(n)), dim3(kT), 0, st, ctx->d_q, stride, n,
+                     ctx->d_flag);
+  INCA_HIP(hipGetLastError());
+  int64_t qc = 0;
+  if ((rc = compact_flags(ctx, ctx->d_flag, n, ctx->d_idx, &qc))) return rc;
+  if (qc != q2) return set_err(ctx, INCA_EINVAL, "record compaction: %lld rows for %lld records",
+                               (long long)qc, (long long)q2);
+  if (qc > 0) {
+    hipLaunchKernelGGL(k_gather_records, dim3(nblk(qc * stride)), dim3(kT), 0, st, ctx->d_q,
+                       ctx->d_idx, qc, stride, ctx->d_qtmp);
+    INCA_HIP(hipGetLastError());
+  }
+  INCA_HIP(hipStreamSynchronize(st));
+  std::swap(ctx->d_q, ctx->d_qtmp);
+  std::swap(ctx->q_cap, ctx->qtmp_cap);
+  ctx->rows_sparse = 0;
   return 0;
 }
 

@@ -173,7 +173,10 @@ int inca_evaluate_prepared(inca_ctx *ctx, int32_t precision, double *ivec);
  * non-zero FP32 tiles, up to summation order.  A mixed preparation whose
  * record split has not been made yet (grouped preparations make it at the
  * first record-by-record mixed evaluation or inca_get_mixed_split) runs
- * every record in FP64.  The tile records and partial
+ * every record in FP64.  A grouped preparation of one pre-pass chunk keeps
+ * its records uncompacted until their first record-by-record use (any of
+ * the calls that read records, e.g. inca_get_prepared with n > 0); before
+ * that, this call fails with INCA_EINVAL.  The tile records and partial
  * sums live in a static 32 MiB array per device, held until the process
  * ends: the grid must fit one partial row of G doubles per precision range
  * plus the tile records (about 4 million points in FP64, 2 million mixed);

@@ -217,3 +217,61 @@ def test_group_rays_match_point_by_point():
             assert not np.array_equal(out[P][0], out[1][0])  # the ray arithmetic ran
             assert er < 1e-11 and ea < 1e-13
             assert rr < 1e-11 and ra < 1e-13
+
+
+def _used(rec):
+    """the header and the coefficient words of each record"""
+    cls = rec[:, 4].view(np.int64)
+    n = 8 + (cls & 0xff) + ((cls >> 8) & 0xff) + ((cls >> 16) & 0xff) + 3
+    mask = np.arange(rec.shape[1])[None, :] < n[:, None]
+    return np.where(mask, rec, 0.0)
+
+
+def _sorted_rows(rec):
+    u = _used(rec)
+    return u[np.lexsort(u.T[::-1])]
+
+
+def test_grouped_records_compact_on_first_use(dsys):
+    """A grouped preparation (one chunk) keeps its pre-pass slots as they are,
+    screen-2 failures marked, and compacts them at the first record-by-record
+    use.  The compacted records are the file-order preparation's records,
+    bit for bit, in group order; the no-allocation entry point refuses the
+    uncompacted state and afterwards matches the file-order records' result
+    up to summation order."""
+    import torch
+    s, ijkl, d = dsys
+    with capi.Engine(0) as a, capi.Engine(0) as b:
+        for e in (a, b):
+            p = _setup(e, s, ijkl, d)
+            cent, Ps, sf = O.centercalc(p["an"], p["cartes"], True, False)
+            e.grid_multicenter(cent, [30] * len(Ps), [110] * len(Ps), sf, Ps, nosym=True)
+            e.set_point_cutoff(0)
+        a.set_grouping(0)
+        ca = a.prepare(capi.INCA_FP64, THRESH)
+        cb = b.prepare(capi.INCA_FP64, THRESH)
+        assert list(ca) == list(cb) and b.groups()[0] > 0
+        G = a.G
+        out = torch.full((G,), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        with pytest.raises(capi.IncaError):
+            b.evaluate_prepared_noalloc(capi.INCA_FP64, out.data_ptr(), None)
+        ra, rb = a.prepared(), b.prepared()  # b compacts here
+        assert ra.shape == rb.shape == (int(ca[2]), ra.shape[1])
+        assert not np.array_equal(_used(ra), _used(rb))  # group order, not file order
+        assert np.array_equal(_sorted_rows(ra), _sorted_rows(rb))
+        res = []
+        for e in (a, b):
+            o = torch.full((G,), float("nan"), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            e.evaluate_prepared_noalloc(capi.INCA_FP64, o.data_ptr(), None)
+            torch.cuda.synchronize()
+            res.append(o.cpu().numpy())
+        er, ea = _close(res[1], res[0])
+        print("no-allocation entry point, group-order records vs file order: rel %.2e abs %.2e"
+              % (er, ea))
+        assert er < 1e-12 and ea < 1e-14
+        # the groups are untouched by the compaction
+        iv = b.evaluate_prepared(capi.INCA_FP64)
+        er, ea = _close(iv, res[0])
+        assert er < 1e-12 and ea < 1e-14

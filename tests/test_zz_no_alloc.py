@@ -95,6 +95,7 @@ def test_noalloc_matches_allocating_entry_point(ch4):
 def test_noalloc_refuses_a_grid_beyond_its_scratch(ch4):
     with capi.Engine(0) as eng:
         eng.set_point_cutoff(0)
+        eng.set_grouping(0)
         eng.set_primitives(ch4["alpha"], ch4["tmn"], ch4["x"], ch4["y"], ch4["z"])
         eng.set_quartets(ijkl=ch4["ijkl"], d=ch4["d"])
         eng.prepare(capi.INCA_FP64, THRESH)
