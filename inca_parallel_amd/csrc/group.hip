@@ -140,19 +140,18 @@ __global__ __launch_bounds__(kT) void k_group_offsets(const int64_t *__restrict_
 
 // One wave per group; lane j of pass p owns coefficient 64 p + j and adds
 // the members' products into it in member order: (A Cx[a] Cy[b]) Cz[c].
-// The members' rows are contiguous: the wave stages kStage of them at a time
-// in LDS (all loads in flight together); lane r of a stage decodes member
-// r's degrees once, and the member loop reads them back as wave-uniform
-// values (readlane), so each member costs a lane three compares, three LDS
-// reads and the product.  `lds` holds kStage * stride doubles per wave.
-constexpr int kStage = 32;
+// The members' rows are contiguous.  Per block of 64 members, lane r reads
+// member r's degrees once; the member loop takes them back as wave-uniform
+// values (readlane) and each lane reads its three factors straight from the
+// row (the same two or three cache lines for the whole wave), so a member
+// costs a lane three compares, three loads and the product, with the loads
+// of consecutive members in flight together.
 constexpr int kMaxStride = 36;  // s/p/d records: 8 + 3 (8 + 1) doubles, rounded even
-constexpr int kLoads = kStage * kMaxStride / 64;
 
 template <int NP>
 __device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int stride, int64_t m0,
                                              int64_t m1, int Ls, int M, double *__restrict__ K,
-                                             double *lds, int lane) {
+                                             int lane) {
   // this lane's (a, b, c) per pass, in the storage order (every monomial of
   // total degree <= Ls: a = Ls..0, b = Ls - a..0, c = Ls - a - b..0)
   int la[NP], lb[NP], lc[NP];
@@ -178,35 +177,21 @@ __device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int s
       }
     }
   }
-  for (int64_t s0 = m0; s0 < m1; s0 += kStage) {
-    const int ns = (int)((m1 - s0) < kStage ? (m1 - s0) : kStage);
-    const double *src = q + s0 * stride;
-    const int nw = ns * stride;
-    double v[kLoads];
-#pragma unroll
-    for (int k = 0; k < kLoads; ++k) {
-      const int t = lane + 64 * k;
-      v[k] = (t < nw) ? src[t] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < kLoads; ++k) {
-      const int t = lane + 64 * k;
-      if (t < nw) lds[t] = v[k];
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int64_t s0 = m0; s0 < m1; s0 += 64) {
+    const int ns = (int)((m1 - s0) < 64 ? (m1 - s0) : 64);
+    const double *rows = q + s0 * stride;
     // lane r: member r's degrees Lx | Ly << 8 | Lz << 16 (-1: failed screen 2)
-    int info = 0;
+    int info = -1;
     if (lane < ns) {
-      const unsigned c = (unsigned)__double_as_longlong(lds[lane * stride + 4]);
+      const unsigned c = (unsigned)__double_as_longlong(rows[(int64_t)lane * stride + 4]);
       info = (c & kQDropped) ? -1 : (int)(c & 0xffffffu);
     }
-#pragma unroll 2
+#pragma unroll 4
     for (int r = 0; r < ns; ++r) {
       const int inf = __builtin_amdgcn_readlane(info, r);
       if (inf < 0) continue;
       const int lx = inf & 0xff, ly = (inf >> 8) & 0xff, lz = (inf >> 16) & 0xff;
-      const double *cx = lds + r * stride + kQHead;
+      const double *cx = rows + (int64_t)r * stride + kQHead;
       const double *cy = cx + lx + 1;
       const double *cz = cy + ly + 1;
 #pragma unroll
@@ -216,8 +201,6 @@ __device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int s
           acc[p] += (cx[lx - a] * cy[ly - b]) * cz[lz - c];
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -230,10 +213,8 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
                                                      const int64_t *__restrict__ first, int64_t ng,
                                                      int64_t n, const double *__restrict__ gh,
                                                      double *__restrict__ pool) {
-  extern __shared__ double lds_all[];
   const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (g >= ng) return;  // whole wave (no block barrier below)
-  double *lds = lds_all + (size_t)(threadIdx.x >> 6) * kStage * stride;
+  if (g >= ng) return;  // whole wave
   const int lane = threadIdx.x & 63;
   const double *h = gh + g * kGStride;
   const uint64_t cls = (uint64_t)__double_as_longlong(h[4]);
@@ -242,9 +223,9 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
   double *K = pool + __double_as_longlong(h[8]);
   const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
   // M <= 165 (total degree 8): at most three passes of the wave
-  if (M <= 64) group_coeffs<1>(q, stride, m0, m1, Ls, M, K, lds, lane);
-  else if (M <= 128) group_coeffs<2>(q, stride, m0, m1, Ls, M, K, lds, lane);
-  else group_coeffs<3>(q, stride, m0, m1, Ls, M, K, lds, lane);
+  if (M <= 64) group_coeffs<1>(q, stride, m0, m1, Ls, M, K, lane);
+  else if (M <= 128) group_coeffs<2>(q, stride, m0, m1, Ls, M, K, lane);
+  else group_coeffs<3>(q, stride, m0, m1, Ls, M, K, lane);
 }
 
 // Operations per point summed over the FP64 range [0, ng64) and the rest,
@@ -415,9 +396,8 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, gh);
   INCA_HIP(hipGetLastError());
   if ((rc = grow_dev(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)pool_end, (size_t)pool0))) return rc;
-  hipLaunchKernelGGL(k_group_coeffs, dim3(gblk), dim3(kT),
-                     (kT / 64) * kStage * stride * sizeof(double), st, q, stride, d_first, ng, n,
-                     gh, ctx->d_gpool);
+  hipLaunchKernelGGL(k_group_coeffs, dim3(gblk), dim3(kT), 0, st, q, stride, d_first, ng, n, gh,
+                     ctx->d_gpool);
   INCA_HIP(hipGetLastError());
   ctx->ng = ng0 + ng;
   ctx->gpool_n = pool_end;
