@@ -276,7 +276,7 @@ int finish_groups(inca_ctx *ctx, int precision);
 // the pre-pass's stable radix sort of its slots by group key
 size_t sort_slots_bytes(int64_t n);
 int sort_slots(inca_ctx *ctx, void *temp, size_t bytes, uint32_t *kin, uint32_t *kout,
-               uint32_t *vin, uint32_t *vout, int64_t n);
+               uint32_t *vin, uint32_t *vout, int64_t n, unsigned key_bits = 32);
 
 // autogrid.hip
 double bragg_slater_bohr(int z);

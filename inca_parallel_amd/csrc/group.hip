@@ -21,8 +21,9 @@
 //                   collision only splits a group, never merges two);
 //   k_group_header  one wave per group: degrees, bound, cutoff radius;
 //   exclusive scan  of the coefficient counts: the coefficient pool offsets;
-//   k_group_coeffs  one wave per group, a lane per coefficient, the members
-//                   added in their record order.
+//   k_group_coeffs  one wave per group, lanes over coefficients (and over
+//                   member slices for groups of up to 64 coefficients), the
+//                   members added in a fixed order.
 // Group header (kGStride doubles), words 0-7 as a prepared record's:
 //   [0] sqe, [1..3] offsets, [4] class Lx | Ly<<8 | Lz<<16 | axes<<24 |
 //   Ls<<32 (Ls the members' largest total degree, Lx = Ly = Lz = Ls),
@@ -138,8 +139,9 @@ __global__ __launch_bounds__(kT) void k_group_offsets(const int64_t *__restrict_
   if (g < ng) gh[g * kGStride + 8] = __longlong_as_double((long long)off[g]);
 }
 
-// One wave per group; lane j of pass p owns coefficient 64 p + j and adds
-// the members' products into it in member order: (A Cx[a] Cy[b]) Cz[c].
+// One wave per group.  Groups of more than 64 coefficients: lane j of pass
+// p owns coefficient 64 p + j and adds the members' products into it in
+// member order: (A Cx[a] Cy[b]) Cz[c].
 // The members' rows are contiguous.  Per block of 64 members, lane r reads
 // member r's degrees once; the member loop takes them back as wave-uniform
 // values (readlane) and each lane reads its three factors straight from the
@@ -209,6 +211,61 @@ __device__ __forceinline__ void group_coeffs(const double *__restrict__ q, int s
   }
 }
 
+// Groups of at most 64 coefficients: W = M rounded up to a power of two
+// lanes per coefficient set and 64 / W slices, slice s adding members s,
+// s + 64/W, ... (so a group of 10 coefficients takes four members at a
+// time), then a fixed-order tree over the slices.
+__device__ __forceinline__ void group_coeffs_sliced(const double *__restrict__ q, int stride,
+                                                    int64_t m0, int64_t m1, int Ls, int M,
+                                                    double *__restrict__ K, int lane) {
+  int W = 1;
+  while (W < M) W <<= 1;
+  const int S = 64 / W;
+  const int idx = lane & (W - 1), slice = lane / W;
+  int la = -1, lb = 0, lc = 0;
+  if (idx < M) {
+    int at = 0;
+    for (int aa = Ls; aa >= 0 && la < 0; --aa) {
+      for (int bb = Ls - aa; bb >= 0; --bb) {
+        const int cm = Ls - aa - bb;
+        if (idx <= at + cm) {
+          la = aa;
+          lb = bb;
+          lc = cm - (idx - at);
+          break;
+        }
+        at += cm + 1;
+      }
+    }
+  }
+  double acc = 0.0;
+  for (int64_t s0 = m0; s0 < m1; s0 += 64) {
+    const int ns = (int)((m1 - s0) < 64 ? (m1 - s0) : 64);
+    const double *rows = q + s0 * stride;
+    // lane r: member r's degrees Lx | Ly << 8 | Lz << 16 (-1: failed screen 2)
+    int info = -1;
+    if (lane < ns) {
+      const unsigned c = (unsigned)__double_as_longlong(rows[(int64_t)lane * stride + 4]);
+      info = (c & kQDropped) ? -1 : (int)(c & 0xffffffu);
+    }
+#pragma unroll 4
+    for (int r0 = 0; r0 < ns; r0 += S) {
+      const int r = r0 + slice;
+      const int inf = __shfl(info, r < ns ? r : 0, 64);
+      if (r < ns && inf >= 0) {
+        const int lx = inf & 0xff, ly = (inf >> 8) & 0xff, lz = (inf >> 16) & 0xff;
+        const double *cx = rows + (int64_t)r * stride + kQHead;
+        const double *cy = cx + lx + 1;
+        const double *cz = cy + ly + 1;
+        if (la >= 0 && la <= lx && lb <= ly && lc <= lz)
+          acc += (cx[lx - la] * cy[ly - lb]) * cz[lz - lc];
+      }
+    }
+  }
+  for (int d = 32; d >= W; d >>= 1) acc += __shfl_down(acc, d, 64);
+  if (lane < M) K[lane] = acc;
+}
+
 __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
                                                      const int64_t *__restrict__ first, int64_t ng,
                                                      int64_t n, const double *__restrict__ gh,
@@ -223,7 +280,7 @@ __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ 
   double *K = pool + __double_as_longlong(h[8]);
   const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
   // M <= 165 (total degree 8): at most three passes of the wave
-  if (M <= 64) group_coeffs<1>(q, stride, m0, m1, Ls, M, K, lane);
+  if (M <= 64) group_coeffs_sliced(q, stride, m0, m1, Ls, M, K, lane);
   else if (M <= 128) group_coeffs<2>(q, stride, m0, m1, Ls, M, K, lane);
   else group_coeffs<3>(q, stride, m0, m1, Ls, M, K, lane);
 }
@@ -404,8 +461,9 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   return 0;
 }
 
-// Stable sort of n (key, value) pairs by the 32-bit key (rocPRIM radix
-// sort); temp of sort_slots_bytes(n) bytes.
+// Stable sort of n (key, value) pairs by the low key_bits bits of the key
+// (rocPRIM radix sort); temp of sort_slots_bytes(n) bytes (enough for any
+// key_bits).
 size_t sort_slots_bytes(int64_t n) {
   size_t bytes = 0;
   uint32_t *k = nullptr, *v = nullptr;
@@ -415,9 +473,9 @@ size_t sort_slots_bytes(int64_t n) {
 }
 
 int sort_slots(inca_ctx *ctx, void *temp, size_t bytes, uint32_t *kin, uint32_t *kout,
-               uint32_t *vin, uint32_t *vout, int64_t n) {
+               uint32_t *vin, uint32_t *vout, int64_t n, unsigned key_bits) {
   if (n == 0) return 0;
-  INCA_HIP(rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, 32u,
+  INCA_HIP(rocprim::radix_sort_pairs(temp, bytes, kin, kout, vin, vout, (size_t)n, 0u, key_bits,
                                      ctx->stream));
   return 0;
 }

@@ -385,6 +385,19 @@ __device__ __forceinline__ int polycoef_pad(const double R[4], double Rik, doubl
   return bad;
 }
 
+// polycoef_pad at size K (K - 1 >= L), its coefficients copied into the
+// first K entries of C (the callers read C[0..L] only).
+template <int K, int NP>
+__device__ __forceinline__ int fit_into(const double R[4], double Rik, double Rjl, double Pijkl,
+                                        const int t[4], int L, double sqe, double invz,
+                                        double alf, double (&C)[NP]) {
+  double c[K];
+  const int bad = polycoef_pad<K>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, c);
+#pragma unroll
+  for (int q = 0; q < K; ++q) C[q] = c[q];
+  return bad;
+}
+
 // One axis of degree L <= NP - 1 through polycoef_pad: second-screen factor,
 // L1 bound and integral factors, and the coefficients written to out[0..L]
 // (fitted polynomials only; the exact-polynomial mode takes polycoef<NPM>).
@@ -393,8 +406,19 @@ __device__ __forceinline__ int axis_pad(const double R[4], double Rik, double Rj
                                         const int t[4], int L, double sqe, double invz,
                                         double alf, double scale, double &U, double &B,
                                         double &M, double *__restrict__ out) {
+  // the fit at the axis's own size, np = L + 1: the padded solve performs
+  // the np x np solve's operations, so every size gives the same bits; the
+  // pre-pass hands out records sorted by degrees, so the lanes of a wave
+  // mostly take one branch
   double C[NP];
-  const int bad = polycoef_pad<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  int bad;
+  switch (L) {
+    case 0: bad = fit_into<1, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
+    case 1: bad = fit_into<2, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
+    case 2: bad = fit_into<3, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
+    case 3: bad = fit_into<4, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
+    default: bad = polycoef_pad<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
+  }
   double u = 0.0;
 #pragma unroll
   for (int q = 0; q < NP; ++q)
@@ -533,6 +557,35 @@ __global__ __launch_bounds__(kT) void k_slot_records(const uint32_t *__restrict_
   dst[0] = src[0];
   dst[1] = src[1];
   dst[2] = src[2];
+}
+
+// Fit order: key[s] = Lx | Ly << 4 | Lz << 8 of slot s (record rec + 6
+// rmap[s], or s when rmap is null), val[s] = s, for a stable sort that lets
+// the lanes of a wave fit axes of one degree.
+__global__ __launch_bounds__(kT) void k_slot_class(const int32_t *__restrict__ rec, int64_t n1,
+                                                   const int64_t *__restrict__ rmap,
+                                                   const int32_t *__restrict__ tmn, int nprim,
+                                                   uint32_t *__restrict__ key,
+                                                   uint32_t *__restrict__ val) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s >= n1) return;
+  const int32_t *p = rec + 6 * (rmap ? rmap[s] : s);
+  const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+  uint32_t c = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int32_t *tt = tmn + ax * nprim;
+    const int L = tt[i] + tt[j] + tt[k] + tt[l];
+    c |= (uint32_t)(L < 15 ? L : 15) << (4 * ax);
+  }
+  key[s] = c;
+  val[s] = (uint32_t)s;
+}
+
+__global__ __launch_bounds__(kT) void k_u32_to_i64(const uint32_t *__restrict__ a, int64_t n,
+                                                   int64_t *__restrict__ b) {
+  const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (s < n) b[s] = a[s];
 }
 
 // One thread per record; the output record layout is described at kQHead.
@@ -945,13 +998,16 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // grouping: the slots are ordered by group key, so the rows of a group
   // come out adjacent (file order within the group)
   const bool sort_groups = groupable(ctx) && chunk <= (int64_t)UINT32_MAX;
-  const int64_t ns = sort_groups ? chunk : 1;
-  const size_t sort_bytes = sort_groups ? sort_slots_bytes(chunk) : 0;
+  // the register-resident fits take their slots in degree order
+  const bool sort_class = split && chunk <= (int64_t)UINT32_MAX;
+  const int64_t ns = (sort_groups || sort_class) ? chunk : 1;
+  const size_t sort_bytes = (sort_groups || sort_class) ? sort_slots_bytes(chunk) : 0;
   const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(chunk),
                o_hist = cv.add<unsigned long long>(kMagBins), o_pass = cv.add<uint8_t>(chunk),
                o_dslot = cv.add<int64_t>(deferred ? chunk : 1), o_kin = cv.add<uint32_t>(ns),
                o_kout = cv.add<uint32_t>(ns), o_vin = cv.add<uint32_t>(ns),
-               o_vout = cv.add<uint32_t>(ns), o_srec = cv.add<int32_t>(6 * ns),
+               o_vout = cv.add<uint32_t>(ns), o_srec = cv.add<int32_t>(6 * (sort_groups ? ns : 1)),
+               o_fit = cv.add<int64_t>(sort_class ? ns : 1),
                o_sort = cv.add<char>((int64_t)sort_bytes);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
   double *d_mom = Carve::at<double>(ctx->d_tmp, o_mom);
@@ -967,6 +1023,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   uint32_t *d_vin = Carve::at<uint32_t>(ctx->d_tmp, o_vin);
   uint32_t *d_vout = Carve::at<uint32_t>(ctx->d_tmp, o_vout);
   int32_t *d_srec = Carve::at<int32_t>(ctx->d_tmp, o_srec);
+  int64_t *d_fit = Carve::at<int64_t>(ctx->d_tmp, o_fit);
   char *d_sort = Carve::at<char>(ctx->d_tmp, o_sort);
   double mom_total = 0.0;  // chunk totals added in file order
   int64_t q2 = 0;
@@ -1006,12 +1063,24 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       srec = d_srec;
       slot_rec = nullptr;
     }
+    // fit order: the slots stably sorted by their three degrees
+    const int64_t *fit_order = nullptr;
+    if (sort_class && n1 > 0) {
+      hipLaunchKernelGGL(k_slot_class, dim3(nblk(n1)), dim3(kT), 0, st, srec, n1, slot_rec,
+                         ctx->d_tmn, nprim, d_kin, d_vin);
+      INCA_HIP(hipGetLastError());
+      if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1, 12u)))
+        return rc;
+      hipLaunchKernelGGL(k_u32_to_i64, dim3(nblk(n1)), dim3(kT), 0, st, d_vout, n1, d_fit);
+      INCA_HIP(hipGetLastError());
+      fit_order = d_fit;
+    }
     // the small-degree pass's nblk(n1) partials end at d_mom + nb1, where the
     // deferred pass's begin, so that one fixed-order sum covers both
     double *small_mom = d_mom + (nb1 - (int64_t)nblk(n1));
     if (split && n1 > 0) {
       hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, srec, n1,
-                         (const int64_t *)nullptr, slot_rec, nprim,
+                         fit_order, slot_rec, nprim,
                          ctx->d_prim, ctx->d_tmn, ctx->d_pair,
                          lim, ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
                          ctx->d_qtmp, ctx->d_flag, d_defer, ctx->d_cnt, small_mom);

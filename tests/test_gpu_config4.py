@@ -3,10 +3,12 @@
 its own autogrid (nrad 50-150, nang 110-590, nosym), streamed through the
 persistent hipGraph-captured batch in arena-sized groups.
 
-All 256 molecules together are about 6e10 records and 2.8e16 evaluations:
-roughly five hours of this device.  The checks here run a stratified subset
-at the molecules' stated sizes: the cheapest molecule of every atom-count
-band plus the cheapest overall.  They are checked through properties that
+All 256 molecules together are about 6e10 records and 3e16 reference
+evaluations: about 45 minutes of the device in FP64 and mixed precision
+(`python -m inca_parallel_amd.batch_stream --rank lo:hi` runs slices of
+them by cost rank).  The checks here run a stratified subset at the
+molecules' stated sizes: the cheapest molecule of every atom-count band
+plus the cheapest overall.  They are checked through properties that
 do not depend on size:
   * the batch against the same molecules' prepared quartets evaluated one
     context at a time (FP64, 1e-12 relative on the integral);
