@@ -308,13 +308,13 @@ def target_bench(capi, torch, nmol=32):
     }
 
 
-PROFILE = os.path.join(ROOT, "profiles", "eval_fp64_radial_plot_cluster151.md")
+PROFILE = os.path.join(ROOT, "profiles", "geval_fp64_radial_plot_cluster151.md")
 
 
 def profiled_traffic():
-    """HBM bytes of one FP64 step's ray-tile kernel on this workload, from
-    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a
-    1 GiB copy)."""
+    """HBM bytes of one FP64 step's group kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a 1 GiB
+    copy)."""
     try:
         for line in open(PROFILE):
             if line.startswith("traffic_bytes_per_fp64_step_eval"):

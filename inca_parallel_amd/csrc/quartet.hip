@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kT) void k_prep(
     double cut_s0, double *__restrict__ qout, uint8_t *__restrict__ flag,
     uint8_t *__restrict__ defer, unsigned long long *__restrict__ cnt,
     double *__restrict__ msum) {
-  static_assert(!SMALL || NPM <= 5, "the compile-time fits cover degrees 0-4");
+  static_assert(!SMALL || NPM <= 9, "the compile-time fits cover degrees 0-8");
   const int64_t g = (int64_t)blockIdx.x * kT + threadIdx.x;
   int pass1 = 0, pass2 = 0, nfail = 0;
   double mom = 0.0;  // signed whole-space integral of a surviving term
@@ -1101,6 +1101,11 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
                      ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
                      (uint8_t *)nullptr, ctx->d_cnt, run_mom)
       if (la <= 4) INCA_PREP(5);
+      else if (la <= 8 && split)  // fitted polynomials up to d: the fits in registers
+        hipLaunchKernelGGL((k_prep<9, true>), dim3(nblk(nrun)), dim3(kT), 0, st, srec, nrun,
+                           run_idx, slot_rec, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,
+                           ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
+                           ctx->d_qtmp, ctx->d_flag, (uint8_t *)nullptr, ctx->d_cnt, run_mom);
       else if (la <= 8) INCA_PREP(9);
       else if (la <= 12) INCA_PREP(13);
       else if (la <= 16) INCA_PREP(17);
