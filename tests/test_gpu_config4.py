@@ -4,7 +4,7 @@ its own autogrid (nrad 50-150, nang 110-590, nosym), streamed through the
 persistent hipGraph-captured batch in arena-sized groups.
 
 All 256 molecules together are about 6e10 records and 3e16 reference
-evaluations: about 45 minutes of the device in FP64 and mixed precision
+evaluations: about an hour of the device in FP64 and mixed precision
 (`python -m inca_parallel_amd.batch_stream --rank lo:hi` runs slices of
 them by cost rank).  The checks here run a stratified subset at the
 molecules' stated sizes: the cheapest molecule of every atom-count band
