@@ -663,6 +663,66 @@ __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
   const double p0x = fma(h.sqe, o[0], h.ax), p0y = fma(h.sqe, o[1], h.ay),
                p0z = fma(h.sqe, o[2], h.az);
   const double dx = h.sqe * hw[0], dy = h.sqe * hw[1], dz = h.sqe * hw[2];
+  if constexpr (P == 8) {
+    // eight points as two sequential halves of four (a loop that is not
+    // unrolled, the two halves' sums swapped between its turns): the two
+    // anchor exponentials spread over eight points, the polynomial keeps
+    // the registers of four
+    const bool rec = fma(h.sqe, span, rcut) <= kRayMaxRp;
+    const double rho = gh[7];
+    double e = 0.0, qk = 0.0;
+    if (rec) {
+      const double s0 = fma(p0x, p0x, fma(p0y, p0y, p0z * p0z));
+      const double dot = fma(dx, p0x, fma(dy, p0y, dz * p0z));
+      const double dd = h.sqe * h.sqe * h2;  // |d|^2 = (sqe h)^2
+      e = exp_neg(s0, tbl);
+      qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
+    }
+    double a0[4], a1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a0[k] = acc[k];
+      a1[k] = acc[k + 4];
+    }
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+      double px[4], py[4], pz[4], w[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int kk = 4 * half + k;
+        px[k] = kk == 0 ? p0x : fma((double)kk, dx, p0x);
+        py[k] = kk == 0 ? p0y : fma((double)kk, dy, p0y);
+        pz[k] = kk == 0 ? p0z : fma((double)kk, dz, p0z);
+      }
+      group_poly_any<4, double, double>(pool + h.off, h, px, py, pz, w);
+      if (rec) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          a0[k] = fma(e, w[k], a0[k]);
+          e *= qk;
+          qk *= rho;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double s = fma(px[k], px[k], fma(py[k], py[k], pz[k] * pz[k]));
+          a0[k] = fma(exp_neg(s, tbl), w[k], a0[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double t = a0[k];
+        a0[k] = a1[k];
+        a1[k] = t;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[k] = a0[k];
+      acc[k + 4] = a1[k];
+    }
+    return;
+  }
   double px[P], py[P], pz[P], v[P];
 #pragma unroll
   for (int k = 0; k < P; ++k) {
