@@ -120,9 +120,10 @@ __global__ __launch_bounds__(kT) void k_geval(const double *__restrict__ gh, int
 
 // Shell-quartet groups on the ray tiles of a radial scan (FP64): the lanes
 // of k_eval_rays (one direction at P consecutive radii), each group by
-// group_ray.  Eight points per lane ask for occupancy 4 (128 VGPRs, 28
-// B/lane saved to scratch): 37.0 ms on the radial-plot step against 40.7 ms
-// at occupancy 3 and 38.3 ms at four points per lane.
+// group_ray.  Eight points per lane ask for occupancy 4 (128 VGPRs, 44
+// B/lane saved to scratch): with the polynomial's values at t = 4..7 taken
+// from its difference table for total degrees up to 5, 28.2 ms on the
+// radial-plot step, against 38.3 ms at four points per lane.
 template <int P, bool CUT>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(P == 8 ? 4 : 1))) void
 k_geval_rays(const double *__restrict__ gh, int64_t ng,

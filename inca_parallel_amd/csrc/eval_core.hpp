@@ -647,6 +647,89 @@ __device__ __forceinline__ void group_fp64(const double *__restrict__ gh,
   }
 }
 
+// The polynomial at the four ray points t = 4 H .. 4 H + 3.
+template <int H>
+__device__ __forceinline__ void ray_half(const double *__restrict__ pool, const GHeader &h,
+                                         double p0x, double p0y, double p0z, double dx,
+                                         double dy, double dz, double (&w)[4]) {
+  double px[4], py[4], pz[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int kk = 4 * H + k;
+    px[k] = kk == 0 ? p0x : fma((double)kk, dx, p0x);
+    py[k] = kk == 0 ? p0y : fma((double)kk, dy, p0y);
+    pz[k] = kk == 0 ? p0z : fma((double)kk, dz, p0z);
+  }
+  group_poly_any<4, double, double>(pool + h.off, h, px, py, pz, w);
+}
+
+// The polynomial at the single ray point t = T.
+template <int T>
+__device__ __forceinline__ void ray_point(const double *__restrict__ pool, const GHeader &h,
+                                          double p0x, double p0y, double p0z, double dx,
+                                          double dy, double dz, double &v) {
+  const double px[1] = {fma((double)T, dx, p0x)}, py[1] = {fma((double)T, dy, p0y)},
+               pz[1] = {fma((double)T, dz, p0z)};
+  double w[1];
+  group_poly_any<1, double, double>(pool + h.off, h, px, py, pz, w);
+  v = w[0];
+}
+
+// Values at t = 4..7 of a polynomial of degree L in t from its values
+// v[0..L] at t = 0..L: the values themselves up to t = L, then steps of the
+// backward-difference table at t = L (nabla^L constant).
+template <int L>
+__device__ __forceinline__ void ray_extrapolate(const double (&v)[6], double (&w)[4]) {
+  double a[L + 1], n[L + 1];
+#pragma unroll
+  for (int i = 0; i <= L; ++i) a[i] = v[i];
+  n[0] = a[L];
+#pragma unroll
+  for (int k = 1; k <= L; ++k) {
+#pragma unroll
+    for (int i = L; i >= k; --i) a[i] = a[i] - a[i - 1];
+    n[k] = a[L];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (4 + k <= L) {
+      w[k] = v[4 + k];
+    } else {
+#pragma unroll
+      for (int j = L - 1; j >= 0; --j) n[j] += n[j + 1];
+      w[k] = n[0];
+    }
+  }
+}
+
+// acc[4 H + k] += exp(-|rp_t|^2) w[k] for t = 4 H + k: by the recurrence
+// (rec: e and qk carried from point to point) or point by point.
+template <int H>
+__device__ __forceinline__ void ray_accumulate(bool rec, double &e, double &qk, double rho,
+                                               const double (&w)[4], double p0x, double p0y,
+                                               double p0z, double dx, double dy, double dz,
+                                               double (&acc)[8],
+                                               const double *__restrict__ tbl) {
+  if (rec) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[4 * H + k] = fma(e, w[k], acc[4 * H + k]);
+      e *= qk;
+      qk *= rho;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int kk = 4 * H + k;
+      const double px = kk == 0 ? p0x : fma((double)kk, dx, p0x);
+      const double py = kk == 0 ? p0y : fma((double)kk, dy, p0y);
+      const double pz = kk == 0 ? p0z : fma((double)kk, dz, p0z);
+      const double s = fma(px, px, fma(py, py, pz * pz));
+      acc[kk] = fma(exp_neg(s, tbl), w[k], acc[kk]);
+    }
+  }
+}
+
 // One group on a ray lane of P points o + k h w (k = 0 .. P-1): rp_k =
 // rp_0 + k d with d = sqe h w, the polynomial at each point, and exp(-s_k)
 // by the two-product recurrence of the record ray kernel (e_{k+1} = e_k q_k,
@@ -664,10 +747,9 @@ __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
                p0z = fma(h.sqe, o[2], h.az);
   const double dx = h.sqe * hw[0], dy = h.sqe * hw[1], dz = h.sqe * hw[2];
   if constexpr (P == 8) {
-    // eight points as two sequential halves of four (a loop that is not
-    // unrolled, the two halves' sums swapped between its turns): the two
-    // anchor exponentials spread over eight points, the polynomial keeps
-    // the registers of four
+    // eight points as two sequential halves of four: the two anchor
+    // exponentials spread over eight points, the polynomial keeps the
+    // registers of four
     const bool rec = fma(h.sqe, span, rcut) <= kRayMaxRp;
     const double rho = gh[7];
     double e = 0.0, qk = 0.0;
@@ -678,49 +760,44 @@ __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
       e = exp_neg(s0, tbl);
       qk = exp_neg(fmax(fma(2.0, dot, dd), -700.0), tbl);
     }
-    double a0[4], a1[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      a0[k] = acc[k];
-      a1[k] = acc[k + 4];
-    }
-#pragma unroll 1
-    for (int half = 0; half < 2; ++half) {
-      double px[4], py[4], pz[4], w[4];
+    // Along the ray the group's polynomial is one of degree Ls in the step
+    // index t, so for Ls <= 3 its values at t = 4..7 follow from those at
+    // t = 0..3 by the backward-difference table at t = 3 (three additions
+    // per point in place of M fused multiply-adds).  The two halves are
+    // separate regions of code, so their polynomials do not interleave.
+    double w[4];
+    ray_half<0>(pool, h, p0x, p0y, p0z, dx, dy, dz, w);
+    ray_accumulate<0>(rec, e, qk, rho, w, p0x, p0y, p0z, dx, dy, dz, acc, tbl);
+    if (h.Ls <= 3) {
+      const double d1 = w[3] - w[2], d1b = w[2] - w[1], d1c = w[1] - w[0];
+      const double d2 = d1 - d1b, d2b = d1b - d1c;
+      // differences of degree above Ls are zero, not roundoff
+      double pv = w[3];
+      double n1 = h.Ls >= 1 ? d1 : 0.0;
+      double n2 = h.Ls >= 2 ? d2 : 0.0;
+      const double n3 = h.Ls >= 3 ? d2 - d2b : 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int kk = 4 * half + k;
-        px[k] = kk == 0 ? p0x : fma((double)kk, dx, p0x);
-        py[k] = kk == 0 ? p0y : fma((double)kk, dy, p0y);
-        pz[k] = kk == 0 ? p0z : fma((double)kk, dz, p0z);
+        n2 += n3;
+        n1 += n2;
+        pv += n1;
+        w[k] = pv;
       }
-      group_poly_any<4, double, double>(pool + h.off, h, px, py, pz, w);
-      if (rec) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          a0[k] = fma(e, w[k], a0[k]);
-          e *= qk;
-          qk *= rho;
-        }
+    } else if (h.Ls <= 5) {
+      // degrees 4 and 5: the polynomial at t = 4 (and 5) by Horner, the
+      // remaining points from the difference table of order Ls at t = Ls
+      double v[6] = {w[0], w[1], w[2], w[3], 0.0, 0.0};
+      ray_point<4>(pool, h, p0x, p0y, p0z, dx, dy, dz, v[4]);
+      if (h.Ls == 4) {
+        ray_extrapolate<4>(v, w);
       } else {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const double s = fma(px[k], px[k], fma(py[k], py[k], pz[k] * pz[k]));
-          a0[k] = fma(exp_neg(s, tbl), w[k], a0[k]);
-        }
+        ray_point<5>(pool, h, p0x, p0y, p0z, dx, dy, dz, v[5]);
+        ray_extrapolate<5>(v, w);
       }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double t = a0[k];
-        a0[k] = a1[k];
-        a1[k] = t;
-      }
+    } else {
+      ray_half<1>(pool, h, p0x, p0y, p0z, dx, dy, dz, w);
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      acc[k] = a0[k];
-      acc[k + 4] = a1[k];
-    }
+    ray_accumulate<1>(rec, e, qk, rho, w, p0x, p0y, p0z, dx, dy, dz, acc, tbl);
     return;
   }
   double px[P], py[P], pz[P], v[P];
