@@ -21,6 +21,7 @@
 // record, so a wavefront streams 1536 contiguous bytes.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -828,9 +829,13 @@ __global__ __launch_bounds__(kT) void k_split_flags(const double *__restrict__ q
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
 
+// Contexts on several host threads may prepare at once: the first upload
+// per device is made under a lock.
 bool g_const_ready[64] = {false};
+std::mutex g_const_mu;
 
 int upload_constants(inca_ctx *ctx) {
+  std::lock_guard<std::mutex> lock(g_const_mu);
   if (ctx->device >= 0 && ctx->device < 64 && g_const_ready[ctx->device]) return 0;
   double w[11][11] = {};
   const double sqrtpi = std::sqrt(std::acos(-1.0));
