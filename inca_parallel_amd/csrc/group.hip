@@ -272,38 +272,23 @@ __device__ __forceinline__ void group_coeffs_sliced(const double *__restrict__ q
   if (lane < M) K[lane] = acc;
 }
 
-// Two groups per wave, 2 w and 2 w + 1: both headers, member ranges and
-// first blocks of member degrees are read before either is summed, so the
-// second group's loads are in flight while the first is computed.
 __global__ __launch_bounds__(kT) void k_group_coeffs(const double *__restrict__ q, int stride,
                                                      const int64_t *__restrict__ first, int64_t ng,
                                                      int64_t n, const double *__restrict__ gh,
                                                      double *__restrict__ pool) {
-  const int64_t g0 = 2 * ((int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6));
-  if (g0 >= ng) return;  // whole wave
+  const int64_t g = (int64_t)blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (g >= ng) return;  // whole wave
   const int lane = threadIdx.x & 63;
-  const int ngw = (g0 + 1 < ng) ? 2 : 1;
-  int Ls[2], M[2], info[2];
-  int64_t off[2], m0[2], m1[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int64_t g = g0 + (j < ngw ? j : 0);
-    const double *h = gh + g * kGStride;
-    Ls[j] = (int)(((uint64_t)__double_as_longlong(h[4]) >> 32) & 0xff);
-    M[j] = (int)__double_as_longlong(h[9]);
-    off[j] = __double_as_longlong(h[8]);
-    m0[j] = first[g];
-    m1[j] = (g + 1 < ng) ? first[g + 1] : n;
-  }
-#pragma unroll
-  for (int j = 0; j < 2; ++j) info[j] = member_info(q, stride, m0[j], m1[j], lane);
-  for (int j = 0; j < ngw; ++j) {
-    double *K = pool + off[j];
-    // M <= 165 (total degree 8): at most three passes of the wave
-    if (M[j] <= 64) group_coeffs_sliced(q, stride, m0[j], m1[j], Ls[j], M[j], K, lane, info[j]);
-    else if (M[j] <= 128) group_coeffs<2>(q, stride, m0[j], m1[j], Ls[j], M[j], K, lane);
-    else group_coeffs<3>(q, stride, m0[j], m1[j], Ls[j], M[j], K, lane);
-  }
+  const double *h = gh + g * kGStride;
+  const int Ls = (int)(((uint64_t)__double_as_longlong(h[4]) >> 32) & 0xff);
+  const int M = (int)__double_as_longlong(h[9]);
+  double *K = pool + __double_as_longlong(h[8]);
+  const int64_t m0 = first[g], m1 = (g + 1 < ng) ? first[g + 1] : n;
+  // M <= 165 (total degree 8): at most three passes of the wave
+  if (M <= 64)
+    group_coeffs_sliced(q, stride, m0, m1, Ls, M, K, lane, member_info(q, stride, m0, m1, lane));
+  else if (M <= 128) group_coeffs<2>(q, stride, m0, m1, Ls, M, K, lane);
+  else group_coeffs<3>(q, stride, m0, m1, Ls, M, K, lane);
 }
 
 // Operations per point summed over the FP64 range [0, ng64) and the rest,
@@ -474,8 +459,7 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   hipLaunchKernelGGL(k_group_offsets, dim3(nblk(ng)), dim3(kT), 0, st, d_off, ng, gh);
   INCA_HIP(hipGetLastError());
   if ((rc = grow_dev(ctx, ctx->d_gpool, ctx->gpool_cap, (size_t)pool_end, (size_t)pool0))) return rc;
-  const unsigned cblk = (unsigned)((ng + 2 * (kT / 64) - 1) / (2 * (kT / 64)));  // two per wave
-  hipLaunchKernelGGL(k_group_coeffs, dim3(cblk), dim3(kT), 0, st, q, stride, d_first, ng, n, gh,
+  hipLaunchKernelGGL(k_group_coeffs, dim3(gblk), dim3(kT), 0, st, q, stride, d_first, ng, n, gh,
                      ctx->d_gpool);
   INCA_HIP(hipGetLastError());
   ctx->ng = ng0 + ng;
