@@ -337,6 +337,49 @@ def profiled_traffic():
     return None
 
 
+def dm2p_file_bench(capi, eng, recs, step_s, rintra):
+    """The drop-in path from a file: the radial-plot workload's records written
+    as a stream-format .dm2p (24-byte records) in the host's temporary
+    directory, flushed and dropped from the page cache, then read by
+    inca_load_dm2p (pinned double-buffered copies to the device) and one FP64
+    step on them; a second read from the page cache beside it."""
+    import tempfile
+    host = recs.cpu().numpy()
+    nbytes = host.nbytes
+    fd, path = tempfile.mkstemp(suffix=".dm2p")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            f.write(host.tobytes())
+            f.flush()
+            os.fsync(f.fileno())
+        del host
+        with open(path, "rb") as f:
+            os.posix_fadvise(f.fileno(), 0, 0, os.POSIX_FADV_DONTNEED)
+        t0 = time.perf_counter()
+        n, fmt = eng.load_dm2p(path)
+        t_cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        eng.evaluate(capi.INCA_FP64, THRESH, want_ivec=False)
+        t_step = time.perf_counter() - t0
+        same = bool(np.array_equal(eng.radial_values(), rintra))
+        t0 = time.perf_counter()
+        eng.load_dm2p(path)
+        t_warm = time.perf_counter() - t0
+    finally:
+        os.unlink(path)
+    return {
+        "bytes": nbytes, "records": int(n), "format": int(fmt),
+        "load_s": t_cold, "load_GB_per_s": nbytes / t_cold / 1e9,
+        "load_page_cache_s": t_warm, "load_page_cache_GB_per_s": nbytes / t_warm / 1e9,
+        "step_after_load_s": t_step, "drop_in_s": t_cold + t_step,
+        "step_resident_s": step_s, "rintra_equal_to_resident_path": same,
+        "note": "stream .dm2p written to the host's temporary directory, fsync'd and dropped from "
+                "the page cache (posix_fadvise), read by inca_load_dm2p; drop-in = that read plus "
+                "one FP64 step (pre-pass and grid loop); a second read from the page cache beside "
+                "it; the bench line's value keeps the records resident",
+    }
+
+
 def cube_bench(capi, eng):
     """The cube workload on the same cluster and DM2: 80^3 voxels, step 0.15
     bohr, centred at the origin (the cubeintra grid).  One FP64 and one mixed
@@ -378,6 +421,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dm2p-file", action="store_true",
+                    help="skip the leg that reads the workload from a .dm2p file")
     ap.add_argument("--natoms", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-molecules", type=int, default=32,
@@ -529,6 +574,9 @@ def main():
                       "reduce": float(r64["kern_ms"][2])},
     }
     grid_r = eng.get_grid()["r"]  # the radial-plot grid, for the CPU baseline
+    if not a.no_dm2p_file:
+        out["dm2p_file"] = dm2p_file_bench(capi, eng, recs, r64["step_s"], r64["rintra"])
+        log("dm2p_file: %s" % out["dm2p_file"])
     if not a.no_cube:
         out["cube"] = cube_bench(capi, eng)
         log("cube: %s" % out["cube"])

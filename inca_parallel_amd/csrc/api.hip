@@ -88,6 +88,8 @@ int inca_destroy(inca_ctx *c) {
     if (p) (void)hipFree(p);
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (char *h : c->h_pin)
+    if (h) (void)hipHostFree(h);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -259,18 +261,46 @@ int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec, int32_t *form
     std::fclose(f);
     return INCA_ENOMEM;
   }
+  // Double-buffered: chunk c is read (and, with markers, repacked) into
+  // pinned buffer c % 2 while the copy of chunk c - 1 to the device runs on
+  // the context's stream; a buffer is refilled once its last copy is done.
   const size_t chunk_rec = 1 << 21;  // 2M records = 48 MiB per staging copy
-  std::vector<char> buf(chunk_rec * 32), pay(chunk_rec * 24);
+  const size_t pin = chunk_rec * 24;
+  if (ctx->pin_bytes < pin) {
+    for (char *&h : ctx->h_pin) {
+      if (h) (void)hipHostFree(h);
+      h = nullptr;
+    }
+    ctx->pin_bytes = 0;
+    for (char *&h : ctx->h_pin)
+      if (hipHostMalloc((void **)&h, pin, hipHostMallocDefault) != hipSuccess) {
+        std::fclose(f);
+        (void)hipGetLastError();
+        return set_err(ctx, INCA_ENOMEM, "hipHostMalloc of %zu bytes failed", pin);
+      }
+    ctx->pin_bytes = pin;
+  }
+  std::vector<char> buf(fmt == 1 ? chunk_rec * 32 : 0);
+  hipEvent_t done[2] = {nullptr, nullptr};
+  for (auto &ev : done)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      std::fclose(f);
+      for (auto &e2 : done)
+        if (e2) (void)hipEventDestroy(e2);
+      return set_err(ctx, INCA_EHIP, "hipEventCreate failed");
+    }
+  bool used[2] = {false, false};
   int64_t got = 0;
   bool stop = false;
-  while (!stop && got < cap) {
+  hipError_t e = hipSuccess;
+  for (int c = 0; !stop && got < cap && e == hipSuccess; ++c) {
+    char *stage = ctx->h_pin[c & 1];
+    if (used[c & 1] && (e = hipEventSynchronize(done[c & 1])) != hipSuccess) break;
     size_t want = (size_t)std::min<int64_t>(chunk_rec, cap - got);
-    size_t rs = fmt == 1 ? 32 : 24;
-    size_t n = std::fread(buf.data(), rs, want, f);
-    if (n < want) stop = true;
-    size_t keep = n;
+    size_t keep = 0;
     if (fmt == 1) {
-      keep = 0;
+      const size_t n = std::fread(buf.data(), 32, want, f);
+      if (n < want) stop = true;
       for (size_t r = 0; r < n; ++r) {
         int32_t m0, m1;
         std::memcpy(&m0, buf.data() + 32 * r, 4);
@@ -279,21 +309,26 @@ int inca_load_dm2p(inca_ctx *ctx, const char *path, int64_t *nrec, int32_t *form
           stop = true;
           break;
         }
-        std::memcpy(pay.data() + 24 * keep, buf.data() + 32 * r + 4, 24);
+        std::memcpy(stage + 24 * keep, buf.data() + 32 * r + 4, 24);
         ++keep;
       }
     } else {
-      std::memcpy(pay.data(), buf.data(), 24 * n);
+      keep = std::fread(stage, 24, want, f);
+      if (keep < want) stop = true;
     }
     if (keep) {
-      hipError_t e = hipMemcpy((char *)ctx->d_rec + 24 * got, pay.data(), 24 * keep,
-                               hipMemcpyHostToDevice);
-      if (e != hipSuccess) {
-        std::fclose(f);
-        return hip_fail(ctx, e, "hipMemcpy(.dm2p)", __LINE__);
-      }
+      e = hipMemcpyAsync((char *)ctx->d_rec + 24 * got, stage, 24 * keep, hipMemcpyHostToDevice,
+                         ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(done[c & 1], ctx->stream);
+      used[c & 1] = true;
     }
     got += (int64_t)keep;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  for (auto &ev : done) (void)hipEventDestroy(ev);
+  if (e != hipSuccess) {
+    std::fclose(f);
+    return hip_fail(ctx, e, "copy of .dm2p records", __LINE__);
   }
   std::fclose(f);
   ctx->nrec = got;
@@ -459,6 +494,11 @@ int inca_release_scratch(inca_ctx *ctx) {
     p = nullptr;
     cap = 0;
   };
+  for (char *&h : ctx->h_pin) {
+    if (h) (void)hipHostFree(h);
+    h = nullptr;
+  }
+  ctx->pin_bytes = 0;
   drop(ctx->d_rec, ctx->rec_cap);
   ctx->nrec = 0;
   drop(ctx->d_qtmp, ctx->qtmp_cap);

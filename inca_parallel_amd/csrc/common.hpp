@@ -101,6 +101,9 @@ struct inca_ctx {
   int64_t nrec = 0;
   char *d_rec = nullptr;
   size_t rec_cap = 0;
+  // .dm2p reader: two pinned staging buffers (hipHostMalloc), kept
+  char *h_pin[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
 
   // prepared quartets
   int qstride = 0;       // doubles per prepared record
