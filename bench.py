@@ -412,6 +412,30 @@ def cube_bench(capi, eng):
     out["mixed"]["max_rel_err_voxels_vs_fp64"] = float(
         np.max(np.abs(iv - iv64)[big] / np.abs(iv64[big])))
     out["mixed"]["rel_err_sum_vs_fp64"] = float(abs(iv.sum() - iv64.sum()) / abs(iv64.sum()))
+    # FP64 by shell-quartet groups on the z lines of the cube (ray tiles,
+    # per-point cutoff), against the contraction's every term at every voxel
+    eng.set_cube_contraction(0)
+    try:
+        eng.prepare(capi.INCA_FP64, THRESH)
+        eng.evaluate_prepared(capi.INCA_FP64, want_ivec=False)
+        t0 = time.perf_counter()
+        ivg = eng.evaluate_prepared(capi.INCA_FP64, want_ivec=True)
+        wall = time.perf_counter() - t0
+        tm = eng.timings()
+        ref, ex = eng.eval_counts()
+        out["groups_fp64"] = {
+            "eval_ms": float(tm[1] + tm[2]), "wall_s": wall,
+            "evals_per_s": ref / ((tm[1] + tm[2]) * 1e-3),
+            "executed_group_evals": ex, "groups": int(eng.groups()[0]),
+            "max_rel_err_voxels_vs_contraction": float(
+                np.max(np.abs(ivg - iv64)[big] / np.abs(iv64[big]))),
+            "max_abs_err_vs_contraction_over_max": float(
+                np.abs(ivg - iv64).max() / np.abs(iv64).max()),
+            "note": "shell-quartet groups on ray tiles along z (eight points per lane, the "
+                    "polynomial's later values from its difference table), per-point cutoff "
+                    "2^-100 of each term's peak"}
+    finally:
+        eng.set_cube_contraction(2)
     return out
 
 

@@ -118,13 +118,13 @@ __global__ __launch_bounds__(kT) void k_geval(const double *__restrict__ gh, int
   }
 }
 
-// Shell-quartet groups on the ray tiles of a radial scan (FP64): the lanes
-// of k_eval_rays (one direction at P consecutive radii), each group by
-// group_ray.  Eight points per lane ask for occupancy 4 (128 VGPRs, 44
+// Shell-quartet groups on ray tiles (FP64): the lanes of k_eval_rays (one
+// direction at P consecutive radii of a radial scan, or P consecutive
+// points of a cube line along z), each group by group_ray.  Eight points per lane ask for occupancy 4 (128 VGPRs, 44
 // B/lane saved to scratch): with the polynomial's values at t = 4..7 taken
 // from its difference table for total degrees up to 5, 28.2 ms on the
 // radial-plot step, against 38.3 ms at four points per lane.
-template <int P, bool CUT>
+template <int P, bool CUT, bool ZLINE>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(P == 8 ? 4 : 1))) void
 k_geval_rays(const double *__restrict__ gh, int64_t ng,
                                                    int64_t gchunk, const double *__restrict__ pool,
@@ -150,11 +150,15 @@ k_geval_rays(const double *__restrict__ gh, int64_t ng,
     o[0] = gx[g0];
     o[1] = gy[g0];
     o[2] = gz[g0];
-    const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
-    const double f = r0 > 0.0 ? h / r0 : 0.0;
-    hw[0] = f * o[0];
-    hw[1] = f * o[1];
-    hw[2] = f * o[2];
+    if (ZLINE) {  // cube line along z
+      hw[2] = h;
+    } else {  // radial ray: w = o / |o|
+      const double r0 = sqrt(fma(o[0], o[0], fma(o[1], o[1], o[2] * o[2])));
+      const double f = r0 > 0.0 ? h / r0 : 0.0;
+      hw[0] = f * o[0];
+      hw[1] = f * o[1];
+      hw[2] = f * o[2];
+    }
   }
   const double span = (P - 1) * h;
   double acc[P];
@@ -951,10 +955,10 @@ int launch_eval(inca_ctx *ctx, bool mixed, int P, const double *q, int64_t n, in
 int group_tiles(inca_ctx *ctx, bool mixed, int P, int64_t *tiles, bool *rays) {
   *rays = false;
   *tiles = (ctx->G + (int64_t)kT * P - 1) / ((int64_t)kT * P);
-  if (mixed || P < 2 || ctx->grid_kind != GRID_RADIAL) return 0;
+  if (mixed || P < 2 || (ctx->grid_kind != GRID_RADIAL && ctx->grid_kind != GRID_CUBE)) return 0;
   int rc;
   if ((rc = ray_plan(ctx, P))) return rc;
-  if (ctx->ray_P != P || ctx->ray_kind != 1) return 0;
+  if (ctx->ray_P != P || (ctx->ray_kind != 1 && ctx->ray_kind != 2)) return 0;
   *rays = true;
   *tiles = ctx->ray_ntile / 4;
   return 0;
@@ -1018,10 +1022,16 @@ int launch_geval_part(inca_ctx *ctx, bool mixed, int P, const double *gh, int64_
   INCA_HIP(hipGetLastError());
 #define INCA_GEVAL2(PP, CC)                                                                    \
   if (rays) {                                                                                  \
-    if constexpr (PP >= 2)                                                                     \
-      hipLaunchKernelGGL((k_geval_rays<PP, CC>), grid, dim3(kT), 0, st, gh, n, gch,            \
-                         ctx->d_gpool, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,          \
-                         ctx->d_mask, ntile, tiles, nch, xcd, ridx, ctx->ray_h, partial);      \
+    if constexpr (PP >= 2) {                                                                   \
+      if (ctx->ray_kind == 2)                                                                  \
+        hipLaunchKernelGGL((k_geval_rays<PP, CC, true>), grid, dim3(kT), 0, st, gh, n, gch,    \
+                           ctx->d_gpool, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,        \
+                           ctx->d_mask, ntile, tiles, nch, xcd, ridx, ctx->ray_h, partial);    \
+      else                                                                                     \
+        hipLaunchKernelGGL((k_geval_rays<PP, CC, false>), grid, dim3(kT), 0, st, gh, n, gch,   \
+                           ctx->d_gpool, ctx->d_gx, ctx->d_gy, ctx->d_gz, G, ctx->d_tb,        \
+                           ctx->d_mask, ntile, tiles, nch, xcd, ridx, ctx->ray_h, partial);    \
+    }                                                                                          \
   } else if (mixed)                                                                            \
     hipLaunchKernelGGL((k_geval<PP, CC, true>), grid, dim3(kT), 0, st, gh, n, gch,             \
                        ctx->d_gpool, ctx->d_gpool32, ctx->d_gx, ctx->d_gy, ctx->d_gz, G,       \

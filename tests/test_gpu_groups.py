@@ -275,3 +275,34 @@ def test_grouped_records_compact_on_first_use(dsys):
         iv = b.evaluate_prepared(capi.INCA_FP64)
         er, ea = _close(iv, res[0])
         assert er < 1e-12 and ea < 1e-14
+
+
+def test_group_rays_on_cube_lines(dsys):
+    """Cube grids by groups (the per-axis contraction off): eight points of
+    one z line per lane, exp(-s) by the recurrence and the polynomial's
+    later values from its difference table, against the groups point by
+    point (P = 1) and against the oracle."""
+    s, ijkl, d = dsys
+    with capi.Engine(0) as e:
+        p = _setup(e, s, ijkl, d)
+        e.grid_cube([0.0, 0.0, 0.0], [0.2, 0.25, 0.15], [12, 10, 33])
+        e.set_cube_contraction(0)
+        try:
+            out = {}
+            for P in (1, 8):
+                e.set_tiles(P, 4)
+                iv, _ = e.evaluate(capi.INCA_FP64, THRESH)
+                assert e.groups()[0] > 0
+                out[P] = iv.copy()
+            g = e.get_grid()
+            ref, _ = O.intracule(p, ijkl, d, g["r"], THRESH)
+        finally:
+            e.set_tiles(4, 4)
+            e.set_cube_contraction(2)
+        er, ea = _close(out[8], out[1])
+        print("cube lines, P=8 rays vs point by point: rel %.2e abs %.2e" % (er, ea))
+        assert not np.array_equal(out[8], out[1])  # the ray arithmetic ran
+        assert er < 1e-11 and ea < 1e-13
+        er, ea = _close(out[8], ref)
+        print("cube lines by groups vs oracle: rel %.2e abs %.2e" % (er, ea))
+        assert er < 1e-10 and ea < 1e-13
