@@ -167,7 +167,7 @@ k_geval_rays(const double *__restrict__ gh, int64_t ng,
   const int64_t q0 = task.chunk * gchunk;
   const int64_t q1 = empty ? q0 : (q0 + gchunk < ng ? q0 + gchunk : ng);
   for_live(CUT ? mask : nullptr, ntile, tile, q0, q1, [&](int64_t gi) {
-    group_ray<P>(gh + gi * kGStride, pool, o, hw, span, h * h, acc, s_exp2);
+    group_ray<P, !ZLINE>(gh + gi * kGStride, pool, o, hw, span, h * h, acc, s_exp2);
   });
 #pragma unroll
   for (int k = 0; k < P; ++k) {

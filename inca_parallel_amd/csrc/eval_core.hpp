@@ -736,7 +736,7 @@ __device__ __forceinline__ void ray_accumulate(bool rec, double &e, double &qk, 
 // q_{k+1} = q_k rho, rho = exp(-2 sqe^2 h^2) in header word 7), under the
 // same range condition (sqe (P-1) h + rcut <= sqrt(690)); groups beyond it
 // take exp(-s_k) point by point at the same ideal points.
-template <int P>
+template <int P, bool DIFF = true>
 __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
                                           const double *__restrict__ pool, const double (&o)[3],
                                           const double (&hw)[3], double span, double h2,
@@ -765,10 +765,13 @@ __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
     // t = 0..3 by the backward-difference table at t = 3 (three additions
     // per point in place of M fused multiply-adds).  The two halves are
     // separate regions of code, so their polynomials do not interleave.
+    // DIFF false (cube lines): Horner at every point -- a line that passes
+    // a group's centre after points where its polynomial is large
+    // amplifies the table's roundoff (3e-13 of the maximum on a test cube).
     double w[4];
     ray_half<0>(pool, h, p0x, p0y, p0z, dx, dy, dz, w);
     ray_accumulate<0>(rec, e, qk, rho, w, p0x, p0y, p0z, dx, dy, dz, acc, tbl);
-    if (h.Ls <= 3) {
+    if (DIFF && h.Ls <= 3) {
       const double d1 = w[3] - w[2], d1b = w[2] - w[1], d1c = w[1] - w[0];
       const double d2 = d1 - d1b, d2b = d1b - d1c;
       // differences of degree above Ls are zero, not roundoff
@@ -783,7 +786,7 @@ __device__ __forceinline__ void group_ray(const double *__restrict__ gh,
         pv += n1;
         w[k] = pv;
       }
-    } else if (h.Ls <= 5) {
+    } else if (DIFF && h.Ls <= 5) {
       // degrees 4 and 5: the polynomial at t = 4 (and 5) by Horner, the
       // remaining points from the difference table of order Ls at t = Ls
       double v[6] = {w[0], w[1], w[2], w[3], 0.0, 0.0};

@@ -399,6 +399,25 @@ __device__ __forceinline__ int fit_into(const double R[4], double Rik, double Rj
   return bad;
 }
 
+// The fit of an axis of degree L <= NP - 1 at size L + 1 up to 5, padded to
+// NP beyond.
+template <int NP>
+__device__ __forceinline__ int fit_sized(const double R[4], double Rik, double Rjl, double Pijkl,
+                                         const int t[4], int L, double sqe, double invz,
+                                         double alf, double (&C)[NP]) {
+  if constexpr (NP >= 2)
+    if (L == 0) return fit_into<1, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  if constexpr (NP >= 3)
+    if (L == 1) return fit_into<2, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  if constexpr (NP >= 4)
+    if (L == 2) return fit_into<3, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  if constexpr (NP >= 5)
+    if (L == 3) return fit_into<4, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  if constexpr (NP >= 6)
+    if (L == 4) return fit_into<5, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+  return polycoef_pad<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
+}
+
 // One axis of degree L <= NP - 1 through polycoef_pad: second-screen factor,
 // L1 bound and integral factors, and the coefficients written to out[0..L]
 // (fitted polynomials only; the exact-polynomial mode takes polycoef<NPM>).
@@ -412,14 +431,7 @@ __device__ __forceinline__ int axis_pad(const double R[4], double Rik, double Rj
   // pre-pass hands out records sorted by degrees, so the lanes of a wave
   // mostly take one branch
   double C[NP];
-  int bad;
-  switch (L) {
-    case 0: bad = fit_into<1, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
-    case 1: bad = fit_into<2, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
-    case 2: bad = fit_into<3, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
-    case 3: bad = fit_into<4, NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
-    default: bad = polycoef_pad<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C); break;
-  }
+  const int bad = fit_sized<NP>(R, Rik, Rjl, Pijkl, t, L, sqe, invz, alf, C);
   double u = 0.0;
 #pragma unroll
   for (int q = 0; q < NP; ++q)
@@ -560,9 +572,10 @@ __global__ __launch_bounds__(kT) void k_slot_records(const uint32_t *__restrict_
   dst[2] = src[2];
 }
 
-// Fit order: key[s] = Lx | Ly << 4 | Lz << 8 of slot s (record rec + 6
-// rmap[s], or s when rmap is null), val[s] = s, for a stable sort that lets
-// the lanes of a wave fit axes of one degree.
+// Fit order: key[s] = Lx | Ly << 4 | Lz << 8 | max(Lx, Ly, Lz) << 12 of
+// slot s (record rec + 6 rmap[s], or s when rmap is null), val[s] = s, for
+// a stable sort that gives each largest degree its own launch and lets the
+// lanes of a wave fit axes of one degree.
 __global__ __launch_bounds__(kT) void k_slot_class(const int32_t *__restrict__ rec, int64_t n1,
                                                    const int64_t *__restrict__ rmap,
                                                    const int32_t *__restrict__ tmn, int nprim,
@@ -572,15 +585,31 @@ __global__ __launch_bounds__(kT) void k_slot_class(const int32_t *__restrict__ r
   if (s >= n1) return;
   const int32_t *p = rec + 6 * (rmap ? rmap[s] : s);
   const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
-  uint32_t c = 0;
+  uint32_t c = 0, lmax = 0;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     const int32_t *tt = tmn + ax * nprim;
-    const int L = tt[i] + tt[j] + tt[k] + tt[l];
-    c |= (uint32_t)(L < 15 ? L : 15) << (4 * ax);
+    const uint32_t L = (uint32_t)min(tt[i] + tt[j] + tt[k] + tt[l], 15);
+    c |= L << (4 * ax);
+    lmax = max(lmax, L);
   }
-  key[s] = c;
+  key[s] = c | (lmax << 12);
   val[s] = (uint32_t)s;
+}
+
+// bnd[v] = first position of the sorted keys whose largest degree is v
+// (slots of degree v: bnd[v] .. the next present start; bnd filled with n1
+// beforehand).
+__global__ __launch_bounds__(kT) void k_degree_bounds(const uint32_t *__restrict__ key,
+                                                      int64_t n1, int64_t *__restrict__ bnd) {
+  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (j >= n1) return;
+  const uint32_t v = key[j] >> 12;
+  if (j == 0 || (key[j - 1] >> 12) != v) bnd[v] = j;
+}
+
+__global__ void k_fill_i64(int64_t *__restrict__ a, int n, int64_t v) {
+  if ((int)threadIdx.x < n) a[threadIdx.x] = v;
 }
 
 __global__ __launch_bounds__(kT) void k_u32_to_i64(const uint32_t *__restrict__ a, int64_t n,
@@ -1003,16 +1032,19 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   // grouping: the slots are ordered by group key, so the rows of a group
   // come out adjacent (file order within the group)
   const bool sort_groups = groupable(ctx) && chunk <= (int64_t)UINT32_MAX;
-  // the register-resident fits take their slots in degree order
-  const bool sort_class = split && chunk <= (int64_t)UINT32_MAX;
+  // the register-resident fits take their slots in degree order, one launch
+  // per largest degree (bases up to d functions)
+  const bool sort_class = split && la <= 8 && chunk <= (int64_t)UINT32_MAX;
   const int64_t ns = (sort_groups || sort_class) ? chunk : 1;
   const size_t sort_bytes = (sort_groups || sort_class) ? sort_slots_bytes(chunk) : 0;
-  const size_t o_mom = cv.add<double>(2 * nb1 + 1), o_def = cv.add<uint8_t>(chunk),
+  // (one launch per largest degree adds at most 9 partial blocks)
+  const int64_t ntot = 2 * nb1 + 16;  // the chunk's total, after the partials
+  const size_t o_mom = cv.add<double>(ntot + 1), o_def = cv.add<uint8_t>(chunk),
                o_hist = cv.add<unsigned long long>(kMagBins), o_pass = cv.add<uint8_t>(chunk),
                o_dslot = cv.add<int64_t>(deferred ? chunk : 1), o_kin = cv.add<uint32_t>(ns),
                o_kout = cv.add<uint32_t>(ns), o_vin = cv.add<uint32_t>(ns),
                o_vout = cv.add<uint32_t>(ns), o_srec = cv.add<int32_t>(6 * (sort_groups ? ns : 1)),
-               o_fit = cv.add<int64_t>(sort_class ? ns : 1),
+               o_fit = cv.add<int64_t>(sort_class ? ns : 1), o_bnd = cv.add<int64_t>(16),
                o_sort = cv.add<char>((int64_t)sort_bytes);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
   double *d_mom = Carve::at<double>(ctx->d_tmp, o_mom);
@@ -1029,6 +1061,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   uint32_t *d_vout = Carve::at<uint32_t>(ctx->d_tmp, o_vout);
   int32_t *d_srec = Carve::at<int32_t>(ctx->d_tmp, o_srec);
   int64_t *d_fit = Carve::at<int64_t>(ctx->d_tmp, o_fit);
+  int64_t *d_bnd = Carve::at<int64_t>(ctx->d_tmp, o_bnd);
   char *d_sort = Carve::at<char>(ctx->d_tmp, o_sort);
   double mom_total = 0.0;  // chunk totals added in file order
   int64_t q2 = 0;
@@ -1068,66 +1101,96 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       srec = d_srec;
       slot_rec = nullptr;
     }
-    // fit order: the slots stably sorted by their three degrees
-    const int64_t *fit_order = nullptr;
-    if (sort_class && n1 > 0) {
-      hipLaunchKernelGGL(k_slot_class, dim3(nblk(n1)), dim3(kT), 0, st, srec, n1, slot_rec,
-                         ctx->d_tmn, nprim, d_kin, d_vin);
-      INCA_HIP(hipGetLastError());
-      if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1, 12u)))
-        return rc;
-      hipLaunchKernelGGL(k_u32_to_i64, dim3(nblk(n1)), dim3(kT), 0, st, d_vout, n1, d_fit);
-      INCA_HIP(hipGetLastError());
-      fit_order = d_fit;
-    }
-    // the small-degree pass's nblk(n1) partials end at d_mom + nb1, where the
-    // deferred pass's begin, so that one fixed-order sum covers both
-    double *small_mom = d_mom + (nb1 - (int64_t)nblk(n1));
-    if (split && n1 > 0) {
-      hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, srec, n1,
-                         fit_order, slot_rec, nprim,
-                         ctx->d_prim, ctx->d_tmn, ctx->d_pair,
-                         lim, ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
-                         ctx->d_qtmp, ctx->d_flag, d_defer, ctx->d_cnt, small_mom);
-      e = hipGetLastError();
-    }
-    int64_t nrun = split ? 0 : n1;  // records for the scratch kernel
-    if (e == hipSuccess && d_defer && n1 > 0) {
-      rc = compact_flags(ctx, d_defer, n1, d_dslot, &nrun);
-      if (rc) return rc;
-    }
-    const int64_t *run_idx = split ? d_dslot : nullptr;
-    double *run_mom = split ? d_mom + nb1 : d_mom;
-    if (e == hipSuccess && nrun > 0) {
+    int64_t nbt = 0;                // block partials of the term integral,
+    const double *sum_from = d_mom;  // from sum_from in a fixed order
+    if (sort_class) {
+      // fit order: the slots stably sorted by (largest degree, degrees); one
+      // launch per largest degree v, its fits in registers at size v + 1
+      // (9 for v = 5..8), its partials after the previous launch's
+      if (n1 > 0) {
+        hipLaunchKernelGGL(k_slot_class, dim3(nblk(n1)), dim3(kT), 0, st, srec, n1, slot_rec,
+                           ctx->d_tmn, nprim, d_kin, d_vin);
+        INCA_HIP(hipGetLastError());
+        if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1, 16u)))
+          return rc;
+        hipLaunchKernelGGL(k_u32_to_i64, dim3(nblk(n1)), dim3(kT), 0, st, d_vout, n1, d_fit);
+        hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, d_bnd, 16, n1);
+        hipLaunchKernelGGL(k_degree_bounds, dim3(nblk(n1)), dim3(kT), 0, st, d_kout, n1, d_bnd);
+        INCA_HIP(hipGetLastError());
+        int64_t bnd[16];
+        INCA_HIP(hipMemcpyAsync(bnd, d_bnd, sizeof(bnd), hipMemcpyDeviceToHost, st));
+        INCA_HIP(hipStreamSynchronize(st));
+        for (int v = 0; v <= 8 && e == hipSuccess; ++v) {
+          const int64_t b0 = bnd[v];
+          if (b0 >= n1) continue;
+          int64_t b1 = n1;
+          for (int u = v + 1; u < 16; ++u)
+            if (bnd[u] < b1) b1 = bnd[u];
+          const int64_t cnt = b1 - b0;
+          if (cnt <= 0) continue;
+#define INCA_PREP_V(NP)                                                                       \
+  hipLaunchKernelGGL((k_prep<NP, true>), dim3(nblk(cnt)), dim3(kT), 0, st, srec, cnt,          \
+                     (const int64_t *)(d_fit + b0), slot_rec, nprim, ctx->d_prim, ctx->d_tmn,  \
+                     ctx->d_pair, lim, ctx->normalize, stride, 0,                              \
+                     ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,            \
+                     (uint8_t *)nullptr, ctx->d_cnt, d_mom + nbt)
+          switch (v) {
+            case 0: INCA_PREP_V(1); break;
+            case 1: INCA_PREP_V(2); break;
+            case 2: INCA_PREP_V(3); break;
+            case 3: INCA_PREP_V(4); break;
+            case 4: INCA_PREP_V(5); break;
+            default: INCA_PREP_V(9); break;
+          }
+#undef INCA_PREP_V
+          e = hipGetLastError();
+          nbt += (int64_t)nblk(cnt);
+        }
+      }
+    } else {
+      // the small-degree pass's nblk(n1) partials end at d_mom + nb1, where
+      // the deferred pass's begin, so that one fixed-order sum covers both
+      double *small_mom = d_mom + (nb1 - (int64_t)nblk(n1));
+      if (split && n1 > 0) {
+        hipLaunchKernelGGL((k_prep<5, true>), dim3(nblk(n1)), dim3(kT), 0, st, srec, n1,
+                           (const int64_t *)nullptr, slot_rec, nprim, ctx->d_prim, ctx->d_tmn,
+                           ctx->d_pair, lim, ctx->normalize, stride, 0,
+                           ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,
+                           d_defer, ctx->d_cnt, small_mom);
+        e = hipGetLastError();
+      }
+      int64_t nrun = split ? 0 : n1;  // records for the scratch kernel
+      if (e == hipSuccess && d_defer && n1 > 0) {
+        rc = compact_flags(ctx, d_defer, n1, d_dslot, &nrun);
+        if (rc) return rc;
+      }
+      const int64_t *run_idx = split ? d_dslot : nullptr;
+      double *run_mom = split ? d_mom + nb1 : d_mom;
+      if (e == hipSuccess && nrun > 0) {
 #define INCA_PREP(NPM)                                                                        \
   hipLaunchKernelGGL((k_prep<NPM, false>), dim3(nblk(nrun)), dim3(kT), 0, st, srec, nrun,    \
                      run_idx, slot_rec, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,     \
                      ctx->normalize, stride, ctx->poly_exact,                                 \
                      ctx->cut_log2 * 0.69314718055994531, ctx->d_qtmp, ctx->d_flag,           \
                      (uint8_t *)nullptr, ctx->d_cnt, run_mom)
-      if (la <= 4) INCA_PREP(5);
-      else if (la <= 8 && split)  // fitted polynomials up to d: the fits in registers
-        hipLaunchKernelGGL((k_prep<9, true>), dim3(nblk(nrun)), dim3(kT), 0, st, srec, nrun,
-                           run_idx, slot_rec, nprim, ctx->d_prim, ctx->d_tmn, ctx->d_pair, lim,
-                           ctx->normalize, stride, 0, ctx->cut_log2 * 0.69314718055994531,
-                           ctx->d_qtmp, ctx->d_flag, (uint8_t *)nullptr, ctx->d_cnt, run_mom);
-      else if (la <= 8) INCA_PREP(9);
-      else if (la <= 12) INCA_PREP(13);
-      else if (la <= 16) INCA_PREP(17);
-      else INCA_PREP(22);
+        if (la <= 4) INCA_PREP(5);
+        else if (la <= 8) INCA_PREP(9);
+        else if (la <= 12) INCA_PREP(13);
+        else if (la <= 16) INCA_PREP(17);
+        else INCA_PREP(22);
 #undef INCA_PREP
-      e = hipGetLastError();
+        e = hipGetLastError();
+      }
+      nbt = split ? (n1 > 0 ? (int64_t)nblk(n1) : 0) + (nrun > 0 ? (int64_t)nblk(nrun) : 0)
+                  : (nrun > 0 ? (int64_t)nblk(nrun) : 0);
+      sum_from = split ? small_mom : d_mom;
     }
-    const int64_t nbt = split ? (n1 > 0 ? (int64_t)nblk(n1) : 0) +
-                                    (nrun > 0 ? (int64_t)nblk(nrun) : 0)
-                              : (nrun > 0 ? (int64_t)nblk(nrun) : 0);
-    const double *sum_from = split ? small_mom : d_mom;
     double chunk_mom = 0.0;
     if (e == hipSuccess && nbt > 0) {
-      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, sum_from, nbt, d_mom + 2 * nb1);
+      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, sum_from, nbt, d_mom + ntot);
       e = hipGetLastError();
       if (e == hipSuccess)
-        e = hipMemcpyAsync(&chunk_mom, d_mom + 2 * nb1, sizeof(double), hipMemcpyDeviceToHost,
+        e = hipMemcpyAsync(&chunk_mom, d_mom + ntot, sizeof(double), hipMemcpyDeviceToHost,
                            st);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
