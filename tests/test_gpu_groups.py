@@ -306,3 +306,29 @@ def test_group_rays_on_cube_lines(dsys):
         er, ea = _close(out[8], ref)
         print("cube lines by groups vs oracle: rel %.2e abs %.2e" % (er, ea))
         assert er < 1e-10 and ea < 1e-13
+
+
+def test_contexts_prepared_from_threads_match_sequential():
+    """Contexts are independent: several prepared and evaluated from host
+    threads at once give the results of the same contexts run one after the
+    other, bit for bit (each context has its own stream and buffers)."""
+    from concurrent.futures import ThreadPoolExecutor
+    systems = [synth.config_cluster(seed=40 + k, natoms=3, nheavy=2, nocc=3,
+                                    shells_heavy={0: 2, 1: 1, 2: 1}, shells_h={0: 2})
+               for k in range(6)]
+
+    def run(s):
+        ijkl, d = s.dm2_unique()
+        with capi.Engine(0) as e:
+            p = _setup(e, s, ijkl, d)
+            cent, Ps, sf = O.centercalc(p["an"], p["cartes"], True, False)
+            e.grid_multicenter(cent, [30] * len(Ps), [110] * len(Ps), sf, Ps, nosym=True)
+            iv, cnt = e.evaluate(capi.INCA_MIXED, THRESH)
+            return iv.copy(), list(cnt), e.integrate()[0]
+
+    seq = [run(s) for s in systems]
+    with ThreadPoolExecutor(max_workers=6) as pool:
+        par = list(pool.map(run, systems))
+    for a, b in zip(seq, par):
+        assert a[1] == b[1]
+        assert np.array_equal(a[0], b[0]) and a[2] == b[2]
