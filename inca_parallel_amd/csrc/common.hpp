@@ -270,11 +270,13 @@ int split_records(inca_ctx *ctx);
 // INCA_EINVAL when the prepared records were released (inca_release_records)
 int need_records(inca_ctx *ctx);
 // group.hip: groups of all prepared records (build_groups), or built chunk
-// by chunk (reset_groups, group_rows per chunk, finish_groups)
-int build_groups(inca_ctx *ctx, int precision);
+// by chunk (reset_groups, group_rows per chunk, finish_groups); heads, when
+// given, holds the rows' group starts (1 where a row's group words differ
+// from the previous row's), else they are found from the rows
+int build_groups(inca_ctx *ctx, int precision, const uint8_t *heads = nullptr);
 bool groupable(const inca_ctx *ctx);
 void reset_groups(inca_ctx *ctx);
-int group_rows(inca_ctx *ctx, const double *q, int64_t n);
+int group_rows(inca_ctx *ctx, const double *q, int64_t n, const uint8_t *heads = nullptr);
 int finish_groups(inca_ctx *ctx, int precision);
 // the pre-pass's stable radix sort of its slots by group key
 size_t sort_slots_bytes(int64_t n);

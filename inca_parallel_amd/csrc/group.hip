@@ -412,8 +412,9 @@ void reset_groups(inca_ctx *ctx) {
 // (adjacent, file order within a group; rows marked kQDropped failed screen
 // 2 and count for nothing), appended to the context's groups:
 // their headers after the existing ones, their coefficients after the
-// existing pool.
-int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
+// existing pool.  heads: the rows' group starts when the caller has them
+// (null: k_group_heads compares the rows).
+int group_rows(inca_ctx *ctx, const double *q, int64_t n, const uint8_t *heads) {
   if (n == 0) return 0;
   hipStream_t st = ctx->stream;
   const int stride = ctx->qstride;
@@ -436,10 +437,13 @@ int group_rows(inca_ctx *ctx, const double *q, int64_t n) {
   int64_t *d_first = Carve::at<int64_t>(base, o_first);
   int64_t *d_m = Carve::at<int64_t>(base, o_m), *d_off = Carve::at<int64_t>(base, o_off);
   char *d_scan = Carve::at<char>(base, o_scan);
-  hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, q, stride, n, d_head);
-  INCA_HIP(hipGetLastError());
+  if (!heads) {
+    hipLaunchKernelGGL(k_group_heads, dim3(nblk(n)), dim3(kT), 0, st, q, stride, n, d_head);
+    INCA_HIP(hipGetLastError());
+    heads = d_head;
+  }
   int64_t ng = 0;
-  if ((rc = compact_flags(ctx, d_head, n, d_first, &ng))) return rc;
+  if ((rc = compact_flags(ctx, heads, n, d_first, &ng))) return rc;
   const int64_t ng0 = ctx->ng, pool0 = ctx->gpool_n;
   if ((rc = grow_dev(ctx, ctx->d_gh, ctx->gh_cap, (size_t)(ng0 + ng) * kGStride,
                      (size_t)ng0 * kGStride)))
@@ -573,11 +577,12 @@ int finish_groups(inca_ctx *ctx, int precision) {
 // by group within each pre-pass chunk; a group split between chunks becomes
 // two).  Leaves ctx->ng = 0 (the evaluation then runs record by record) when
 // there is nothing to group or primitives beyond d.
-int build_groups(inca_ctx *ctx, int precision) {
+int build_groups(inca_ctx *ctx, int precision, const uint8_t *heads) {
   reset_groups(ctx);
   if (!groupable(ctx) || ctx->q2 == 0) return 0;
   int rc;
-  if ((rc = group_rows(ctx, ctx->d_q, ctx->rows_sparse ? ctx->rows_sparse : ctx->q2))) return rc;
+  if ((rc = group_rows(ctx, ctx->d_q, ctx->rows_sparse ? ctx->rows_sparse : ctx->q2, heads)))
+    return rc;
   return finish_groups(ctx, precision);
 }
 

@@ -528,6 +528,24 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   return x;
 }
 
+// The group words of record p (1-based primitive indices i, j, k, l in
+// words 0-3): sqe and sqe (P_ik - P_jl), by the operations k_prep stores
+// them with, so that equal inputs give the bits of k_prep's row words.
+__device__ __forceinline__ void group_words(const int32_t *p, int nprim,
+                                            const PairEntry *__restrict__ pair, double w[4]) {
+  const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
+  const PairEntry pik = pair[(int64_t)i * nprim + k];
+  const PairEntry pjl = pair[(int64_t)j * nprim + l];
+  const double aik = pik.aik, ajl = pjl.aik;
+  const double zeta = aik + ajl;
+  const double eijkl = (aik * ajl) * (1.0 / zeta);
+  const double sqe = sqrt(eijkl);
+  w[0] = sqe;
+  w[1] = sqe * (pik.P[0] - pjl.P[0]);
+  w[2] = sqe * (pik.P[1] - pjl.P[1]);
+  w[3] = sqe * (pik.P[2] - pjl.P[2]);
+}
+
 // Group keys of the screen-1 survivors s < n1 (record idx[s]): a 32-bit
 // hash of the bits of sqe and sqe (P_ik - P_jl), computed by k_prep's own
 // operations, so that the records of a shell-quartet group (equal words)
@@ -539,16 +557,8 @@ __global__ __launch_bounds__(kT) void k_slot_keys(const int32_t *__restrict__ re
                                                   uint32_t *__restrict__ val) {
   const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (s >= n1) return;
-  const int32_t *p = rec + 6 * idx[s];
-  const int i = p[0] - 1, j = p[1] - 1, k = p[2] - 1, l = p[3] - 1;
-  const PairEntry pik = pair[(int64_t)i * nprim + k];
-  const PairEntry pjl = pair[(int64_t)j * nprim + l];
-  const double aik = pik.aik, ajl = pjl.aik;
-  const double zeta = aik + ajl;
-  const double eijkl = (aik * ajl) * (1.0 / zeta);
-  const double sqe = sqrt(eijkl);
-  const double w[4] = {sqe, sqe * (pik.P[0] - pjl.P[0]), sqe * (pik.P[1] - pjl.P[1]),
-                       sqe * (pik.P[2] - pjl.P[2])};
+  double w[4];
+  group_words(rec + 6 * idx[s], nprim, pair, w);
   uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
   for (int q = 0; q < 4; ++q) h = mix64(h ^ (uint64_t)__double_as_longlong(w[q]));
@@ -570,6 +580,25 @@ __global__ __launch_bounds__(kT) void k_slot_records(const uint32_t *__restrict_
   dst[0] = src[0];
   dst[1] = src[1];
   dst[2] = src[2];
+}
+
+// head[j] = 1 where sorted slot j starts a run of equal group words (slot
+// j - 1 has other words): the group boundaries of the rows k_prep writes
+// for these slots, from the 24-byte sorted records instead of the rows.
+__global__ __launch_bounds__(kT) void k_slot_heads(const int32_t *__restrict__ srec, int64_t n1,
+                                                   int nprim, const PairEntry *__restrict__ pair,
+                                                   uint8_t *__restrict__ head) {
+  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+  if (j >= n1) return;
+  bool h = j == 0;
+  if (!h) {
+    double a[4], b[4];
+    group_words(srec + 6 * j, nprim, pair, a);
+    group_words(srec + 6 * (j - 1), nprim, pair, b);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h = h || __double_as_longlong(a[q]) != __double_as_longlong(b[q]);
+  }
+  head[j] = (uint8_t)h;
 }
 
 // Fit order: key[s] = Lx | Ly << 4 | Lz << 8 | max(Lx, Ly, Lz) << 12 of
@@ -1044,6 +1073,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
                o_dslot = cv.add<int64_t>(deferred ? chunk : 1), o_kin = cv.add<uint32_t>(ns),
                o_kout = cv.add<uint32_t>(ns), o_vin = cv.add<uint32_t>(ns),
                o_vout = cv.add<uint32_t>(ns), o_srec = cv.add<int32_t>(6 * (sort_groups ? ns : 1)),
+               o_head = cv.add<uint8_t>(sort_groups ? ns : 1),
                o_fit = cv.add<int64_t>(sort_class ? ns : 1), o_bnd = cv.add<int64_t>(16),
                o_sort = cv.add<char>((int64_t)sort_bytes);
   if ((rc = ensure(ctx, ctx->d_tmp, ctx->tmp_cap, cv.size))) return rc;
@@ -1060,6 +1090,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   uint32_t *d_vin = Carve::at<uint32_t>(ctx->d_tmp, o_vin);
   uint32_t *d_vout = Carve::at<uint32_t>(ctx->d_tmp, o_vout);
   int32_t *d_srec = Carve::at<int32_t>(ctx->d_tmp, o_srec);
+  uint8_t *d_head = Carve::at<uint8_t>(ctx->d_tmp, o_head);  // group starts of the sorted slots
   int64_t *d_fit = Carve::at<int64_t>(ctx->d_tmp, o_fit);
   int64_t *d_bnd = Carve::at<int64_t>(ctx->d_tmp, o_bnd);
   char *d_sort = Carve::at<char>(ctx->d_tmp, o_sort);
@@ -1097,6 +1128,8 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
       if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1))) return rc;
       hipLaunchKernelGGL(k_slot_records, dim3(nblk(n1)), dim3(kT), 0, st, d_vout,
                          (const int64_t *)ctx->d_idx, n1, recs, d_srec);
+      hipLaunchKernelGGL(k_slot_heads, dim3(nblk(n1)), dim3(kT), 0, st, d_srec, n1, nprim,
+                         ctx->d_pair, d_head);
       INCA_HIP(hipGetLastError());
       srec = d_srec;
       slot_rec = nullptr;
@@ -1204,7 +1237,8 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     if ((rc = compact_flags(ctx, ctx->d_flag, n1, ctx->d_idx, &qc))) return rc;
     if (groups_only) {
       // the chunk's slots grouped as they are (screen-2 failures skipped)
-      if (qc > 0 && (rc = group_rows(ctx, ctx->d_qtmp, n1))) return rc;
+      if (qc > 0 && (rc = group_rows(ctx, ctx->d_qtmp, n1, sort_groups ? d_head : nullptr)))
+        return rc;
     } else if (sparse) {
       std::swap(ctx->d_q, ctx->d_qtmp);
       std::swap(ctx->q_cap, ctx->qtmp_cap);
@@ -1260,7 +1294,9 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
   counts[1] = (int64_t)hcnt[1];
   counts[2] = q2;
   counts[3] = (int64_t)hcnt[3];
-  if ((rc = build_groups(ctx, precision))) return rc;
+  // sparse rows are the single chunk's sorted slots: their group starts
+  // are known
+  if ((rc = build_groups(ctx, precision, ctx->rows_sparse ? d_head : nullptr))) return rc;
   // without groups the evaluation runs record by record: split now
   return ctx->ng > 0 ? 0 : split_records(ctx);
 }
