@@ -17,8 +17,10 @@
 // has already ordered its slots by a 32-bit hash of the bits of (sqe,
 // offset) with a stable radix sort (sort_slots below), so the rows of a
 // group are adjacent and keep their file order:
-//   k_group_heads   a group starts wherever the four doubles change (a hash
-//                   collision only splits a group, never merges two);
+//   group starts    wherever the four doubles change (a hash collision only
+//                   splits a group, never merges two): marked by the
+//                   pre-pass's sorted-record copy, or by k_group_heads
+//                   from the rows when those marks are not at hand;
 //   k_group_header  one wave per group: degrees, bound, cutoff radius;
 //   exclusive scan  of the coefficient counts: the coefficient pool offsets;
 //   k_group_coeffs  one wave per group, lanes over coefficients (and over

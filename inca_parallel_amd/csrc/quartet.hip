@@ -549,7 +549,7 @@ __device__ __forceinline__ void group_words(const int32_t *p, int nprim,
 // Group keys of the screen-1 survivors s < n1 (record idx[s]): a 32-bit
 // hash of the bits of sqe and sqe (P_ik - P_jl), computed by k_prep's own
 // operations, so that the records of a shell-quartet group (equal words)
-// share a key; val[s] = s for the stable sort.
+// share a key; val[s] = idx[s] (increasing in s) for the stable sort.
 __global__ __launch_bounds__(kT) void k_slot_keys(const int32_t *__restrict__ rec, int64_t n1,
                                                   const int64_t *__restrict__ idx, int nprim,
                                                   const PairEntry *__restrict__ pair,
@@ -557,46 +557,57 @@ __global__ __launch_bounds__(kT) void k_slot_keys(const int32_t *__restrict__ re
                                                   uint32_t *__restrict__ val) {
   const int64_t s = (int64_t)blockIdx.x * kT + threadIdx.x;
   if (s >= n1) return;
+  const int64_t r = idx[s];
   double w[4];
-  group_words(rec + 6 * idx[s], nprim, pair, w);
+  group_words(rec + 6 * r, nprim, pair, w);
   uint64_t h = 0x9e3779b97f4a7c15ull;
 #pragma unroll
   for (int q = 0; q < 4; ++q) h = mix64(h ^ (uint64_t)__double_as_longlong(w[q]));
   key[s] = (uint32_t)(h >> 32);
-  val[s] = (uint32_t)s;
+  val[s] = (uint32_t)r;
 }
 
-// Slot j after the sort holds survivor order[j], record idx[order[j]]: its
-// 24 bytes are copied to out[j], so that the fits read their records in
-// slot order.
+// Slot j after the sort holds record order[j]: its 24 bytes are copied to
+// out[j], so that the fits read their records in slot order, and head[j] = 1
+// where slot j starts a run of equal group words (slot j - 1 has other
+// words): the group boundaries of the rows k_prep writes for these slots,
+// found here instead of from the rows.  The previous slot's words come from
+// the neighbouring thread through LDS, or, for a block's first thread, from
+// its own read of that record.
 __global__ __launch_bounds__(kT) void k_slot_records(const uint32_t *__restrict__ order,
-                                                     const int64_t *__restrict__ idx, int64_t n1,
-                                                     const int32_t *__restrict__ rec,
-                                                     int32_t *__restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
-  if (j >= n1) return;
-  const int64_t *src = reinterpret_cast<const int64_t *>(rec + 6 * idx[order[j]]);
-  int64_t *dst = reinterpret_cast<int64_t *>(out + 6 * j);
-  dst[0] = src[0];
-  dst[1] = src[1];
-  dst[2] = src[2];
-}
-
-// head[j] = 1 where sorted slot j starts a run of equal group words (slot
-// j - 1 has other words): the group boundaries of the rows k_prep writes
-// for these slots, from the 24-byte sorted records instead of the rows.
-__global__ __launch_bounds__(kT) void k_slot_heads(const int32_t *__restrict__ srec, int64_t n1,
-                                                   int nprim, const PairEntry *__restrict__ pair,
-                                                   uint8_t *__restrict__ head) {
-  const int64_t j = (int64_t)blockIdx.x * kT + threadIdx.x;
+                                                     int64_t n1, const int32_t *__restrict__ rec,
+                                                     int nprim,
+                                                     const PairEntry *__restrict__ pair,
+                                                     int32_t *__restrict__ out,
+                                                     uint8_t *__restrict__ head) {
+  __shared__ double wl[4][kT];
+  const int t = threadIdx.x;
+  const int64_t j = (int64_t)blockIdx.x * kT + t;
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  if (j < n1) {
+    const int32_t *src = rec + 6 * (int64_t)order[j];
+    const int64_t *s8 = reinterpret_cast<const int64_t *>(src);
+    int64_t *dst = reinterpret_cast<int64_t *>(out + 6 * j);
+    dst[0] = s8[0];
+    dst[1] = s8[1];
+    dst[2] = s8[2];
+    group_words(src, nprim, pair, w);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) wl[q][t] = w[q];
+  __syncthreads();
   if (j >= n1) return;
   bool h = j == 0;
   if (!h) {
-    double a[4], b[4];
-    group_words(srec + 6 * j, nprim, pair, a);
-    group_words(srec + 6 * (j - 1), nprim, pair, b);
+    double b[4];
+    if (t > 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) h = h || __double_as_longlong(a[q]) != __double_as_longlong(b[q]);
+      for (int q = 0; q < 4; ++q) b[q] = wl[q][t - 1];
+    } else {
+      group_words(rec + 6 * (int64_t)order[j - 1], nprim, pair, b);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) h = h || __double_as_longlong(w[q]) != __double_as_longlong(b[q]);
   }
   head[j] = (uint8_t)h;
 }
@@ -1126,10 +1137,8 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
                          (const int64_t *)ctx->d_idx, nprim, ctx->d_pair, d_kin, d_vin);
       INCA_HIP(hipGetLastError());
       if ((rc = sort_slots(ctx, d_sort, sort_bytes, d_kin, d_kout, d_vin, d_vout, n1))) return rc;
-      hipLaunchKernelGGL(k_slot_records, dim3(nblk(n1)), dim3(kT), 0, st, d_vout,
-                         (const int64_t *)ctx->d_idx, n1, recs, d_srec);
-      hipLaunchKernelGGL(k_slot_heads, dim3(nblk(n1)), dim3(kT), 0, st, d_srec, n1, nprim,
-                         ctx->d_pair, d_head);
+      hipLaunchKernelGGL(k_slot_records, dim3(nblk(n1)), dim3(kT), 0, st, d_vout, n1, recs,
+                         nprim, ctx->d_pair, d_srec, d_head);
       INCA_HIP(hipGetLastError());
       srec = d_srec;
       slot_rec = nullptr;
