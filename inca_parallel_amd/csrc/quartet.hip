@@ -813,19 +813,30 @@ __global__ __launch_bounds__(kT) void k_prep(
   }
 }
 
-// Sum of n block partials by one block in a fixed order.
-__global__ __launch_bounds__(kT) void k_sum_fixed(const double *__restrict__ x, int64_t n,
-                                                  double *__restrict__ out) {
-  __shared__ double red[kT];
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kT) s += x[i];
-  red[threadIdx.x] = s;
+// Sum of n block partials by one block of kSumT threads in a fixed order:
+// thread t keeps four running sums, of the partials i = t + kSumT (4 k + u)
+// for u = 0..3 (independent chains, so that several loads are in flight),
+// adds them as (s0 + s1) + (s2 + s3), and a tree over the threads follows.
+constexpr int kSumT = 1024;
+__global__ __launch_bounds__(kSumT) void k_sum_fixed(const double *__restrict__ x, int64_t n,
+                                                     double *__restrict__ out) {
+  __shared__ double red[kSumT];
+  const int t = threadIdx.x;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = t; i < n; i += 4 * (int64_t)kSumT) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = i + (int64_t)u * kSumT;
+      if (k < n) s[u] += x[k];
+    }
+  }
+  red[t] = (s[0] + s[1]) + (s[2] + s[3]);
   __syncthreads();
-  for (int h = kT / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+  for (int h = kSumT / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = red[0];
+  if (t == 0) out[0] = red[0];
 }
 
 __global__ __launch_bounds__(kT) void k_gather_records(const double *__restrict__ src,
@@ -1229,7 +1240,7 @@ int prepare_quartets(inca_ctx *ctx, double thresh, int precision, int64_t *count
     }
     double chunk_mom = 0.0;
     if (e == hipSuccess && nbt > 0) {
-      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kT), 0, st, sum_from, nbt, d_mom + ntot);
+      hipLaunchKernelGGL(k_sum_fixed, dim3(1), dim3(kSumT), 0, st, sum_from, nbt, d_mom + ntot);
       e = hipGetLastError();
       if (e == hipSuccess)
         e = hipMemcpyAsync(&chunk_mom, d_mom + ntot, sizeof(double), hipMemcpyDeviceToHost,
